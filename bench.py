@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Headline benchmark: FP64 NVE of N=262,144 bidisperse disks on one MI355X.
+
+Workload (config 2): 50/50 diameters 1.0/1.4 at packing fraction 0.85 in a periodic
+square box, harmonic contacts (e_c=1, n_c=2, m=1), seeded uniform positions relaxed
+by 200 FIRE steps, Gaussian velocities at T=1e-5, dt=0.01, cell list with
+rc = 1.4 + 0.3 and rebuild when the max displacement exceeds 0.15.
+
+One step = one velocity-Verlet step of the hot path over all N particles
+(kick+drift kernel, force+kick kernel, and any neighbour rebuild the step needs).
+`value` is particle-steps/s over exactly --steps steps with the state resident in
+HBM, bracketed by device synchronisation.
+
+The roofline object covers the force kernel (the dominant one): algorithmic bytes
+per launch = N * (92 + 4*k_mean) (positions, radius, count and 4-byte indices read,
+forces/energy written, the fused closing half-kick's velocity/mass traffic), divided
+by the kernel's mean duration from per-launch HIP events recorded on the system's
+own stream during a second pass of the same --steps steps.
+
+cpu_baseline: the CPU oracle (C++ restatement of the reference, OpenMP) timed on a
+bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "FP64 particle-steps/s at N=262k; batched jams/s & p99 latency vs plain loop"
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_pkg():
+    P = _load("dpcuda_v2_amd", os.path.join(ROOT, "dpcuda-v2_amd", "__init__.py"))
+    W = _load("dpcuda_v2_amd_workloads", os.path.join(ROOT, "dpcuda-v2_amd", "workloads.py"))
+    return P, W
+
+
+def prepare(P, W, n, seed, backend=None, fire_steps=200, temperature=1e-5):
+    st = W.disk_config(n, 0.85, seed=seed)
+    spec = W.neighbor_spec(st.side)
+    s = W.make_disk_system(P.System, st, spec, backend=backend)
+    s.minimize_fire(P.fire_params(max_steps=fire_steps))
+    vx, vy = W.gaussian_velocities(n, temperature, seed)
+    s.upload(P.F_VX, vx)
+    s.upload(P.F_VY, vy)
+    s.zero_forces()
+    s.compute_forces()
+    return s
+
+
+def cpu_baseline(P, W, n, seed, steps):
+    """The oracle on this host's cores (bounded sample of the same workload)."""
+    oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(oracle_so):
+        return None
+    be = P.Backend(oracle_so, "orc_", gpu=False)
+    s = prepare(P, W, n, seed, backend=be)
+    t0 = time.perf_counter()
+    s.nve_run(0.01, steps)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} NVE steps of the same N={n} workload (after the same 200 "
+                      f"FIRE steps), C++ oracle with OpenMP, {threads} threads, "
+                      f"{dt:.2f} s"}
+
+
+def traffic_from_profiles(n):
+    path = os.path.join(ROOT, "profiles", "force_kick_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        if int(d.get("n_particles", -1)) != n:
+            return None
+        return float(d["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--cpu-steps", type=int, default=1500)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--chunk", type=int, default=64)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of "
+              "it (systems shard by independent systems or spatial slabs, see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    P, W = load_pkg()
+    n = args.n
+    print(f"[bench] preparing N={n} (FIRE 200 steps, T=1e-5)", file=sys.stderr, flush=True)
+    s = prepare(P, W, n, args.seed)
+    s.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
+    s.set_option(P.OPT_CHUNK, args.chunk)
+    if args.warmup:
+        s.nve_run(0.01, args.warmup)
+    s.synchronize()
+    r0 = s.neighbor_stats()
+    print(f"[bench] timed region: {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    s.nve_run(0.01, args.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    r1 = s.neighbor_stats()
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+
+    # kernel durations: a second pass of the same length with per-launch HIP events
+    print("[bench] kernel-timing pass", file=sys.stderr, flush=True)
+    s.set_option(P.OPT_TIMING, 1)
+    s.reset_kernel_time()
+    s.nve_run(0.01, args.steps)
+    f_ms, f_n = s.kernel_time(P.K_FORCE)
+    d_ms, d_n = s.kernel_time(P.K_DRIFT)
+    s.set_option(P.OPT_TIMING, 0)
+    stats = s.neighbor_stats()
+    k_mean = stats["total_entries"] / n
+    force_bytes = n * (92.0 + 4.0 * k_mean)
+    drift_bytes = n * 120.0
+    f_avg_s = f_ms / max(f_n, 1) / 1e3
+    d_avg_s = d_ms / max(d_n, 1) / 1e3
+    f_gbs = force_bytes / f_avg_s / 1e9
+    d_gbs = drift_bytes / d_avg_s / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(P, W, n, args.seed, args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform positions, FIRE-relaxed, Gaussian velocities)",
+        "config": {
+            "workload": "config 2: FP64 NVE, N=262144 bidisperse disks (1.0/1.4, 50/50), "
+                        "phi=0.85, T=1e-5, dt=0.01, cell list rc=1.7, rebuild at max "
+                        "displacement > 0.15",
+            "n_particles": n,
+            "k_mean": k_mean,
+            "rebuilds_in_timed_region": r1["n_rebuilds"] - r0["n_rebuilds"],
+            "cell_rebuilds_in_timed_region": r1["n_cell_rebuilds"] - r0["n_cell_rebuilds"],
+            "graph": not args.no_graph,
+            "chunk": args.chunk,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_nve_force_kick",
+            "achieved": f_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": f_gbs / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(n),
+            "algorithmic_bytes_per_launch": force_bytes,
+            "mean_launch_us": f_avg_s * 1e6,
+            "launches": f_n,
+        },
+        "kernels": {
+            "k_nve_kick_drift": {"bytes_per_launch": drift_bytes, "mean_launch_us": d_avg_s * 1e6,
+                                 "achieved_GBps": d_gbs, "frac": d_gbs / HBM_PEAK_GBS},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,402 @@
+"""dpcuda-v2_amd -- MI355X-native particle hot path of dpCUDA-v2 (Python host side).
+
+The compute lives in ``libdpm.so`` (hand-written gfx950 HIP kernels behind the C
+ABI in ``include/dpm.h``).  This module is a thin ctypes layer over that ABI:
+``System`` mirrors the reference's ``Particle`` operations
+(include/particles/base/particle.h:28-717) one call per entry point, moving
+numpy arrays in static particle order.
+
+There is no CPU fallback: if ``libdpm.so`` is missing or no HIP device is
+visible, ``load()`` / ``System()`` raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdpm.so")
+
+DISK, BUMPY = 0, 1
+NBR_NONE, NBR_ALL, NBR_VERLET, NBR_CELL = 0, 1, 2, 3
+
+F_X, F_Y, F_VX, F_VY, F_FX, F_FY, F_RAD, F_MASS, F_PE, F_KE = range(10)
+F_ANGLE, F_OMEGA, F_TORQUE, F_MOI = 10, 11, 12, 13
+F_STATIC_INDEX, F_NUM_NEIGHBORS, F_CONTACTS = 14, 15, 16
+F_STRESS_XX, F_STRESS_XY, F_STRESS_YX, F_STRESS_YY = 17, 18, 19, 20
+F_VERTEX_X, F_VERTEX_Y, F_VERTEX_MASS, F_NUM_VERTICES, F_VERTEX_CONTACTS = 21, 22, 23, 24, 25
+INT_FIELDS = {F_STATIC_INDEX, F_NUM_NEIGHBORS, F_CONTACTS, F_NUM_VERTICES, F_VERTEX_CONTACTS}
+VERTEX_FIELDS = {F_VERTEX_X, F_VERTEX_Y, F_VERTEX_MASS}
+
+(RED_PE, RED_KE, RED_POWER, RED_FMAX_SQ, RED_FORCE_BALANCE, RED_PRESSURE, RED_CONTACTS,
+ RED_TEMPERATURE, RED_PACKING_FRACTION, RED_MAX_NEIGH_DISP_SQ, RED_MAX_CELL_DISP_SQ,
+ RED_STRESS_XX, RED_STRESS_XY, RED_STRESS_YX, RED_STRESS_YY, RED_VERTEX_CONTACTS) = range(16)
+
+OPT_GRAPH, OPT_CHUNK, OPT_TIMING = 0, 1, 2
+K_FORCE, K_DRIFT, K_REBUILD = 0, 1, 2
+
+
+class DpmError(RuntimeError):
+    pass
+
+
+class FireParams(C.Structure):
+    _fields_ = [("alpha_init", C.c_double), ("dt", C.c_double),
+                ("avg_pe_target", C.c_double), ("avg_pe_diff_target", C.c_double),
+                ("fmax_target", C.c_double), ("max_steps", C.c_int64)]
+
+
+class FireResult(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("stopped", C.c_int32), ("converged", C.c_int32),
+                ("avg_pe", C.c_double), ("fmax_sq", C.c_double), ("dt", C.c_double),
+                ("alpha", C.c_double)]
+
+
+class FireState(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("alpha_init", C.c_double), ("dt", C.c_double),
+                ("dt_max", C.c_double), ("dt_min", C.c_double), ("n_good", C.c_int64),
+                ("n_bad", C.c_int64), ("stopped", C.c_int32), ("power", C.c_double)]
+
+
+class JamParams(C.Structure):
+    _fields_ = [("pressure_target", C.c_double), ("window_rel", C.c_double),
+                ("strain_step", C.c_double), ("decompress_factor", C.c_double),
+                ("length_tol", C.c_double), ("min_contacts_per_particle", C.c_double),
+                ("max_compression_steps", C.c_int64), ("fire", FireParams)]
+
+
+class JamResult(C.Structure):
+    _fields_ = [("compression_steps", C.c_int64), ("fire_steps_total", C.c_int64),
+                ("status", C.c_int32), ("pressure", C.c_double),
+                ("packing_fraction", C.c_double), ("pe", C.c_double),
+                ("box_length", C.c_double), ("contacts", C.c_int64)]
+
+
+def fire_params(alpha_init=0.1, dt=1e-2, avg_pe_target=1e-16, avg_pe_diff_target=1e-16,
+                fmax_target=0.0, max_steps=100000):
+    """Defaults of minimizeFire(Particle&) (src/routines/minimization.cu:142-148)."""
+    return FireParams(alpha_init, dt, avg_pe_target, avg_pe_diff_target, fmax_target,
+                      int(max_steps))
+
+
+def jam_params(pressure_target=1e-7, window_rel=1e-2, strain_step=1e-4,
+               decompress_factor=1.01, length_tol=1e-10, min_contacts_per_particle=0.0,
+               max_compression_steps=100000, fire=None):
+    return JamParams(pressure_target, window_rel, strain_step, decompress_factor, length_tol,
+                     min_contacts_per_particle, int(max_compression_steps),
+                     fire if fire is not None else fire_params())
+
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_L = C.c_int64
+_SIGS = {
+    "create": (_P, [_I, _L, _L]),
+    "destroy": (None, [_P]),
+    "last_error": (C.c_char_p, []),
+    "set_box": (_I, [_P, _D, _D]),
+    "get_box": (_I, [_P, C.POINTER(_D), C.POINTER(_D)]),
+    "set_energy": (_I, [_P, _D, _D]),
+    "upload": (_I, [_P, _I, _P, _L]),
+    "download": (_I, [_P, _I, _P, _L]),
+    "neighbor_config": (_I, [_P, _I, _D, _D, _L, _D]),
+    "bumpy_neighbor_config": (_I, [_P, _D, _D]),
+    "neighbor_init": (_I, [_P]),
+    "neighbor_check": (_I, [_P, C.POINTER(_I)]),
+    "neighbor_pairs": (_I, [_P, _P, _P, _L, C.POINTER(_L)]),
+    "vertex_pairs": (_I, [_P, _P, _P, _L, C.POINTER(_L)]),
+    "neighbor_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
+    "zero_forces": (_I, [_P]),
+    "compute_forces": (_I, [_P]),
+    "kick": (_I, [_P, _D]),
+    "drift": (_I, [_P, _D]),
+    "mix": (_I, [_P, _D]),
+    "zero_velocities": (_I, [_P]),
+    "nve_run": (_I, [_P, _D, _L]),
+    "reduce": (_I, [_P, _I, C.POINTER(_D)]),
+    "minimize_fire": (_I, [_P, C.POINTER(FireParams), C.POINTER(FireResult)]),
+    "fire_state_init": (_I, [C.POINTER(FireState), _D, _D]),
+    "fire_step": (_I, [_P, C.POINTER(FireState)]),
+    "adam_step": (_I, [_P, _L, _D, _D, _D, _D]),
+    "minimize_adam": (_I, [_P, _D, _D, _D, _D, _D, _D, _L, C.POINTER(_L)]),
+    "compute_stress": (_I, [_P]),
+    "count_contacts": (_I, [_P]),
+    "scale_positions": (_I, [_P, _D]),
+    "scale_box": (_I, [_P, _D]),
+    "scale_to_packing_fraction": (_I, [_P, _D]),
+    "snapshot_save": (_I, [_P]),
+    "snapshot_restore": (_I, [_P]),
+    "remove_mean_velocity": (_I, [_P]),
+    "scale_to_temperature": (_I, [_P, _D]),
+    "jam": (_I, [_P, C.POINTER(JamParams), C.POINTER(JamResult)]),
+    "bumpy_setup": (_I, [_P, _P, _D, _I]),
+}
+# entry points only the HIP library has
+_GPU_ONLY = {
+    "device_count": (_I, []),
+    "set_option": (_I, [_P, _I, _L]),
+    "kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_L)]),
+    "reset_kernel_time": (_I, [_P]),
+    "synchronize": (_I, [_P]),
+}
+
+
+class Backend:
+    """One C library implementing the dpm_ call set (or the oracle's orc_ mirror)."""
+
+    def __init__(self, path: str, prefix: str, gpu: bool):
+        if not os.path.exists(path):
+            raise DpmError(f"{path} is not built (run __graft_entry__.build())")
+        self.path = path
+        self.prefix = prefix
+        self.gpu = gpu
+        self.lib = C.CDLL(path)
+        sigs = dict(_SIGS)
+        if gpu:
+            sigs.update(_GPU_ONLY)
+        self.fn = {}
+        for name, (res, args) in sigs.items():
+            f = getattr(self.lib, prefix + name)
+            f.restype = res
+            f.argtypes = args
+            self.fn[name] = f
+
+    def err(self) -> str:
+        m = self.fn["last_error"]()
+        return m.decode() if m else ""
+
+    def call(self, name, *args):
+        r = self.fn[name](*args)
+        if r is not None and isinstance(r, int) and r < 0:
+            raise DpmError(f"{self.prefix}{name}: {self.err()}")
+        return r
+
+
+_gpu_backend = None
+
+
+def load(path: str = LIB_PATH) -> Backend:
+    """Load libdpm.so.  Raises DpmError when it is missing (there is no fallback)."""
+    global _gpu_backend
+    if _gpu_backend is None or _gpu_backend.path != path:
+        _gpu_backend = Backend(path, "dpm_", gpu=True)
+    return _gpu_backend
+
+
+def device_count() -> int:
+    return int(load().fn["device_count"]())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class System:
+    """One particle system on one GPU (or, for tests, on the CPU oracle backend)."""
+
+    def __init__(self, kind: int, n_particles: int, n_vertices: int = 0,
+                 backend: Backend | None = None):
+        self.be = backend if backend is not None else load()
+        if self.be.gpu and self.be.fn["device_count"]() <= 0:
+            raise DpmError("no HIP device visible: the HIP path cannot run")
+        self.kind = kind
+        self.n = int(n_particles)
+        self.nv = int(n_vertices)
+        h = self.be.fn["create"](kind, self.n, self.nv)
+        if not h:
+            raise DpmError(f"{self.be.prefix}create: {self.be.err()}")
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.be.fn["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, name, *args):
+        return self.be.call(name, self.h, *args)
+
+    # parameters
+    def set_box(self, lx, ly=None):
+        self._c("set_box", float(lx), float(lx if ly is None else ly))
+
+    def box(self):
+        a, b = C.c_double(), C.c_double()
+        self._c("get_box", C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_energy(self, e_c=1.0, n_c=2.0):
+        self._c("set_energy", float(e_c), float(n_c))
+
+    # data
+    def upload(self, field: int, values):
+        if field == F_NUM_VERTICES:
+            a = np.ascontiguousarray(values, dtype=np.int64)
+        else:
+            a = np.ascontiguousarray(values, dtype=np.float64)
+        self._c("upload", field, _ptr(a), a.size)
+
+    def download(self, field: int) -> np.ndarray:
+        n = self.nv if field in VERTEX_FIELDS else self.n
+        a = np.empty(n, dtype=np.int64 if field in INT_FIELDS else np.float64)
+        self._c("download", field, _ptr(a), n)
+        return a
+
+    def set_state(self, x, y, rad, mass, vx=None, vy=None):
+        self.upload(F_X, x)
+        self.upload(F_Y, y)
+        self.upload(F_RAD, rad)
+        self.upload(F_MASS, mass)
+        self.upload(F_VX, np.zeros(self.n) if vx is None else vx)
+        self.upload(F_VY, np.zeros(self.n) if vy is None else vy)
+
+    # neighbours
+    def neighbor_config(self, method, cutoff=0.0, disp_thresh_sq=0.0, n_cells_dim=0,
+                        cell_disp_thresh_sq=0.0):
+        self._c("neighbor_config", int(method), float(cutoff), float(disp_thresh_sq),
+                int(n_cells_dim), float(cell_disp_thresh_sq))
+
+    def bumpy_neighbor_config(self, vertex_cutoff, vertex_particle_cutoff):
+        self._c("bumpy_neighbor_config", float(vertex_cutoff), float(vertex_particle_cutoff))
+
+    def neighbor_init(self):
+        self._c("neighbor_init")
+
+    def neighbor_check(self) -> int:
+        r = C.c_int(0)
+        self._c("neighbor_check", C.byref(r))
+        return r.value
+
+    def _pairs(self, name, cap):
+        cnt = C.c_int64(0)
+        while True:
+            oi = np.empty(cap, np.int64)
+            oj = np.empty(cap, np.int64)
+            r = self.be.fn[name](self.h, _ptr(oi), _ptr(oj), cap, C.byref(cnt))
+            if r == 0:
+                return oi[: cnt.value], oj[: cnt.value]
+            if cnt.value > cap:
+                cap = int(cnt.value)
+                continue
+            raise DpmError(f"{self.be.prefix}{name}: {self.be.err()}")
+
+    def neighbor_pairs(self):
+        return self._pairs("neighbor_pairs", max(16, 16 * self.n))
+
+    def vertex_pairs(self):
+        return self._pairs("vertex_pairs", max(16, 16 * self.nv))
+
+    def neighbor_stats(self):
+        v = [C.c_int64() for _ in range(4)]
+        self._c("neighbor_stats", *[C.byref(x) for x in v])
+        return dict(max_alloc=v[0].value, total_entries=v[1].value, n_rebuilds=v[2].value,
+                    n_cell_rebuilds=v[3].value)
+
+    # dynamics
+    def zero_forces(self):
+        self._c("zero_forces")
+
+    def compute_forces(self):
+        self._c("compute_forces")
+
+    def kick(self, dt):
+        self._c("kick", float(dt))
+
+    def drift(self, dt):
+        self._c("drift", float(dt))
+
+    def mix(self, alpha):
+        self._c("mix", float(alpha))
+
+    def zero_velocities(self):
+        self._c("zero_velocities")
+
+    def nve_run(self, dt, n_steps):
+        self._c("nve_run", float(dt), int(n_steps))
+
+    def reduce(self, what: int) -> float:
+        r = C.c_double()
+        self._c("reduce", int(what), C.byref(r))
+        return r.value
+
+    def minimize_fire(self, params: FireParams | None = None) -> FireResult:
+        res = FireResult()
+        self._c("minimize_fire", C.byref(params or fire_params()), C.byref(res))
+        return res
+
+    def fire_state(self, alpha_init=0.1, dt=1e-2) -> FireState:
+        st = FireState()
+        self.be.call("fire_state_init", C.byref(st), float(alpha_init), float(dt))
+        return st
+
+    def fire_step(self, st: FireState):
+        self._c("fire_step", C.byref(st))
+
+    def adam_step(self, step, alpha, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._c("adam_step", int(step), float(alpha), float(beta1), float(beta2), float(eps))
+
+    def minimize_adam(self, alpha=1e-4, beta1=0.9, beta2=0.999, eps=1e-8, avg_pe_target=1e-16,
+                      avg_pe_diff_target=1e-16, max_steps=100000) -> int:
+        st = C.c_int64()
+        self._c("minimize_adam", float(alpha), float(beta1), float(beta2), float(eps),
+                float(avg_pe_target), float(avg_pe_diff_target), int(max_steps), C.byref(st))
+        return st.value
+
+    def compute_stress(self):
+        self._c("compute_stress")
+
+    def count_contacts(self):
+        self._c("count_contacts")
+
+    def scale_positions(self, f):
+        self._c("scale_positions", float(f))
+
+    def scale_box(self, f):
+        self._c("scale_box", float(f))
+
+    def scale_to_packing_fraction(self, phi):
+        self._c("scale_to_packing_fraction", float(phi))
+
+    def snapshot_save(self):
+        self._c("snapshot_save")
+
+    def snapshot_restore(self):
+        self._c("snapshot_restore")
+
+    def remove_mean_velocity(self):
+        self._c("remove_mean_velocity")
+
+    def scale_to_temperature(self, t):
+        self._c("scale_to_temperature", float(t))
+
+    def jam(self, params: JamParams | None = None) -> JamResult:
+        res = JamResult()
+        self._c("jam", C.byref(params or jam_params()), C.byref(res))
+        return res
+
+    def bumpy_setup(self, n_vertices_per_particle, vertex_radius, rotation=True):
+        a = np.ascontiguousarray(n_vertices_per_particle, dtype=np.int64)
+        self._c("bumpy_setup", _ptr(a), float(vertex_radius), int(bool(rotation)))
+
+    # GPU-only execution controls
+    def set_option(self, opt, value):
+        self._c("set_option", int(opt), int(value))
+
+    def kernel_time(self, cls):
+        ms, n = C.c_double(), C.c_int64()
+        self._c("kernel_time", int(cls), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def reset_kernel_time(self):
+        self._c("reset_kernel_time")
+
+    def synchronize(self):
+        self._c("synchronize")

@@ -1,0 +1,762 @@
+// dpm_bumpy.hip -- rigid bumpy particles (rings of vertex disks) on gfx950.
+//
+// Layout in HBM (in addition to the per-particle arrays of dpm_core.hip):
+//   vpos      : double2[Nv]  vertex positions, particle-contiguous (vstart[p] .. +nverts[p])
+//   vmass     : double[Nv]
+//   vpart     : int32[Nv]    owning particle of each vertex
+//   vnbr      : int32 ELL column-major vnbr[k*Nv + v]; vnum[Nv]
+//   vfrc,vtq,vpe : per-vertex force/torque/energy of the last evaluation
+//   angle, omega, torque, moi : double[N]
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "dpm_kernels.hpp"
+
+namespace dpm {
+
+int neighbor_build(dpm_system* s);
+int check_and_rebuild(dpm_system* s, int* kind_out);
+int sum_to_host(dpm_system* s, int what, double* out);
+int max_to_host(dpm_system* s, int what, double* out);
+
+namespace {
+
+inline dim3 g1(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK)); }
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("kernel launch: ") + hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+template <class T>
+int dalloc(dpm_system* s, T** p, int64_t n) {
+    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    void* q = nullptr;
+    DPM_HIP(hipMalloc(&q, bytes));
+    DPM_HIP(hipMemset(q, 0, bytes));
+    s->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ kernels
+// kernelCalcRigidBumpyForces1, src/particles/rigid_bumpy/kernels.cu:366-417
+// mode 0: write (arrays known zero), 1: accumulate (+=)
+__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams p, double rv,
+                                                               int mode) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    int nn = d.vnum[v];
+    if (nn == 0) {
+        if (mode == 0) {
+            d.vfrc[v] = make_double2(0.0, 0.0);
+            d.vtq[v] = 0.0;
+            d.vpe[v] = 0.0;
+        }
+        return;
+    }
+    double2 xv = d.vpos[v];
+    double2 c = d.pos[d.vpart[v]];
+    double ax = 0.0, ay = 0.0, e = 0.0;
+    for (int k = 0; k < nn; ++k) {
+        int u = d.vnbr[(int64_t)k * d.Nv + v];
+        double2 xu = d.vpos[u];
+        double fx, fy;
+        double ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+        ax += fx;
+        ay += fy;
+        e += ie;
+    }
+    // calcTorque, include/particles/base/kernels.cuh:237-241
+    double dx = xv.x - c.x, dy = xv.y - c.y;
+    double t = ay * dx - ax * dy;
+    if (mode == 0) {
+        d.vfrc[v] = make_double2(0.0 + ax, 0.0 + ay);
+        d.vtq[v] = 0.0 + t;
+        d.vpe[v] = 0.0 + e;
+    } else {
+        double2 f = d.vfrc[v];
+        f.x += ax;
+        f.y += ay;
+        d.vfrc[v] = f;
+        d.vtq[v] += t;
+        d.vpe[v] += e;
+    }
+}
+
+// kernelCalcRigidBumpyParticleForces1, rigid_bumpy/kernels.cu:419-442 (sequential
+// vertex order, as the reference).  mode 0 write, 1 accumulate.
+__global__ void k_bumpy_particle_reduce(Dev d, int mode) {
+    int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d.N) return;
+    double ax = 0.0, ay = 0.0, at = 0.0, e = 0.0;
+    int st = d.vstart[p], nv = d.nverts[p];
+    for (int k = 0; k < nv; ++k) {
+        double2 f = d.vfrc[st + k];
+        ax += f.x;
+        ay += f.y;
+        at += d.vtq[st + k];
+        e += d.vpe[st + k];
+    }
+    if (mode == 0) {
+        d.frc[p] = make_double2(0.0 + ax, 0.0 + ay);
+        d.torque[p] = 0.0 + at;
+        d.pe[p] = 0.0 + e;
+    } else {
+        double2 f = d.frc[p];
+        f.x += ax;
+        f.y += ay;
+        d.frc[p] = f;
+        d.torque[p] += at;
+        d.pe[p] += e;
+    }
+}
+
+__global__ void k_bumpy_zero_vertex(Dev d) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    d.vfrc[v] = make_double2(0.0, 0.0);
+    d.vtq[v] = 0.0;
+    d.vpe[v] = 0.0;
+}
+
+// kernelUpdateRigidVelocities, rigid_bumpy/kernels.cu:123-150
+__global__ void k_bumpy_kick(Dev d, double dt, int rotation) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double2 f = d.frc[i], v = d.vel[i];
+    double w = d.omega[i];
+    if (rotation) w += d.torque[i] * dt / d.moi[i];
+    else w = 0.0;
+    double m = d.mass[i];
+    v.x += f.x * dt / m;
+    v.y += f.y * dt / m;
+    d.vel[i] = v;
+    d.omega[i] = w;
+}
+
+// kernelUpdateRigidPositions, rigid_bumpy/kernels.cu:21-78
+__global__ __launch_bounds__(BLOCK) void k_bumpy_drift_com(Dev d, double dt) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double nd = 0.0, cd = 0.0;
+    if (i < d.N) {
+        double2 o = d.pos[i], v = d.vel[i];
+        double a = d.angle[i];
+        d.lpos[i] = o;
+        double ddx = v.x * dt, ddy = v.y * dt, dda = d.omega[i] * dt;
+        d.dpos[i] = make_double2(ddx, ddy);
+        d.dth[i] = dda;
+        double nx = o.x + ddx, ny = o.y + ddy;
+        d.pos[i] = make_double2(nx, ny);
+        d.angle[i] = a + dda;
+        double2 ln = d.lref[i], lc = d.cref[i];
+        double dxn = nx - ln.x, dyn = ny - ln.y;
+        nd = dxn * dxn + dyn * dyn;
+        double dxc = nx - lc.x, dyc = ny - lc.y;
+        cd = dxc * dxc + dyc * dyc;
+    }
+    double bn = block_max(nd);
+    double bc = block_max(cd);
+    if (threadIdx.x == 0) {
+        d.pmax_nd[blockIdx.x] = bn;
+        d.pmax_cd[blockIdx.x] = bc;
+    }
+}
+
+// kernelTranslateAndRotateVertices1, rigid_bumpy/kernels.cu:152-201
+__global__ void k_bumpy_drift_vertices(Dev d) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    int p = d.vpart[v];
+    double2 o = d.lpos[p], n = d.pos[p], x = d.vpos[v];
+    double dth = d.dth[p];
+    double rx = x.x - o.x, ry = x.y - o.y;
+    double c = cos(dth), sn = sin(dth);
+    double rxr = rx * c - ry * sn;
+    double ryr = rx * sn + ry * c;
+    d.vpos[v] = make_double2(n.x + rxr, n.y + ryr);
+}
+
+// kernelMixRigidVelocitiesAndForces, rigid_bumpy/kernels.cu:89-121
+__global__ void k_bumpy_mix(Dev d, double alpha) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double2 f = d.frc[i], v = d.vel[i];
+    double fn = sqrt(f.x * f.x + f.y * f.y);
+    double vn = sqrt(v.x * v.x + v.y * v.y);
+    double mr = 0.0;
+    if (fn > 1e-16) {
+        mr = vn / fn * alpha;
+    } else {
+        v.x = 0.0;
+        v.y = 0.0;
+    }
+    double w = d.omega[i], t = d.torque[i];
+    double tn = fabs(t), wn = fabs(w), tmr = 0.0;
+    if (tn > 1e-16) {
+        tmr = wn / tn * alpha;
+    } else {
+        w = 0.0;
+        tmr = 0.0;
+    }
+    d.vel[i] = make_double2(v.x * (1 - alpha) + f.x * mr, v.y * (1 - alpha) + f.y * mr);
+    d.omega[i] = w * (1 - alpha) + t * tmr;
+}
+
+__device__ __forceinline__ bool within_div2(double2 a, double2 b, double c2, double Lx, double Ly) {
+    double dx = pbc_div(a.x, b.x, Lx);
+    double dy = pbc_div(a.y, b.y, Ly);
+    return dx * dx + dy * dy < c2;
+}
+
+// kernelUpdateVertexNeighborList, rigid_bumpy/kernels.cu:870-922
+__global__ void k_bumpy_vertex_build(Dev d, double c2, double p2, double Lx, double Ly,
+                                     int64_t cap) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    double2 xv = d.vpos[v];
+    int p = d.vpart[v];
+    int nn = d.nnum[p];
+    int cnt = 0;
+    for (int k = 0; k < nn; ++k) {
+        int q = d.nbr[(int64_t)k * d.N + p];
+        if (within_div2(xv, d.pos[q], p2, Lx, Ly)) {
+            int f0 = d.vstart[q], nv = d.nverts[q];
+            for (int u = f0; u < f0 + nv; ++u) {
+                if (within_div2(xv, d.vpos[u], c2, Lx, Ly)) {
+                    if (cnt < cap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
+                    cnt++;
+                }
+            }
+        }
+    }
+    d.vnum[v] = cnt;
+    atomicMax(&d.ctl->max_vnum, cnt);
+}
+
+// exclusive scan of nverts (new order) into vstart for the vertex reorder
+__global__ __launch_bounds__(1024) void k_scan_counts(const int* cnt, int* start, int64_t n) {
+    __shared__ int sums[1024];
+    int t = threadIdx.x;
+    int64_t per = (n + 1023) / 1024;
+    int64_t lo = t * per, hi = min(lo + per, n);
+    int loc = 0;
+    for (int64_t c = lo; c < hi; ++c) loc += cnt[c];
+    sums[t] = loc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int base = sums[t] - loc;
+    for (int64_t c = lo; c < hi; ++c) {
+        start[c] = base;
+        base += cnt[c];
+    }
+}
+
+// kernelReorderRigidBumpyVertexData, rigid_bumpy/kernels.cu:844-868
+__global__ void k_bumpy_reorder_vertices(Dev d, Spare sp) {
+    int64_t ov = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (ov >= d.Nv) return;
+    int op = d.vpart[ov];
+    int np = sp.old_to_new[op];
+    int rel = (int)ov - d.vstart[op];
+    int nv = sp.vstart[np] + rel;
+    sp.vpart[nv] = np;
+    sp.vpos[nv] = d.vpos[ov];
+    sp.vmass[nv] = d.vmass[ov];
+    sp.svidx[nv] = d.svidx[ov];
+}
+
+__global__ void k_bumpy_vertex_copyback(Dev d, Spare sp) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v < d.Nv) {
+        d.vpart[v] = sp.vpart[v];
+        d.vpos[v] = sp.vpos[v];
+        d.vmass[v] = sp.vmass[v];
+        d.svidx[v] = sp.svidx[v];
+    }
+    if (v < d.N) d.vstart[v] = sp.vstart[v];
+}
+
+// kernelCalcRigidBumpyStressTensor, rigid_bumpy/kernels.cu:705-789
+__global__ void k_bumpy_stress(Dev d, PairParams p, double rv, double area) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double axx = 0.0, axy = 0.0, ayx = 0.0, ayy = 0.0;
+    double2 c = d.pos[i], vel = d.vel[i];
+    double w = d.omega[i];
+    int st = d.vstart[i], nv = d.nverts[i];
+    for (int a = 0; a < nv; ++a) {
+        int v = st + a;
+        double2 xv = d.vpos[v];
+        double vm = d.vmass[v];
+        double rx = xv.x - c.x, ry = xv.y - c.y;
+        double prad = sqrt(rx * rx + ry * ry);
+        double vvx = vel.x, vvy = vel.y;
+        if (w != 0.0) {
+            double ang = atan2(ry, rx);
+            vvx -= sin(ang) * w * prad;
+            vvy += cos(ang) * w * prad;
+        }
+        axx += vm * vvx * vvx;
+        axy += vm * vvx * vvy;
+        ayx += vm * vvy * vvx;
+        ayy += vm * vvy * vvy;
+        int nn = d.vnum[v];
+        for (int k = 0; k < nn; ++k) {
+            int u = d.vnbr[(int64_t)k * d.Nv + v];
+            if (d.vpart[u] == i) continue;
+            double2 xu = d.vpos[u];
+            double xd = pbc_mul(xv.x, xu.x, p.Lx, p.invLx);
+            double yd = pbc_mul(xv.y, xu.y, p.Ly, p.invLy);
+            double fx, fy;
+            pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+            axx += xd * fx / 2;
+            axy += xd * fy / 2;
+            ayx += yd * fx / 2;
+            ayy += yd * fy / 2;
+        }
+    }
+    d.sxx[i] = axx / area;
+    d.sxy[i] = axy / area;
+    d.syx[i] = ayx / area;
+    d.syy[i] = ayy / area;
+}
+
+// kernelCountRigidBumpyContacts, rigid_bumpy/kernels.cu:924-997
+__global__ void k_bumpy_contacts(Dev d, PairParams p, double rv) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double vrs = 2 * rv;
+    int nc = 0, nvc = 0;
+    int nn = d.nnum[i];
+    int st = d.vstart[i], nv = d.nverts[i];
+    for (int k = 0; k < nn; ++k) {
+        int q = d.nbr[(int64_t)k * d.N + i];
+        bool is_contact = false;
+        for (int a = 0; a < nv; ++a) {
+            int v = st + a;
+            double2 xv = d.vpos[v];
+            int vn = d.vnum[v];
+            for (int k2 = 0; k2 < vn; ++k2) {
+                int u = d.vnbr[(int64_t)k2 * d.Nv + v];
+                if (d.vpart[u] != q) continue;
+                double2 xu = d.vpos[u];
+                double dx = pbc_mul(xv.x, xu.x, p.Lx, p.invLx);
+                double dy = pbc_mul(xv.y, xu.y, p.Ly, p.invLy);
+                double dist = sqrt(dx * dx + dy * dy);
+                if (dist < vrs) {
+                    if (!is_contact) nc++;
+                    nvc++;
+                    is_contact = true;
+                }
+            }
+            if (is_contact) break;
+        }
+    }
+    d.contacts[i] = nc;
+    d.vcontacts[i] = nvc;
+}
+
+// kernelInitializeVerticesOnParticles (rigid_bumpy/kernels.cu:1137-1162),
+// kernelGetVertexMasses (:1164-1169), kernelGetMomentsOfInertia (:1171-1189)
+__global__ void k_bumpy_init_vertices(Dev d, double rv) {
+    int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d.N) return;
+    double a = d.angle[p], r = d.rad[p];
+    int nv = d.nverts[p], st = d.vstart[p];
+    double inc = 2 * M_PI / nv;
+    double inner = r - rv;
+    double2 c = d.pos[p];
+    for (int i = 0; i < nv; ++i) {
+        double va = i * inc + a;
+        d.vpos[st + i] = make_double2(c.x + inner * cos(va), c.y + inner * sin(va));
+        d.vpart[st + i] = (int)p;
+    }
+    double vm = d.mass[p] / static_cast<double>(nv);
+    double I = 0.0;
+    for (int i = 0; i < nv; ++i) {
+        d.vmass[st + i] = vm;
+        double rx = d.vpos[st + i].x - c.x, ry = d.vpos[st + i].y - c.y;
+        I += vm * (rx * rx + ry * ry);
+    }
+    d.moi[p] = I;
+}
+
+// ------------------------------------------------------------------ host side
+int bumpy_alloc(dpm_system* s) {
+    Dev& d = s->d;
+    int64_t n = s->N, v = s->Nv;
+    if (dalloc(s, &d.angle, n) || dalloc(s, &d.omega, n) || dalloc(s, &d.torque, n) ||
+        dalloc(s, &d.moi, n) || dalloc(s, &d.vpos, v) || dalloc(s, &d.vmass, v) ||
+        dalloc(s, &d.vpart, v) || dalloc(s, &d.vstart, n) || dalloc(s, &d.nverts, n) ||
+        dalloc(s, &d.svidx, v) || dalloc(s, &d.vnum, v) || dalloc(s, &d.vfrc, v) ||
+        dalloc(s, &d.vtq, v) || dalloc(s, &d.vpe, v) || dalloc(s, &d.lpos, n) ||
+        dalloc(s, &d.dpos, n) || dalloc(s, &d.dth, n))
+        return -1;
+    Spare& sp = s->sp;
+    if (dalloc(s, &sp.angle, n) || dalloc(s, &sp.omega, n) || dalloc(s, &sp.torque, n) ||
+        dalloc(s, &sp.moi, n) || dalloc(s, &sp.nverts, n) || dalloc(s, &sp.vstart, n) ||
+        dalloc(s, &sp.old_to_new, n) || dalloc(s, &sp.vpos, v) || dalloc(s, &sp.vmass, v) ||
+        dalloc(s, &sp.vpart, v) || dalloc(s, &sp.svidx, v))
+        return -1;
+    std::vector<int> id(v);
+    for (int64_t k = 0; k < v; ++k) id[k] = (int)k;
+    DPM_HIP(hipMemcpy(d.svidx, id.data(), sizeof(int) * v, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int ensure_vell(dpm_system* s, int64_t cap) {
+    if (cap <= s->vphys_cap) return 0;
+    if (s->d.vnbr) {
+        auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->d.vnbr);
+        if (it != s->allocs.end()) s->allocs.erase(it);
+        (void)hipFree(s->d.vnbr);
+        s->d.vnbr = nullptr;
+    }
+    if (dalloc(s, &s->d.vnbr, s->Nv * cap)) return -1;
+    s->vphys_cap = cap;
+    s->gen++;
+    return 0;
+}
+
+static int64_t pow2_at_least(int64_t m) {
+    return (int64_t)std::pow(2.0, std::ceil(std::log2((double)m)));
+}
+
+// RigidBumpy::updateVertexVerletList, rigid_bumpy.cu:1198-1212
+int bumpy_vertex_build(dpm_system* s) {
+    if (s->vphys_cap == 0 && ensure_vell(s, 16)) return -1;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        DPM_HIP(hipMemsetAsync(&s->d.ctl->max_vnum, 0, sizeof(int), s->stream));
+        hipLaunchKernelGGL(k_bumpy_vertex_build, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
+                           s->rc_v * s->rc_v, s->pcut_v * s->pcut_v, s->box[0], s->box[1],
+                           s->vphys_cap);
+        if (check_launch()) return -1;
+        int mx = 0;
+        DPM_HIP(hipMemcpyAsync(&mx, &s->d.ctl->max_vnum, sizeof(int), hipMemcpyDeviceToHost,
+                               s->stream));
+        DPM_HIP(hipStreamSynchronize(s->stream));
+        if (mx > s->vmax_alloc) s->vmax_alloc = pow2_at_least(mx);
+        if (mx > s->vphys_cap) {
+            if (ensure_vell(s, std::max<int64_t>(s->vmax_alloc, pow2_at_least(mx)))) return -1;
+            continue;
+        }
+        return 0;
+    }
+    set_error("bumpy_vertex_build: capacity did not converge");
+    return -1;
+}
+
+int bumpy_cell_reorder_vertices(dpm_system* s) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s->stream, s->sp.nverts,
+                       s->sp.vstart, s->N);
+    hipLaunchKernelGGL(k_bumpy_reorder_vertices, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
+                       s->sp);
+    hipLaunchKernelGGL(k_bumpy_vertex_copyback, g1(std::max(s->Nv, s->N)), dim3(BLOCK), 0,
+                       s->stream, s->d, s->sp);
+    return check_launch();
+}
+
+// mode -1: zero the vertex arrays; 1: accumulate (Particle::calculateForces semantics)
+int bumpy_forces(dpm_system* s, int mode) {
+    if (mode < 0) {
+        hipLaunchKernelGGL(k_bumpy_zero_vertex, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
+        return check_launch();
+    }
+    PairParams p = pair_params(s, s->e_c, s->n_c);
+    hipLaunchKernelGGL(k_bumpy_vertex_forces, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p,
+                       s->rv, 1);
+    hipLaunchKernelGGL(k_bumpy_particle_reduce, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, 1);
+    return check_launch();
+}
+
+int bumpy_kick(dpm_system* s, double dt) {
+    hipLaunchKernelGGL(k_bumpy_kick, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt,
+                       (int)s->rotation);
+    return check_launch();
+}
+
+int bumpy_drift(dpm_system* s, double dt) {
+    hipLaunchKernelGGL(k_bumpy_drift_com, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
+    hipLaunchKernelGGL(k_bumpy_drift_vertices, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
+    return check_launch();
+}
+
+int bumpy_mix(dpm_system* s, double alpha) {
+    hipLaunchKernelGGL(k_bumpy_mix, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha);
+    return check_launch();
+}
+
+int bumpy_stress(dpm_system* s) {
+    PairParams p = pair_params(s, s->e_c, s->n_c);
+    hipLaunchKernelGGL(k_bumpy_stress, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
+                       s->box[0] * s->box[1]);
+    return check_launch();
+}
+
+int bumpy_contacts(dpm_system* s) {
+    PairParams p = pair_params(s, s->e_c, s->n_c);
+    hipLaunchKernelGGL(k_bumpy_contacts, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv);
+    return check_launch();
+}
+
+// NVE::step for rigid bodies, one step at a time (src/integrator/nve.cu:15-25)
+int bumpy_nve_run(dpm_system* s, double dt, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) {
+        if (bumpy_kick(s, 0.5 * dt) || bumpy_drift(s, dt) || check_and_rebuild(s, nullptr) ||
+            dpm_zero_forces(s) || bumpy_forces(s, 1) || bumpy_kick(s, 0.5 * dt))
+            return -1;
+    }
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int bumpy_fire_step(dpm_system* s, dpm_fire_state* st) { return dpm_fire_step(s, st); }
+
+// minimizeFire (src/routines/minimization.cu:75-112) driven from the host.
+int bumpy_minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
+    dpm_fire_state st;
+    dpm_fire_state_init(&st, prm->alpha_init, prm->dt);
+    int64_t step = 0;
+    double dof = (double)s->N;
+    if (dpm_zero_velocities(s) || dpm_zero_forces(s) || check_and_rebuild(s, nullptr) ||
+        dpm_compute_forces(s))
+        return -1;
+    double pe;
+    if (sum_to_host(s, S_PE, &pe)) return -1;
+    double last = pe / dof, avg = last;
+    int converged = 0;
+    while (step < prm->max_steps) {
+        if (dpm_fire_step(s, &st)) return -1;
+        if (sum_to_host(s, S_PE, &pe)) return -1;
+        avg = pe / dof;
+        double diff = std::abs(avg / last - 1);
+        last = avg;
+        bool fok = false;
+        if (prm->fmax_target > 0) {
+            double f2;
+            if (dpm_reduce(s, DPM_RED_FMAX_SQ, &f2)) return -1;
+            fok = f2 < prm->fmax_target * prm->fmax_target;
+        }
+        if (diff < prm->avg_pe_diff_target || avg < prm->avg_pe_target || st.stopped || fok) {
+            converged = st.stopped ? 0 : 1;
+            break;
+        }
+        step++;
+    }
+    if (dpm_zero_velocities(s)) return -1;
+    if (out) {
+        out->steps = std::min<int64_t>(step + 1, prm->max_steps);
+        out->stopped = st.stopped;
+        out->converged = converged;
+        out->avg_pe = avg;
+        double f2;
+        if (dpm_reduce(s, DPM_RED_FMAX_SQ, &f2)) return -1;
+        out->fmax_sq = f2;
+        out->dt = st.dt;
+        out->alpha = st.alpha;
+    }
+    return 0;
+}
+
+int bumpy_snapshot(dpm_system* s, bool save) {
+    auto cp = [&](void* a, void* b, size_t n) {
+        return save ? hipMemcpyAsync(a, b, n, hipMemcpyDeviceToDevice, s->stream)
+                    : hipMemcpyAsync(b, a, n, hipMemcpyDeviceToDevice, s->stream);
+    };
+    int64_t n = s->N, v = s->Nv;
+    DPM_HIP(cp(s->snap.angle, s->d.angle, sizeof(double) * n));
+    DPM_HIP(cp(s->snap.omega, s->d.omega, sizeof(double) * n));
+    DPM_HIP(cp(s->snap.torque, s->d.torque, sizeof(double) * n));
+    DPM_HIP(cp(s->snap.moi, s->d.moi, sizeof(double) * n));
+    DPM_HIP(cp(s->snap.nverts, s->d.nverts, sizeof(int) * n));
+    DPM_HIP(cp(s->snap.vstart, s->d.vstart, sizeof(int) * n));
+    DPM_HIP(cp(s->snap.vpos, s->d.vpos, sizeof(double2) * v));
+    DPM_HIP(cp(s->snap.vmass, s->d.vmass, sizeof(double) * v));
+    DPM_HIP(cp(s->snap.vpart, s->d.vpart, sizeof(int) * v));
+    DPM_HIP(cp(s->snap.svidx, s->d.svidx, sizeof(int) * v));
+    return 0;
+}
+
+static int fetch_svidx(dpm_system* s, std::vector<int>& sv) {
+    sv.resize(s->Nv);
+    DPM_HIP(hipMemcpyAsync(sv.data(), s->d.svidx, sizeof(int) * s->Nv, hipMemcpyDeviceToHost,
+                           s->stream));
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+static double* pfield(dpm_system* s, int f) {
+    switch (f) {
+        case DPM_F_ANGLE: return s->d.angle;
+        case DPM_F_OMEGA: return s->d.omega;
+        case DPM_F_TORQUE: return s->d.torque;
+        case DPM_F_MOI: return s->d.moi;
+    }
+    return nullptr;
+}
+
+int bumpy_upload(dpm_system* s, int f, const void* data, int64_t count,
+                 const std::vector<int>& sidx) {
+    if (double* p = pfield(s, f)) {
+        if (count != s->N) { set_error("dpm_upload: count mismatch"); return -1; }
+        const double* src = (const double*)data;
+        std::vector<double> h(s->N);
+        for (int64_t k = 0; k < s->N; ++k) h[k] = src[sidx[k]];
+        DPM_HIP(hipMemcpy(p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice));
+        return 0;
+    }
+    if (f == DPM_F_VERTEX_X || f == DPM_F_VERTEX_Y || f == DPM_F_VERTEX_MASS) {
+        if (count != s->Nv) { set_error("dpm_upload: count mismatch"); return -1; }
+        std::vector<int> sv;
+        if (fetch_svidx(s, sv)) return -1;
+        const double* src = (const double*)data;
+        if (f == DPM_F_VERTEX_MASS) {
+            std::vector<double> h(s->Nv);
+            for (int64_t k = 0; k < s->Nv; ++k) h[k] = src[sv[k]];
+            DPM_HIP(hipMemcpy(s->d.vmass, h.data(), sizeof(double) * s->Nv, hipMemcpyHostToDevice));
+            return 0;
+        }
+        std::vector<double2> h(s->Nv);
+        DPM_HIP(hipMemcpy(h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < s->Nv; ++k) (f == DPM_F_VERTEX_X ? h[k].x : h[k].y) = src[sv[k]];
+        DPM_HIP(hipMemcpy(s->d.vpos, h.data(), sizeof(double2) * s->Nv, hipMemcpyHostToDevice));
+        return 0;
+    }
+    set_error("dpm_upload: field not uploadable");
+    return -1;
+}
+
+int bumpy_download(dpm_system* s, int f, void* data, int64_t count,
+                   const std::vector<int>& sidx) {
+    if (double* p = pfield(s, f)) {
+        if (count != s->N) { set_error("dpm_download: count mismatch"); return -1; }
+        std::vector<double> h(s->N);
+        DPM_HIP(hipMemcpy(h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost));
+        double* dst = (double*)data;
+        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
+        return 0;
+    }
+    if (f == DPM_F_NUM_VERTICES || f == DPM_F_VERTEX_CONTACTS) {
+        if (count != s->N) { set_error("dpm_download: count mismatch"); return -1; }
+        int* src = f == DPM_F_NUM_VERTICES ? s->d.nverts : s->d.vcontacts;
+        if (!src) { set_error("dpm_download: field not computed"); return -1; }
+        std::vector<int> h(s->N);
+        DPM_HIP(hipMemcpy(h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+        int64_t* dst = (int64_t*)data;
+        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
+        return 0;
+    }
+    if (f == DPM_F_VERTEX_X || f == DPM_F_VERTEX_Y || f == DPM_F_VERTEX_MASS) {
+        if (count != s->Nv) { set_error("dpm_download: count mismatch"); return -1; }
+        std::vector<int> sv;
+        if (fetch_svidx(s, sv)) return -1;
+        double* dst = (double*)data;
+        if (f == DPM_F_VERTEX_MASS) {
+            std::vector<double> h(s->Nv);
+            DPM_HIP(hipMemcpy(h.data(), s->d.vmass, sizeof(double) * s->Nv, hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < s->Nv; ++k) dst[sv[k]] = h[k];
+            return 0;
+        }
+        std::vector<double2> h(s->Nv);
+        DPM_HIP(hipMemcpy(h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < s->Nv; ++k) dst[sv[k]] = f == DPM_F_VERTEX_X ? h[k].x : h[k].y;
+        return 0;
+    }
+    set_error("dpm_download: unknown field");
+    return -1;
+}
+
+}  // namespace dpm
+
+using namespace dpm;
+
+extern "C" {
+
+int dpm_bumpy_neighbor_config(dpm_system* s, double vc, double vpc) {
+    if (s->kind != DPM_BUMPY) {
+        set_error("dpm_bumpy_neighbor_config: not a bumpy system");
+        return -1;
+    }
+    s->rc_v = vc;
+    s->pcut_v = vpc;
+    return 0;
+}
+
+int dpm_vertex_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int64_t* count) {
+    if (s->kind != DPM_BUMPY || !s->nbr_ready) {
+        set_error("dpm_vertex_pairs: no vertex list");
+        return -1;
+    }
+    std::vector<int> sv, nn(s->Nv), nb((size_t)(s->Nv * s->vphys_cap));
+    if (fetch_svidx(s, sv)) return -1;
+    DPM_HIP(hipMemcpy(nn.data(), s->d.vnum, sizeof(int) * s->Nv, hipMemcpyDeviceToHost));
+    DPM_HIP(hipMemcpy(nb.data(), s->d.vnbr, sizeof(int) * s->Nv * s->vphys_cap,
+                      hipMemcpyDeviceToHost));
+    int64_t c = 0;
+    for (int64_t v = 0; v < s->Nv; ++v)
+        for (int64_t k = 0; k < nn[v] && k < s->vphys_cap; ++k) {
+            int u = nb[k * s->Nv + v];
+            if (c < cap) {
+                oi[c] = sv[v];
+                oj[c] = sv[u];
+            }
+            c++;
+        }
+    *count = c;
+    if (c > cap) {
+        set_error("dpm_vertex_pairs: capacity too small");
+        return -1;
+    }
+    return 0;
+}
+
+int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) {
+    if (s->kind != DPM_BUMPY) {
+        set_error("dpm_bumpy_setup: not a bumpy system");
+        return -1;
+    }
+    std::vector<int> sidx(s->N);
+    DPM_HIP(hipMemcpy(sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+    std::vector<int> nv(s->N), st(s->N);
+    int64_t acc = 0;
+    for (int64_t k = 0; k < s->N; ++k) {
+        nv[k] = (int)nvp[sidx[k]];
+        if (nv[k] <= 0) {
+            set_error("dpm_bumpy_setup: every particle needs at least one vertex");
+            return -1;
+        }
+        st[k] = (int)acc;
+        acc += nv[k];
+    }
+    if (acc != s->Nv) {
+        set_error("dpm_bumpy_setup: vertex counts do not sum to n_vertices");
+        return -1;
+    }
+    DPM_HIP(hipMemcpy(s->d.nverts, nv.data(), sizeof(int) * s->N, hipMemcpyHostToDevice));
+    DPM_HIP(hipMemcpy(s->d.vstart, st.data(), sizeof(int) * s->N, hipMemcpyHostToDevice));
+    std::vector<int> id(s->Nv);
+    for (int64_t k = 0; k < s->Nv; ++k) id[k] = (int)k;
+    DPM_HIP(hipMemcpy(s->d.svidx, id.data(), sizeof(int) * s->Nv, hipMemcpyHostToDevice));
+    s->rv = rv;
+    s->rotation = rotation != 0;
+    hipLaunchKernelGGL(k_bumpy_init_vertices, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, rv);
+    if (check_launch()) return -1;
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    s->gen++;
+    return 0;
+}
+
+}  // extern "C"

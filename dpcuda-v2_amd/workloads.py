@@ -1,0 +1,154 @@
+"""Synthetic inputs for the benchmark configurations (numpy only, no device work).
+
+Bidispersity follows Particle::setBiDispersity (src/particles/base/particle.cu:926-944):
+the first floor(N * count_ratio) particles are large (diameter size_ratio), the rest
+have diameter 1.  The box side for a packing fraction follows
+Particle::scaleToPackingFraction (particle.cu:1037-1045): L = sqrt(pi * sum r^2 / phi).
+Positions come from numpy's seeded generator (the reference's Thrust stream is not
+needed: parity runs feed the same arrays to the GPU and to the checker).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+PI = 3.141592653589793238462643383279502884197
+
+
+def bidisperse_radii(n: int, size_ratio: float = 1.4, count_ratio: float = 0.5) -> np.ndarray:
+    n_large = int(n * count_ratio)
+    r = np.full(n, 0.5)
+    r[:n_large] = size_ratio / 2.0
+    return r
+
+
+def box_side(radii: np.ndarray, phi: float) -> float:
+    return math.sqrt(float(np.sum(radii * radii)) * PI / phi)
+
+
+def uniform_positions(n: int, side: float, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, side, n), rng.uniform(0.0, side, n)
+
+
+def gaussian_velocities(n: int, temperature: float, seed: int, n_dim: int = 2):
+    """Normal draws with mean removed and rescaled to exactly T
+    (Particle::setRandomVelocities, particle.cu:908-912), done on the host."""
+    rng = np.random.default_rng(seed + 7919)
+    vx = rng.normal(0.0, math.sqrt(temperature), n)
+    vy = rng.normal(0.0, math.sqrt(temperature), n)
+    vx -= vx.mean()
+    vy -= vy.mean()
+    t = (0.5 * (vx * vx + vy * vy)).sum() * 2.0 / (n_dim * n)
+    f = math.sqrt(temperature / t)
+    return vx * f, vy * f
+
+
+@dataclass
+class NeighborSpec:
+    """Neighbour-list numbers for "cell size = max cutoff + skin" and "rebuild when
+    max displacement > skin/2" (the reference triggers on 3*max d^2 > thr^2,
+    particle.cu:1141-1161)."""
+    method: int
+    cutoff: float
+    disp_thresh_sq: float
+    n_cells_dim: int
+    cell_disp_thresh_sq: float
+
+
+def neighbor_spec(side: float, max_diam: float = 1.4, skin: float = 0.3,
+                  method: int = 3) -> NeighborSpec:
+    rc = max_diam + skin
+    half_skin = skin / 2.0
+    thr_sq = 3.0 * half_skin * half_skin
+    ncd = max(3, int(math.floor(side / rc)))
+    cs = side / ncd
+    cell_thr_sq = (0.5 * cs) ** 2
+    return NeighborSpec(method, rc, thr_sq, ncd, cell_thr_sq)
+
+
+def configure(sys_, spec: NeighborSpec):
+    sys_.neighbor_config(spec.method, spec.cutoff, spec.disp_thresh_sq, spec.n_cells_dim,
+                         spec.cell_disp_thresh_sq)
+
+
+@dataclass
+class DiskState:
+    n: int
+    side: float
+    x: np.ndarray
+    y: np.ndarray
+    rad: np.ndarray
+    mass: np.ndarray
+
+
+def disk_config(n: int, phi: float, seed: int = 12345) -> DiskState:
+    rad = bidisperse_radii(n)
+    side = box_side(rad, phi)
+    x, y = uniform_positions(n, side, seed)
+    return DiskState(n, side, x, y, rad, np.ones(n))
+
+
+def make_disk_system(sys_cls, st: DiskState, spec: NeighborSpec | None = None, backend=None,
+                     e_c: float = 1.0, n_c: float = 2.0):
+    """Create, fill and neighbour-initialise a disk system on `backend`."""
+    s = sys_cls(0, st.n, 0, backend=backend)
+    s.set_box(st.side, st.side)
+    s.set_energy(e_c, n_c)
+    s.set_state(st.x, st.y, st.rad, st.mass)
+    configure(s, spec or neighbor_spec(st.side))
+    s.neighbor_init()
+    return s
+
+
+# rigid bumpy geometry: RigidBumpy::calculateVertexRadius (rigid_bumpy.cu:570-579)
+def vertex_radius(d_min: float = 1.0, n_small: int = 20, seg: float = 1.0) -> float:
+    ang = 2 * math.pi / n_small
+    return d_min / (1 + seg / math.sin(ang / 2)) / 2.0
+
+
+@dataclass
+class BumpyState:
+    n: int
+    nv: int
+    side: float
+    x: np.ndarray
+    y: np.ndarray
+    angle: np.ndarray
+    rad: np.ndarray
+    mass: np.ndarray
+    nverts: np.ndarray
+    rv: float
+
+
+def bumpy_config(n: int, phi: float, seed: int = 12345, n_small: int = 20,
+                 n_large: int = 28) -> BumpyState:
+    rad = bidisperse_radii(n)
+    nverts = np.where(rad == 0.7, n_large, n_small).astype(np.int64)
+    side = box_side(rad, phi)
+    x, y = uniform_positions(n, side, seed)
+    rng = np.random.default_rng(seed + 1)
+    angle = rng.uniform(0.0, 2 * math.pi, n)
+    return BumpyState(n, int(nverts.sum()), side, x, y, angle, rad, np.ones(n), nverts,
+                      vertex_radius(1.0, n_small, 1.0))
+
+
+def make_bumpy_system(sys_cls, st: BumpyState, backend=None, e_c: float = 1.0,
+                      rotation: bool = True, method: int = 3):
+    s = sys_cls(1, st.n, st.nv, backend=backend)
+    s.set_box(st.side, st.side)
+    s.set_energy(e_c, 2.0)
+    s.set_state(st.x, st.y, st.rad, st.mass)
+    s.upload(10, st.angle)
+    s.bumpy_setup(st.nverts, st.rv, rotation)
+    # RigidBumpy::setNeighborSize (rigid_bumpy.cu:1278-1300) with multiplier 1.5 /
+    # displacement multiplier 0.2 and segment ratio 1 (standard_configs.h:58-65)
+    rc = 1.5 * 1.4
+    spec = neighbor_spec(st.side, 1.4, rc - 1.4, method)
+    thr_sq = (0.2 * 1.0 * st.rv) ** 2
+    s.neighbor_config(method, rc, thr_sq, spec.n_cells_dim, spec.cell_disp_thresh_sq)
+    s.bumpy_neighbor_config(1.5 * 2.0 * st.rv, 1.4)
+    s.neighbor_init()
+    return s

@@ -1,0 +1,206 @@
+/*
+ * dpm.h -- C ABI of the MI355X-native dpCUDA-v2 particle hot path.
+ *
+ * One opaque handle per particle system (disk or rigid bumpy).  All state is
+ * device resident (HBM); every call enqueues on the handle's own HIP stream.
+ * Status returns: 0 = OK, negative = error (text from dpm_last_error()).
+ * No exceptions cross this boundary and no torch types appear in it.
+ *
+ * The reference has no FFI: its boundary is the C++ virtual interface of
+ * class Particle (include/particles/base/particle.h:28-717) and its
+ * subclasses Disk (include/particles/disk/disk.h:18-80) and RigidBumpy
+ * (include/particles/rigid_bumpy/rigid_bumpy.h:19-294), driven by the
+ * integrators (src/integrator/) and routines (src/routines/).  Each
+ * entry point below names the reference member(s) it replaces.  The C++
+ * classes in dpcuda-v2_amd/host/ re-expose that interface on top of this ABI.
+ *
+ * Per-particle values cross the ABI in static (original upload) order; the
+ * cell-list permutation stays internal, as the reference's
+ * static_particle_index does (particle.h:136).
+ */
+#ifndef DPM_H
+#define DPM_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dpm_system dpm_system;
+
+enum { DPM_DISK = 0, DPM_BUMPY = 1 };
+enum { DPM_NBR_NONE = 0, DPM_NBR_ALL = 1, DPM_NBR_VERLET = 2, DPM_NBR_CELL = 3 };
+enum {
+    DPM_F_X = 0, DPM_F_Y = 1, DPM_F_VX = 2, DPM_F_VY = 3, DPM_F_FX = 4, DPM_F_FY = 5,
+    DPM_F_RAD = 6, DPM_F_MASS = 7, DPM_F_PE = 8, DPM_F_KE = 9,
+    DPM_F_ANGLE = 10, DPM_F_OMEGA = 11, DPM_F_TORQUE = 12, DPM_F_MOI = 13,
+    DPM_F_STATIC_INDEX = 14,   /* int64 */
+    DPM_F_NUM_NEIGHBORS = 15,  /* int64 */
+    DPM_F_CONTACTS = 16,       /* int64 */
+    DPM_F_STRESS_XX = 17, DPM_F_STRESS_XY = 18, DPM_F_STRESS_YX = 19, DPM_F_STRESS_YY = 20,
+    DPM_F_VERTEX_X = 21, DPM_F_VERTEX_Y = 22, DPM_F_VERTEX_MASS = 23,
+    DPM_F_NUM_VERTICES = 24,   /* int64, per particle */
+    DPM_F_VERTEX_CONTACTS = 25 /* int64, per particle */
+};
+
+/* ---- lifecycle ------------------------------------------------------------
+ * Replaces Particle::setParticleCounts/initDynamicVariables
+ * (src/particles/base/particle.cu:335-341, :394-403) and the RigidBumpy
+ * vertex-array set-up (src/particles/rigid_bumpy/rigid_bumpy.cu:138-303). */
+dpm_system* dpm_create(int kind, int64_t n_particles, int64_t n_vertices);
+void dpm_destroy(dpm_system* s);
+const char* dpm_last_error(void);
+int dpm_device_count(void);
+
+/* ---- parameters -------------------------------------------------------------
+ * Particle::setBoxSize/getBoxSize (particle.cu:694-709),
+ * Particle::setEnergyScale/setExponent (particle.cu:729-839). */
+int dpm_set_box(dpm_system* s, double lx, double ly);
+int dpm_get_box(dpm_system* s, double* lx, double* ly);
+int dpm_set_energy(dpm_system* s, double e_c, double n_c);
+
+/* ---- data transfer (host <-> HBM, static order) -----------------------------
+ * Replaces the Data1D/Data2D setData/getData round trips
+ * (src/data/data_1d.cu, src/data/data_2d.cu) used by Particle::tryLoadArrayData
+ * (particle.cu:102-149) and Particle::getArrayData (particle.cu:485-692). */
+int dpm_upload(dpm_system* s, int field, const void* data, int64_t count);
+int dpm_download(dpm_system* s, int field, void* data, int64_t count);
+
+/* ---- neighbours ----------------------------------------------------------------
+ * dpm_neighbor_config: Particle::setNeighborMethod/setNeighborSize/setCellSize
+ *   (particle.cu:237-265, :1223-1272), taking the final numbers as
+ *   scripts/runShortNVE.cpp:43-54 assigns them.
+ * dpm_neighbor_init:   Particle::initNeighborList (particle.cu:1163-1221).
+ * dpm_neighbor_check:  Particle::checkForNeighborUpdate (particle.cu:1137-1161).
+ * dpm_neighbor_pairs:  the (i,j) pairs of the list in static ids (parity tests). */
+int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double disp_thresh_sq,
+                        int64_t n_cells_dim, double cell_disp_thresh_sq);
+int dpm_bumpy_neighbor_config(dpm_system* s, double vertex_cutoff, double vertex_particle_cutoff);
+int dpm_neighbor_init(dpm_system* s);
+int dpm_neighbor_check(dpm_system* s, int* rebuilt);
+int dpm_neighbor_pairs(dpm_system* s, int64_t* out_i, int64_t* out_j, int64_t capacity,
+                       int64_t* count);
+int dpm_vertex_pairs(dpm_system* s, int64_t* out_i, int64_t* out_j, int64_t capacity,
+                     int64_t* count);
+int dpm_neighbor_stats(dpm_system* s, int64_t* max_alloc, int64_t* total_entries,
+                       int64_t* n_rebuilds, int64_t* n_cell_rebuilds);
+
+/* ---- forces and integration ------------------------------------------------------
+ * zero/compute: Particle::zeroForceAndPotentialEnergy (particle.cu:1334-1336),
+ *   Disk::calculateForces (disk.cu:218-221), RigidBumpy::calculateForces
+ *   (rigid_bumpy.cu:1088-1105).
+ * kick/drift/mix: Particle::updateVelocities/updatePositions (particle.cu:1086-1092),
+ *   Particle::mixVelocitiesAndForces (particle.cu:895-897), rigid overrides
+ *   (rigid_bumpy.cu:1107-1128, :1271-1275); setVelocitiesToZero (particle.cu:871-873).
+ * dpm_nve_run: n_steps of NVE::step (src/integrator/nve.cu:15-25) as looped by
+ *   scripts/runNVE.cpp:43-47, run device-resident. */
+int dpm_zero_forces(dpm_system* s);
+int dpm_compute_forces(dpm_system* s);
+int dpm_kick(dpm_system* s, double dt);
+int dpm_drift(dpm_system* s, double dt);
+int dpm_mix(dpm_system* s, double alpha);
+int dpm_zero_velocities(dpm_system* s);
+int dpm_nve_run(dpm_system* s, double dt, int64_t n_steps);
+
+/* ---- reductions ----------------------------------------------------------------------
+ * Particle::totalPotentialEnergy/totalKineticEnergy/calculateTemperature/getPowerFire/
+ * getForceBalance/getPressure/getContactCount/getPackingFraction/
+ * getMaxSquared{Neighbor,Cell}Displacement (particle.cu:231-235, :993-1015,
+ * :1074-1100, :1338-1341, :1367-1385). */
+enum { DPM_RED_PE = 0, DPM_RED_KE = 1, DPM_RED_POWER = 2, DPM_RED_FMAX_SQ = 3,
+       DPM_RED_FORCE_BALANCE = 4, DPM_RED_PRESSURE = 5, DPM_RED_CONTACTS = 6,
+       DPM_RED_TEMPERATURE = 7, DPM_RED_PACKING_FRACTION = 8, DPM_RED_MAX_NEIGH_DISP_SQ = 9,
+       DPM_RED_MAX_CELL_DISP_SQ = 10, DPM_RED_STRESS_XX = 11, DPM_RED_STRESS_XY = 12,
+       DPM_RED_STRESS_YX = 13, DPM_RED_STRESS_YY = 14, DPM_RED_VERTEX_CONTACTS = 15 };
+int dpm_reduce(dpm_system* s, int what, double* out);
+
+/* ---- FIRE ---------------------------------------------------------------------------
+ * dpm_minimize_fire: minimizeFire (src/routines/minimization.cu:75-148) with
+ *   Fire::step (src/integrator/fire.cu:13-46) run as a device-resident state
+ *   machine; fmax_target > 0 adds the max-force stop criterion.
+ * dpm_fire_step: one Fire::step with the state held by the caller. */
+typedef struct {
+    double alpha_init, dt;
+    double avg_pe_target, avg_pe_diff_target;
+    double fmax_target;
+    int64_t max_steps;
+} dpm_fire_params;
+typedef struct {
+    int64_t steps;
+    int32_t stopped, converged;
+    double avg_pe, fmax_sq, dt, alpha;
+} dpm_fire_result;
+typedef struct {
+    double alpha, alpha_init, dt, dt_max, dt_min;
+    int64_t n_good, n_bad;
+    int32_t stopped;
+    double power;
+} dpm_fire_state;
+int dpm_minimize_fire(dpm_system* s, const dpm_fire_params* p, dpm_fire_result* out);
+int dpm_fire_state_init(dpm_fire_state* st, double alpha_init, double dt);
+int dpm_fire_step(dpm_system* s, dpm_fire_state* st);
+
+/* ---- Adam -------------------------------------------------------------------------------
+ * Adam::minimize (src/integrator/adam.cu:11-18) + Particle::updatePositionsAdam
+ * (particle.cu:329-333); minimizeAdam (src/routines/minimization.cu:3-33). */
+int dpm_adam_step(dpm_system* s, int64_t step, double alpha, double beta1, double beta2,
+                  double eps);
+int dpm_minimize_adam(dpm_system* s, double alpha, double beta1, double beta2, double eps,
+                      double avg_pe_target, double avg_pe_diff_target, int64_t max_steps,
+                      int64_t* steps_out);
+
+/* ---- analytics, box and state -------------------------------------------------------------
+ * calculateStressTensor (disk.cu:281-285, rigid_bumpy.cu:1513-1519), countContacts
+ * (disk.cu:250-255, rigid_bumpy.cu:1505-1511), scalePositionsFull/scaleBox/
+ * scaleToPackingFractionFull (particle.cu:1047-1072, rigid_bumpy.cu:1044-1048),
+ * setLastState/revertToLastState (particle.cu:956-991, rigid_bumpy.cu:1144-1179),
+ * removeMeanVelocities/scaleVelocitiesToTemperature (particle.cu:865-906). */
+int dpm_compute_stress(dpm_system* s);
+int dpm_count_contacts(dpm_system* s);
+int dpm_scale_positions(dpm_system* s, double factor);
+int dpm_scale_box(dpm_system* s, double factor);
+int dpm_scale_to_packing_fraction(dpm_system* s, double phi);
+int dpm_snapshot_save(dpm_system* s);
+int dpm_snapshot_restore(dpm_system* s);
+int dpm_remove_mean_velocity(dpm_system* s);
+int dpm_scale_to_temperature(dpm_system* s, double temperature);
+
+/* ---- jamming ----------------------------------------------------------------------------
+ * The compress/minimise pressure bisection of scripts/runArthurJamWithFire.cpp:36-106
+ * (minimizeFire + getPressure + countContacts + scaleBox), with the reference's
+ * file round trip replaced by a device snapshot. */
+typedef struct {
+    double pressure_target, window_rel, strain_step, decompress_factor, length_tol;
+    double min_contacts_per_particle;
+    int64_t max_compression_steps;
+    dpm_fire_params fire;
+} dpm_jam_params;
+typedef struct {
+    int64_t compression_steps, fire_steps_total;
+    int32_t status;
+    double pressure, packing_fraction, pe, box_length;
+    int64_t contacts;
+} dpm_jam_result;
+int dpm_jam(dpm_system* s, const dpm_jam_params* p, dpm_jam_result* out);
+
+/* ---- rigid bumpy set-up --------------------------------------------------------------------
+ * RigidBumpy::setParticleStartIndex/setVertexParticleIndex/setVertexPositions/setMass/
+ * setDegreesOfFreedom (rigid_bumpy.cu:476-478, :562-568, :783-787, :1068-1081, :618-624). */
+int dpm_bumpy_setup(dpm_system* s, const int64_t* n_vertices_per_particle, double vertex_radius,
+                    int rotation);
+
+/* ---- execution options and measurement (no reference counterpart) -------------------------
+ * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
+ * DPM_OPT_CHUNK: steps per host poll of the device status (default 64).
+ * DPM_OPT_TIMING: record HIP events around every launch of the hot kernels (default 0);
+ *   dpm_kernel_time returns the summed milliseconds and launch count per kernel class. */
+enum { DPM_OPT_GRAPH = 0, DPM_OPT_CHUNK = 1, DPM_OPT_TIMING = 2 };
+enum { DPM_K_FORCE = 0, DPM_K_DRIFT = 1, DPM_K_REBUILD = 2 };
+int dpm_set_option(dpm_system* s, int option, int64_t value);
+int dpm_kernel_time(dpm_system* s, int kernel_class, double* total_ms, int64_t* launches);
+int dpm_reset_kernel_time(dpm_system* s);
+int dpm_synchronize(dpm_system* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
