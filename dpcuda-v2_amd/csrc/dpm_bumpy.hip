@@ -24,6 +24,12 @@ namespace {
 
 inline dim3 g1(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK)); }
 
+int scopy(dpm_system* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    DPM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s->stream));
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 int check_launch() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -53,7 +59,7 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams
                                                                int mode) {
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= d.Nv) return;
-    int nn = d.vnum[v];
+    int nn = min(d.vnum[v], d.vcap);
     if (nn == 0) {
         if (mode == 0) {
             d.vfrc[v] = make_double2(0.0, 0.0);
@@ -223,7 +229,7 @@ __global__ void k_bumpy_vertex_build(Dev d, double c2, double p2, double Lx, dou
     if (v >= d.Nv) return;
     double2 xv = d.vpos[v];
     int p = d.vpart[v];
-    int nn = d.nnum[p];
+    int nn = min(d.nnum[p], d.cap);
     int cnt = 0;
     for (int k = 0; k < nn; ++k) {
         int q = d.nbr[(int64_t)k * d.N + p];
@@ -313,7 +319,7 @@ __global__ void k_bumpy_stress(Dev d, PairParams p, double rv, double area) {
         axy += vm * vvx * vvy;
         ayx += vm * vvy * vvx;
         ayy += vm * vvy * vvy;
-        int nn = d.vnum[v];
+        int nn = min(d.vnum[v], d.vcap);
         for (int k = 0; k < nn; ++k) {
             int u = d.vnbr[(int64_t)k * d.Nv + v];
             if (d.vpart[u] == i) continue;
@@ -340,7 +346,7 @@ __global__ void k_bumpy_contacts(Dev d, PairParams p, double rv) {
     if (i >= d.N) return;
     double vrs = 2 * rv;
     int nc = 0, nvc = 0;
-    int nn = d.nnum[i];
+    int nn = min(d.nnum[i], d.cap);
     int st = d.vstart[i], nv = d.nverts[i];
     for (int k = 0; k < nn; ++k) {
         int q = d.nbr[(int64_t)k * d.N + i];
@@ -348,7 +354,7 @@ __global__ void k_bumpy_contacts(Dev d, PairParams p, double rv) {
         for (int a = 0; a < nv; ++a) {
             int v = st + a;
             double2 xv = d.vpos[v];
-            int vn = d.vnum[v];
+            int vn = min(d.vnum[v], d.vcap);
             for (int k2 = 0; k2 < vn; ++k2) {
                 int u = d.vnbr[(int64_t)k2 * d.Nv + v];
                 if (d.vpart[u] != q) continue;
@@ -413,7 +419,7 @@ int bumpy_alloc(dpm_system* s) {
         return -1;
     std::vector<int> id(v);
     for (int64_t k = 0; k < v; ++k) id[k] = (int)k;
-    DPM_HIP(hipMemcpy(d.svidx, id.data(), sizeof(int) * v, hipMemcpyHostToDevice));
+    if (scopy(s, d.svidx, id.data(), sizeof(int) * v, hipMemcpyHostToDevice)) return -1;
     return 0;
 }
 
@@ -427,6 +433,7 @@ static int ensure_vell(dpm_system* s, int64_t cap) {
     }
     if (dalloc(s, &s->d.vnbr, s->Nv * cap)) return -1;
     s->vphys_cap = cap;
+    s->d.vcap = (int)cap;
     s->gen++;
     return 0;
 }
@@ -615,7 +622,7 @@ int bumpy_upload(dpm_system* s, int f, const void* data, int64_t count,
         const double* src = (const double*)data;
         std::vector<double> h(s->N);
         for (int64_t k = 0; k < s->N; ++k) h[k] = src[sidx[k]];
-        DPM_HIP(hipMemcpy(p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice));
+        if (scopy(s, p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice)) return -1;
         return 0;
     }
     if (f == DPM_F_VERTEX_X || f == DPM_F_VERTEX_Y || f == DPM_F_VERTEX_MASS) {
@@ -626,13 +633,13 @@ int bumpy_upload(dpm_system* s, int f, const void* data, int64_t count,
         if (f == DPM_F_VERTEX_MASS) {
             std::vector<double> h(s->Nv);
             for (int64_t k = 0; k < s->Nv; ++k) h[k] = src[sv[k]];
-            DPM_HIP(hipMemcpy(s->d.vmass, h.data(), sizeof(double) * s->Nv, hipMemcpyHostToDevice));
+            if (scopy(s, s->d.vmass, h.data(), sizeof(double) * s->Nv, hipMemcpyHostToDevice)) return -1;
             return 0;
         }
         std::vector<double2> h(s->Nv);
-        DPM_HIP(hipMemcpy(h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost)) return -1;
         for (int64_t k = 0; k < s->Nv; ++k) (f == DPM_F_VERTEX_X ? h[k].x : h[k].y) = src[sv[k]];
-        DPM_HIP(hipMemcpy(s->d.vpos, h.data(), sizeof(double2) * s->Nv, hipMemcpyHostToDevice));
+        if (scopy(s, s->d.vpos, h.data(), sizeof(double2) * s->Nv, hipMemcpyHostToDevice)) return -1;
         return 0;
     }
     set_error("dpm_upload: field not uploadable");
@@ -644,7 +651,7 @@ int bumpy_download(dpm_system* s, int f, void* data, int64_t count,
     if (double* p = pfield(s, f)) {
         if (count != s->N) { set_error("dpm_download: count mismatch"); return -1; }
         std::vector<double> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost)) return -1;
         double* dst = (double*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
         return 0;
@@ -654,7 +661,7 @@ int bumpy_download(dpm_system* s, int f, void* data, int64_t count,
         int* src = f == DPM_F_NUM_VERTICES ? s->d.nverts : s->d.vcontacts;
         if (!src) { set_error("dpm_download: field not computed"); return -1; }
         std::vector<int> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
         int64_t* dst = (int64_t*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
         return 0;
@@ -666,12 +673,12 @@ int bumpy_download(dpm_system* s, int f, void* data, int64_t count,
         double* dst = (double*)data;
         if (f == DPM_F_VERTEX_MASS) {
             std::vector<double> h(s->Nv);
-            DPM_HIP(hipMemcpy(h.data(), s->d.vmass, sizeof(double) * s->Nv, hipMemcpyDeviceToHost));
+            if (scopy(s, h.data(), s->d.vmass, sizeof(double) * s->Nv, hipMemcpyDeviceToHost)) return -1;
             for (int64_t k = 0; k < s->Nv; ++k) dst[sv[k]] = h[k];
             return 0;
         }
         std::vector<double2> h(s->Nv);
-        DPM_HIP(hipMemcpy(h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost)) return -1;
         for (int64_t k = 0; k < s->Nv; ++k) dst[sv[k]] = f == DPM_F_VERTEX_X ? h[k].x : h[k].y;
         return 0;
     }
@@ -702,7 +709,7 @@ int dpm_vertex_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int64
     }
     std::vector<int> sv, nn(s->Nv), nb((size_t)(s->Nv * s->vphys_cap));
     if (fetch_svidx(s, sv)) return -1;
-    DPM_HIP(hipMemcpy(nn.data(), s->d.vnum, sizeof(int) * s->Nv, hipMemcpyDeviceToHost));
+    if (scopy(s, nn.data(), s->d.vnum, sizeof(int) * s->Nv, hipMemcpyDeviceToHost)) return -1;
     DPM_HIP(hipMemcpy(nb.data(), s->d.vnbr, sizeof(int) * s->Nv * s->vphys_cap,
                       hipMemcpyDeviceToHost));
     int64_t c = 0;
@@ -729,7 +736,7 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) 
         return -1;
     }
     std::vector<int> sidx(s->N);
-    DPM_HIP(hipMemcpy(sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+    if (scopy(s, sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
     std::vector<int> nv(s->N), st(s->N);
     int64_t acc = 0;
     for (int64_t k = 0; k < s->N; ++k) {
@@ -745,11 +752,11 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) 
         set_error("dpm_bumpy_setup: vertex counts do not sum to n_vertices");
         return -1;
     }
-    DPM_HIP(hipMemcpy(s->d.nverts, nv.data(), sizeof(int) * s->N, hipMemcpyHostToDevice));
-    DPM_HIP(hipMemcpy(s->d.vstart, st.data(), sizeof(int) * s->N, hipMemcpyHostToDevice));
+    if (scopy(s, s->d.nverts, nv.data(), sizeof(int) * s->N, hipMemcpyHostToDevice)) return -1;
+    if (scopy(s, s->d.vstart, st.data(), sizeof(int) * s->N, hipMemcpyHostToDevice)) return -1;
     std::vector<int> id(s->Nv);
     for (int64_t k = 0; k < s->Nv; ++k) id[k] = (int)k;
-    DPM_HIP(hipMemcpy(s->d.svidx, id.data(), sizeof(int) * s->Nv, hipMemcpyHostToDevice));
+    if (scopy(s, s->d.svidx, id.data(), sizeof(int) * s->Nv, hipMemcpyHostToDevice)) return -1;
     s->rv = rv;
     s->rotation = rotation != 0;
     hipLaunchKernelGGL(k_bumpy_init_vertices, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, rv);

@@ -84,7 +84,7 @@ __device__ __forceinline__ void disk_row(const Dev& d, const PairParams& p, int6
                                          double& ay, double& e) {
     double2 xi = d.pos[i];
     double ri = d.rad[i];
-    int nn = d.nnum[i];
+    int nn = min(d.nnum[i], d.cap);
     const int* col = d.nbr + i;
     ax = 0.0;
     ay = 0.0;
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void k_disk_stress(Dev d, PairParams p, doub
     double2 xi = d.pos[i], v = d.vel[i];
     double m = d.mass[i], ri = d.rad[i];
     double axx = m * v.x * v.x, axy = m * v.x * v.y, ayx = axy, ayy = m * v.y * v.y;
-    int nn = d.nnum[i];
+    int nn = min(d.nnum[i], d.cap);
     for (int k = 0; k < nn; ++k) {
         int j = d.nbr[(int64_t)k * d.N + i];
         double2 xj = d.pos[j];
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(BLOCK) void k_disk_contacts(Dev d, PairParams p) {
     double2 xi = d.pos[i];
     double ri = d.rad[i];
     int c = 0;
-    int nn = d.nnum[i];
+    int nn = min(d.nnum[i], d.cap);
     for (int k = 0; k < nn; ++k) {
         int j = d.nbr[(int64_t)k * d.N + i];
         double2 xj = d.pos[j];
@@ -420,6 +420,10 @@ __global__ void k_cell_bin(Dev d, double cs, int64_t ncd) {
     if (i >= d.N) return;
     double2 x = d.pos[i];
     int64_t c = cell_coord(x.x, cs, ncd) + cell_coord(x.y, cs, ncd) * ncd;
+    if (!(c >= 0 && c < ncd * ncd)) {  // non-finite position: keep the index in range, report
+        c = 0;
+        d.ctl->overflow = 1;
+    }
     d.cell_tmp[i] = (int)c;
     atomicAdd(&d.cell_cnt[c], 1);
 }
@@ -874,6 +878,7 @@ static int ensure_ell(dpm_system* s, int64_t cap) {
     s->d.nbr = nullptr;
     if (dalloc(s, &s->d.nbr, s->N * cap)) return -1;
     s->phys_cap = cap;
+    s->d.cap = (int)cap;
     s->gen++;
     return 0;
 }
@@ -897,13 +902,17 @@ int neighbor_build(dpm_system* s) {
         DPM_HIP(hipStreamSynchronize(s->stream));
         if (mx > s->max_alloc) s->max_alloc = pow2_at_least(mx);
         if (mx > s->phys_cap) {
-            if (ensure_ell(s, std::max<int64_t>(s->max_alloc, pow2_at_least(mx)))) return -1;
+            if (ensure_ell(s, std::max<int64_t>(s->max_alloc, pow2_at_least(mx)))) {
+                s->nbr_ready = false;
+                return -1;
+            }
             continue;
         }
         s->n_rebuilds++;
         if (s->kind == DPM_BUMPY) return bumpy_vertex_build(s);
         return 0;
     }
+    s->nbr_ready = false;
     set_error("neighbor_build: capacity did not converge");
     return -1;
 }
@@ -1252,6 +1261,14 @@ using namespace dpm;
 
 namespace {
 
+// Host<->device copies ordered on the system stream (the stream is non-blocking,
+// so the legacy null stream would not wait for kernels queued on it).
+int scopy(dpm_system* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    DPM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s->stream));
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 int fetch_sidx(dpm_system* s, std::vector<int>& sidx) {
     sidx.resize(s->N);
     DPM_HIP(hipMemcpyAsync(sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost,
@@ -1429,15 +1446,15 @@ int dpm_upload(dpm_system* s, int f, const void* data, int64_t count) {
     int comp;
     if (field_d2(s, f, &p2, &comp)) {
         std::vector<double2> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
         for (int64_t k = 0; k < s->N; ++k) (comp ? h[k].y : h[k].x) = src[sidx[k]];
-        DPM_HIP(hipMemcpy(p2, h.data(), sizeof(double2) * s->N, hipMemcpyHostToDevice));
+        if (scopy(s, p2, h.data(), sizeof(double2) * s->N, hipMemcpyHostToDevice)) return -1;
         return 0;
     }
     if (double* p = field_d(s, f)) {
         std::vector<double> h(s->N);
         for (int64_t k = 0; k < s->N; ++k) h[k] = src[sidx[k]];
-        DPM_HIP(hipMemcpy(p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice));
+        if (scopy(s, p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice)) return -1;
         return 0;
     }
     set_error("dpm_upload: field not uploadable");
@@ -1458,14 +1475,14 @@ int dpm_download(dpm_system* s, int f, void* data, int64_t count) {
     int comp;
     if (field_d2(s, f, &p2, &comp)) {
         std::vector<double2> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
         double* dst = (double*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = comp ? h[k].y : h[k].x;
         return 0;
     }
     if (double* p = field_d(s, f)) {
         std::vector<double> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost)) return -1;
         double* dst = (double*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
         return 0;
@@ -1480,7 +1497,7 @@ int dpm_download(dpm_system* s, int f, void* data, int64_t count) {
     }
     if (pi) {
         std::vector<int> h(s->N);
-        DPM_HIP(hipMemcpy(h.data(), pi, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, h.data(), pi, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
         int64_t* dst = (int64_t*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
         return 0;
@@ -1505,7 +1522,7 @@ int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq,
     {
         std::vector<int> id(s->N);
         for (int64_t k = 0; k < s->N; ++k) id[k] = (int)k;
-        DPM_HIP(hipMemcpy(s->d.sidx, id.data(), sizeof(int) * s->N, hipMemcpyHostToDevice));
+        if (scopy(s, s->d.sidx, id.data(), sizeof(int) * s->N, hipMemcpyHostToDevice)) return -1;
     }
     if (method == DPM_NBR_CELL) {
         if (ncd < 3) {
@@ -1566,7 +1583,7 @@ int dpm_neighbor_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int
     if (require_nbr(s, "dpm_neighbor_pairs")) return -1;
     std::vector<int> sidx, nn(s->N), nb((size_t)(s->N * s->phys_cap));
     if (fetch_sidx(s, sidx)) return -1;
-    DPM_HIP(hipMemcpy(nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+    if (scopy(s, nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
     DPM_HIP(hipMemcpy(nb.data(), s->d.nbr, sizeof(int) * s->N * s->phys_cap,
                       hipMemcpyDeviceToHost));
     int64_t c = 0;
@@ -1591,7 +1608,7 @@ int dpm_neighbor_stats(dpm_system* s, int64_t* ma, int64_t* tot, int64_t* nr, in
     if (ma) *ma = s->max_alloc;
     if (tot) {
         std::vector<int> nn(s->N);
-        DPM_HIP(hipMemcpy(nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+        if (scopy(s, nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
         int64_t t = 0;
         for (int v : nn) t += v;
         *tot = t;
@@ -1714,7 +1731,7 @@ int dpm_reduce(dpm_system* s, int what, double* out) {
             if (dpm_count_contacts(s)) return -1;
             std::vector<int> h(s->N);
             int* src = what == DPM_RED_CONTACTS ? s->d.contacts : s->d.vcontacts;
-            DPM_HIP(hipMemcpy(h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost));
+            if (scopy(s, h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
             int64_t t = 0;
             for (int v : h) t += v;
             *out = (double)t;

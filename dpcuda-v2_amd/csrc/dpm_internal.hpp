@@ -54,6 +54,7 @@ struct Dev {
     int *sidx;                  // static id of internal slot k
     // neighbour list, ELL column-major: nbr[k*N + i]
     int *nbr, *nnum;
+    int cap;                    // physical ELL slots per particle
     int *cell_of;               // cell of each particle at the last cell rebuild
     int *cell_start, *cell_cnt, *cell_cur, *cell_tmp, *perm;
     double *pmax_nd, *pmax_cd;  // per-block max displacement partials
@@ -71,6 +72,7 @@ struct Dev {
     double *vmass;
     int *vpart, *vstart, *nverts, *svidx;
     int *vnbr, *vnum;           // vertex ELL: vnbr[k*Nv + v]
+    int vcap;                   // physical vertex ELL slots
     double2 *vfrc;              // per-vertex force (two-kernel path scratch)
     double *vtq, *vpe;
     double2 *lpos;              // COM before the last drift (bumpy rotate)
