@@ -103,7 +103,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (torch) five times; its "
+                         "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(256 * 2**20 // 8, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "launches": 5}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of "
