@@ -6,16 +6,20 @@ square box, harmonic contacts (e_c=1, n_c=2, m=1), seeded uniform positions rela
 by 200 FIRE steps, Gaussian velocities at T=1e-5, dt=0.01, cell list with
 rc = 1.4 + 0.3 and rebuild when the max displacement exceeds 0.15.
 
-One step = one velocity-Verlet step of the hot path over all N particles
-(kick+drift kernel, force+kick kernel, and any neighbour rebuild the step needs).
-`value` is particle-steps/s over exactly --steps steps with the state resident in
-HBM, bracketed by device synchronisation.
+One step = one velocity-Verlet step of the hot path over all N particles (one
+launch of the fused force + kick + kick + drift kernel, plus the gated rebuild
+chain at the end of each chunk of steps, which rebuilds the lists when a step
+needs it).  `value` is particle-steps/s over exactly --steps steps with the state
+resident in HBM, bracketed by device synchronisation.
 
-The roofline object covers the force kernel (the dominant one): algorithmic bytes
-per launch = N * (92 + 4*k_mean) (positions, radius, count and 4-byte indices read,
-forces/energy written, the fused closing half-kick's velocity/mass traffic), divided
-by the kernel's mean duration from per-launch HIP events recorded on the system's
-own stream during a second pass of the same --steps steps.
+The roofline object covers the fused step kernel (the dominant one): algorithmic
+bytes per launch = N * (132 + 4*k_mean): per particle the 32 B {x,y,r,m} record
+read and written (ping-pong), velocity read and written (32 B), both displacement
+references read (32 B), the row length (4 B) and k_mean 4-byte neighbour indices;
+the neighbours' records are gathered from spatially sorted memory and counted
+once, in the particle's own read.  It is divided by the kernel's mean duration
+from per-launch HIP events recorded on the system's own stream during a second
+pass of the same --steps steps.
 
 cpu_baseline: the CPU oracle (C++ restatement of the reference, OpenMP) timed on a
 bounded sample of the same workload on this host.
@@ -80,7 +84,7 @@ def cpu_baseline(P, W, n, seed, steps):
 
 
 def traffic_from_profiles(n):
-    path = os.path.join(ROOT, "profiles", "force_kick_traffic.json")
+    path = os.path.join(ROOT, "profiles", "nve_fused_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -99,7 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--seed", type=int, default=12345)
-    ap.add_argument("--cpu-steps", type=int, default=1500)
+    ap.add_argument("--cpu-steps", type=int, default=6000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=64)
@@ -150,16 +154,14 @@ def main():
     s.reset_kernel_time()
     s.nve_run(0.01, args.steps)
     f_ms, f_n = s.kernel_time(P.K_FORCE)
-    d_ms, d_n = s.kernel_time(P.K_DRIFT)
+    r_ms, r_n = s.kernel_time(P.K_REBUILD)
     s.set_option(P.OPT_TIMING, 0)
     stats = s.neighbor_stats()
     k_mean = stats["total_entries"] / n
-    force_bytes = n * (92.0 + 4.0 * k_mean)
-    drift_bytes = n * 120.0
+    step_bytes = n * (132.0 + 4.0 * k_mean)
     f_avg_s = f_ms / max(f_n, 1) / 1e3
-    d_avg_s = d_ms / max(d_n, 1) / 1e3
-    f_gbs = force_bytes / f_avg_s / 1e9
-    d_gbs = drift_bytes / d_avg_s / 1e9
+    f_gbs = step_bytes / f_avg_s / 1e9
+    r_avg_us = r_ms / max(r_n, 1) * 1e3
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -192,20 +194,17 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_nve_force_kick",
+            "kernel": "k_nve_fused",
             "achieved": f_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": f_gbs / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(n),
-            "algorithmic_bytes_per_launch": force_bytes,
+            "algorithmic_bytes_per_launch": step_bytes,
             "mean_launch_us": f_avg_s * 1e6,
             "launches": f_n,
         },
-        "kernels": {
-            "k_nve_kick_drift": {"bytes_per_launch": drift_bytes, "mean_launch_us": d_avg_s * 1e6,
-                                 "achieved_GBps": d_gbs, "frac": d_gbs / HBM_PEAK_GBS},
-        },
+        "rebuild_chain": {"mean_us_per_chunk": r_avg_us, "chunks": r_n},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -15,43 +15,6 @@
 
 namespace dpm {
 
-int neighbor_build(dpm_system* s);
-int check_and_rebuild(dpm_system* s, int* kind_out);
-int sum_to_host(dpm_system* s, int what, double* out);
-int max_to_host(dpm_system* s, int what, double* out);
-
-namespace {
-
-inline dim3 g1(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK)); }
-
-int scopy(dpm_system* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
-    DPM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
-}
-
-int check_launch() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error(std::string("kernel launch: ") + hipGetErrorString(e));
-        return -1;
-    }
-    return 0;
-}
-
-template <class T>
-int dalloc(dpm_system* s, T** p, int64_t n) {
-    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
-    void* q = nullptr;
-    DPM_HIP(hipMalloc(&q, bytes));
-    DPM_HIP(hipMemset(q, 0, bytes));
-    s->allocs.push_back(q);
-    *p = (T*)q;
-    return 0;
-}
-
-}  // namespace
-
 // ------------------------------------------------------------------ kernels
 // kernelCalcRigidBumpyForces1, src/particles/rigid_bumpy/kernels.cu:366-417
 // mode 0: write (arrays known zero), 1: accumulate (+=)
@@ -69,7 +32,8 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams
         return;
     }
     double2 xv = d.vpos[v];
-    double2 c = d.pos[d.vpart[v]];
+    double4 cq = d.P0[d.vpart[v]];
+    double2 c = make_double2(cq.x, cq.y);
     double ax = 0.0, ay = 0.0, e = 0.0;
     for (int k = 0; k < nn; ++k) {
         int u = d.vnbr[(int64_t)k * d.Nv + v];
@@ -141,7 +105,7 @@ __global__ void k_bumpy_kick(Dev d, double dt, int rotation) {
     double w = d.omega[i];
     if (rotation) w += d.torque[i] * dt / d.moi[i];
     else w = 0.0;
-    double m = d.mass[i];
+    double m = d.P0[i].w;
     v.x += f.x * dt / m;
     v.y += f.y * dt / m;
     d.vel[i] = v;
@@ -153,14 +117,14 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_drift_com(Dev d, double dt) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double nd = 0.0, cd = 0.0;
     if (i < d.N) {
-        double2 o = d.pos[i], v = d.vel[i];
+        double4 q = d.P0[i];
+        double2 v = d.vel[i];
         double a = d.angle[i];
-        d.lpos[i] = o;
+        d.lpos[i] = make_double2(q.x, q.y);
         double ddx = v.x * dt, ddy = v.y * dt, dda = d.omega[i] * dt;
-        d.dpos[i] = make_double2(ddx, ddy);
         d.dth[i] = dda;
-        double nx = o.x + ddx, ny = o.y + ddy;
-        d.pos[i] = make_double2(nx, ny);
+        double nx = q.x + ddx, ny = q.y + ddy;
+        d.P0[i] = make_double4(nx, ny, q.z, q.w);
         d.angle[i] = a + dda;
         double2 ln = d.lref[i], lc = d.cref[i];
         double dxn = nx - ln.x, dyn = ny - ln.y;
@@ -181,7 +145,8 @@ __global__ void k_bumpy_drift_vertices(Dev d) {
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= d.Nv) return;
     int p = d.vpart[v];
-    double2 o = d.lpos[p], n = d.pos[p], x = d.vpos[v];
+    double4 nq = d.P0[p];
+    double2 o = d.lpos[p], n = make_double2(nq.x, nq.y), x = d.vpos[v];
     double dth = d.dth[p];
     double rx = x.x - o.x, ry = x.y - o.y;
     double c = cos(dth), sn = sin(dth);
@@ -216,15 +181,17 @@ __global__ void k_bumpy_mix(Dev d, double alpha) {
     d.omega[i] = w * (1 - alpha) + t * tmr;
 }
 
-__device__ __forceinline__ bool within_div2(double2 a, double2 b, double c2, double Lx, double Ly) {
-    double dx = pbc_div(a.x, b.x, Lx);
-    double dy = pbc_div(a.y, b.y, Ly);
+__device__ __forceinline__ bool within_v(double ax, double ay, double bx, double by,
+                                         const NbrParams& nb, double c2) {
+    double dx = pbc(ax, bx, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+    double dy = pbc(ay, by, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
     return dx * dx + dy * dy < c2;
 }
 
 // kernelUpdateVertexNeighborList, rigid_bumpy/kernels.cu:870-922
-__global__ void k_bumpy_vertex_build(Dev d, double c2, double p2, double Lx, double Ly,
-                                     int64_t cap) {
+// gated: run only when the rebuild chain chose a list rebuild
+__global__ void k_bumpy_vertex_build(Dev d, NbrParams nb, int gated) {
+    if (gated && d.ctl->kind < 1) return;
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= d.Nv) return;
     double2 xv = d.vpos[v];
@@ -233,11 +200,13 @@ __global__ void k_bumpy_vertex_build(Dev d, double c2, double p2, double Lx, dou
     int cnt = 0;
     for (int k = 0; k < nn; ++k) {
         int q = d.nbr[(int64_t)k * d.N + p];
-        if (within_div2(xv, d.pos[q], p2, Lx, Ly)) {
+        double4 cq = d.P0[q];
+        if (within_v(xv.x, xv.y, cq.x, cq.y, nb, nb.vp2)) {
             int f0 = d.vstart[q], nv = d.nverts[q];
             for (int u = f0; u < f0 + nv; ++u) {
-                if (within_div2(xv, d.vpos[u], c2, Lx, Ly)) {
-                    if (cnt < cap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
+                double2 xu = d.vpos[u];
+                if (within_v(xv.x, xv.y, xu.x, xu.y, nb, nb.vc2)) {
+                    if (cnt < d.vcap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
                     cnt++;
                 }
             }
@@ -245,10 +214,13 @@ __global__ void k_bumpy_vertex_build(Dev d, double c2, double p2, double Lx, dou
     }
     d.vnum[v] = cnt;
     atomicMax(&d.ctl->max_vnum, cnt);
+    atomicMax(&d.ctl->max_vnum_all, cnt);
 }
 
 // exclusive scan of nverts (new order) into vstart for the vertex reorder
-__global__ __launch_bounds__(1024) void k_scan_counts(const int* cnt, int* start, int64_t n) {
+__global__ __launch_bounds__(1024) void k_scan_counts(const Ctl* ctl, int gated, const int* cnt,
+                                                      int* start, int64_t n) {
+    if (gated && ctl->kind != 2) return;
     __shared__ int sums[1024];
     int t = threadIdx.x;
     int64_t per = (n + 1023) / 1024;
@@ -271,7 +243,8 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int* cnt, int* start
 }
 
 // kernelReorderRigidBumpyVertexData, rigid_bumpy/kernels.cu:844-868
-__global__ void k_bumpy_reorder_vertices(Dev d, Spare sp) {
+__global__ void k_bumpy_reorder_vertices(Dev d, Spare sp, int gated) {
+    if (gated && d.ctl->kind != 2) return;
     int64_t ov = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (ov >= d.Nv) return;
     int op = d.vpart[ov];
@@ -284,7 +257,8 @@ __global__ void k_bumpy_reorder_vertices(Dev d, Spare sp) {
     sp.svidx[nv] = d.svidx[ov];
 }
 
-__global__ void k_bumpy_vertex_copyback(Dev d, Spare sp) {
+__global__ void k_bumpy_vertex_copyback(Dev d, Spare sp, int gated) {
+    if (gated && d.ctl->kind != 2) return;
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v < d.Nv) {
         d.vpart[v] = sp.vpart[v];
@@ -300,7 +274,8 @@ __global__ void k_bumpy_stress(Dev d, PairParams p, double rv, double area) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
     double axx = 0.0, axy = 0.0, ayx = 0.0, ayy = 0.0;
-    double2 c = d.pos[i], vel = d.vel[i];
+    double4 cq = d.P0[i];
+    double2 c = make_double2(cq.x, cq.y), vel = d.vel[i];
     double w = d.omega[i];
     int st = d.vstart[i], nv = d.nverts[i];
     for (int a = 0; a < nv; ++a) {
@@ -324,8 +299,8 @@ __global__ void k_bumpy_stress(Dev d, PairParams p, double rv, double area) {
             int u = d.vnbr[(int64_t)k * d.Nv + v];
             if (d.vpart[u] == i) continue;
             double2 xu = d.vpos[u];
-            double xd = pbc_mul(xv.x, xu.x, p.Lx, p.invLx);
-            double yd = pbc_mul(xv.y, xu.y, p.Ly, p.invLy);
+            double xd = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+            double yd = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
             double fx, fy;
             pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
             axx += xd * fx / 2;
@@ -359,8 +334,8 @@ __global__ void k_bumpy_contacts(Dev d, PairParams p, double rv) {
                 int u = d.vnbr[(int64_t)k2 * d.Nv + v];
                 if (d.vpart[u] != q) continue;
                 double2 xu = d.vpos[u];
-                double dx = pbc_mul(xv.x, xu.x, p.Lx, p.invLx);
-                double dy = pbc_mul(xv.y, xu.y, p.Ly, p.invLy);
+                double dx = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+                double dy = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
                 double dist = sqrt(dx * dx + dy * dy);
                 if (dist < vrs) {
                     if (!is_contact) nc++;
@@ -380,17 +355,18 @@ __global__ void k_bumpy_contacts(Dev d, PairParams p, double rv) {
 __global__ void k_bumpy_init_vertices(Dev d, double rv) {
     int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= d.N) return;
-    double a = d.angle[p], r = d.rad[p];
+    double4 cq = d.P0[p];
+    double a = d.angle[p], r = cq.z;
     int nv = d.nverts[p], st = d.vstart[p];
     double inc = 2 * M_PI / nv;
     double inner = r - rv;
-    double2 c = d.pos[p];
+    double2 c = make_double2(cq.x, cq.y);
     for (int i = 0; i < nv; ++i) {
         double va = i * inc + a;
         d.vpos[st + i] = make_double2(c.x + inner * cos(va), c.y + inner * sin(va));
         d.vpart[st + i] = (int)p;
     }
-    double vm = d.mass[p] / static_cast<double>(nv);
+    double vm = cq.w / static_cast<double>(nv);
     double I = 0.0;
     for (int i = 0; i < nv; ++i) {
         d.vmass[st + i] = vm;
@@ -409,7 +385,7 @@ int bumpy_alloc(dpm_system* s) {
         dalloc(s, &d.vpart, v) || dalloc(s, &d.vstart, n) || dalloc(s, &d.nverts, n) ||
         dalloc(s, &d.svidx, v) || dalloc(s, &d.vnum, v) || dalloc(s, &d.vfrc, v) ||
         dalloc(s, &d.vtq, v) || dalloc(s, &d.vpe, v) || dalloc(s, &d.lpos, n) ||
-        dalloc(s, &d.dpos, n) || dalloc(s, &d.dth, n))
+        dalloc(s, &d.dth, n))
         return -1;
     Spare& sp = s->sp;
     if (dalloc(s, &sp.angle, n) || dalloc(s, &sp.omega, n) || dalloc(s, &sp.torque, n) ||
@@ -420,17 +396,13 @@ int bumpy_alloc(dpm_system* s) {
     std::vector<int> id(v);
     for (int64_t k = 0; k < v; ++k) id[k] = (int)k;
     if (scopy(s, d.svidx, id.data(), sizeof(int) * v, hipMemcpyHostToDevice)) return -1;
-    return 0;
+    return bumpy_ensure_vcap(s, 16);
 }
 
-static int ensure_vell(dpm_system* s, int64_t cap) {
+int bumpy_ensure_vcap(dpm_system* s, int64_t cap) {
     if (cap <= s->vphys_cap) return 0;
-    if (s->d.vnbr) {
-        auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->d.vnbr);
-        if (it != s->allocs.end()) s->allocs.erase(it);
-        (void)hipFree(s->d.vnbr);
-        s->d.vnbr = nullptr;
-    }
+    dfree(s, s->d.vnbr);
+    s->d.vnbr = nullptr;
     if (dalloc(s, &s->d.vnbr, s->Nv * cap)) return -1;
     s->vphys_cap = cap;
     s->d.vcap = (int)cap;
@@ -438,84 +410,64 @@ static int ensure_vell(dpm_system* s, int64_t cap) {
     return 0;
 }
 
-static int64_t pow2_at_least(int64_t m) {
-    return (int64_t)std::pow(2.0, std::ceil(std::log2((double)m)));
+// RigidBumpy::updateVertexVerletList (rigid_bumpy.cu:1198-1212), enqueued after the
+// particle list build; overflow is detected by the chain's end kernel.
+void bumpy_enqueue_vertex_build(dpm_system* s, int gated) {
+    hipLaunchKernelGGL(k_bumpy_vertex_build, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
+                       nbr_params(s), gated);
 }
 
-// RigidBumpy::updateVertexVerletList, rigid_bumpy.cu:1198-1212
-int bumpy_vertex_build(dpm_system* s) {
-    if (s->vphys_cap == 0 && ensure_vell(s, 16)) return -1;
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        DPM_HIP(hipMemsetAsync(&s->d.ctl->max_vnum, 0, sizeof(int), s->stream));
-        hipLaunchKernelGGL(k_bumpy_vertex_build, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
-                           s->rc_v * s->rc_v, s->pcut_v * s->pcut_v, s->box[0], s->box[1],
-                           s->vphys_cap);
-        if (check_launch()) return -1;
-        int mx = 0;
-        DPM_HIP(hipMemcpyAsync(&mx, &s->d.ctl->max_vnum, sizeof(int), hipMemcpyDeviceToHost,
-                               s->stream));
-        DPM_HIP(hipStreamSynchronize(s->stream));
-        if (mx > s->vmax_alloc) s->vmax_alloc = pow2_at_least(mx);
-        if (mx > s->vphys_cap) {
-            if (ensure_vell(s, std::max<int64_t>(s->vmax_alloc, pow2_at_least(mx)))) return -1;
-            continue;
-        }
-        return 0;
-    }
-    set_error("bumpy_vertex_build: capacity did not converge");
-    return -1;
-}
-
-int bumpy_cell_reorder_vertices(dpm_system* s) {
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s->stream, s->sp.nverts,
-                       s->sp.vstart, s->N);
-    hipLaunchKernelGGL(k_bumpy_reorder_vertices, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
-                       s->sp);
-    hipLaunchKernelGGL(k_bumpy_vertex_copyback, g1(std::max(s->Nv, s->N)), dim3(BLOCK), 0,
-                       s->stream, s->d, s->sp);
-    return check_launch();
+// Vertex half of the cell reorder: new vertex starts from the permuted counts, then
+// the vertex arrays follow their particles.
+void bumpy_enqueue_vertex_reorder(dpm_system* s, int gated) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s->stream, s->d.ctl, gated,
+                       s->sp.nverts, s->sp.vstart, s->N);
+    hipLaunchKernelGGL(k_bumpy_reorder_vertices, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
+                       s->sp, gated);
+    hipLaunchKernelGGL(k_bumpy_vertex_copyback, pgrid(std::max(s->Nv, s->N)), dim3(BLOCK), 0,
+                       s->stream, s->d, s->sp, gated);
 }
 
 // mode -1: zero the vertex arrays; 1: accumulate (Particle::calculateForces semantics)
 int bumpy_forces(dpm_system* s, int mode) {
     if (mode < 0) {
-        hipLaunchKernelGGL(k_bumpy_zero_vertex, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
+        hipLaunchKernelGGL(k_bumpy_zero_vertex, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
         return check_launch();
     }
-    PairParams p = pair_params(s, s->e_c, s->n_c);
-    hipLaunchKernelGGL(k_bumpy_vertex_forces, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p,
+    PairParams p = params_of(s);
+    hipLaunchKernelGGL(k_bumpy_vertex_forces, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p,
                        s->rv, 1);
-    hipLaunchKernelGGL(k_bumpy_particle_reduce, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, 1);
+    hipLaunchKernelGGL(k_bumpy_particle_reduce, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, 1);
     return check_launch();
 }
 
 int bumpy_kick(dpm_system* s, double dt) {
-    hipLaunchKernelGGL(k_bumpy_kick, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt,
+    hipLaunchKernelGGL(k_bumpy_kick, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, dt,
                        (int)s->rotation);
     return check_launch();
 }
 
 int bumpy_drift(dpm_system* s, double dt) {
-    hipLaunchKernelGGL(k_bumpy_drift_com, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
-    hipLaunchKernelGGL(k_bumpy_drift_vertices, g1(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
+    hipLaunchKernelGGL(k_bumpy_drift_com, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
+    hipLaunchKernelGGL(k_bumpy_drift_vertices, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
     return check_launch();
 }
 
 int bumpy_mix(dpm_system* s, double alpha) {
-    hipLaunchKernelGGL(k_bumpy_mix, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha);
+    hipLaunchKernelGGL(k_bumpy_mix, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha);
     return check_launch();
 }
 
 int bumpy_stress(dpm_system* s) {
-    PairParams p = pair_params(s, s->e_c, s->n_c);
-    hipLaunchKernelGGL(k_bumpy_stress, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
+    PairParams p = params_of(s);
+    hipLaunchKernelGGL(k_bumpy_stress, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
                        s->box[0] * s->box[1]);
     return check_launch();
 }
 
 int bumpy_contacts(dpm_system* s) {
-    PairParams p = pair_params(s, s->e_c, s->n_c);
-    hipLaunchKernelGGL(k_bumpy_contacts, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv);
+    PairParams p = params_of(s);
+    hipLaunchKernelGGL(k_bumpy_contacts, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv);
     return check_launch();
 }
 
@@ -530,7 +482,6 @@ int bumpy_nve_run(dpm_system* s, double dt, int64_t n) {
     return 0;
 }
 
-int bumpy_fire_step(dpm_system* s, dpm_fire_state* st) { return dpm_fire_step(s, st); }
 
 // minimizeFire (src/routines/minimization.cu:75-112) driven from the host.
 int bumpy_minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
@@ -599,10 +550,7 @@ int bumpy_snapshot(dpm_system* s, bool save) {
 
 static int fetch_svidx(dpm_system* s, std::vector<int>& sv) {
     sv.resize(s->Nv);
-    DPM_HIP(hipMemcpyAsync(sv.data(), s->d.svidx, sizeof(int) * s->Nv, hipMemcpyDeviceToHost,
-                           s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
+    return scopy(s, sv.data(), s->d.svidx, sizeof(int) * s->Nv, hipMemcpyDeviceToHost);
 }
 
 static double* pfield(dpm_system* s, int f) {
@@ -699,6 +647,7 @@ int dpm_bumpy_neighbor_config(dpm_system* s, double vc, double vpc) {
     }
     s->rc_v = vc;
     s->pcut_v = vpc;
+    s->gen++;
     return 0;
 }
 
@@ -710,8 +659,9 @@ int dpm_vertex_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int64
     std::vector<int> sv, nn(s->Nv), nb((size_t)(s->Nv * s->vphys_cap));
     if (fetch_svidx(s, sv)) return -1;
     if (scopy(s, nn.data(), s->d.vnum, sizeof(int) * s->Nv, hipMemcpyDeviceToHost)) return -1;
-    DPM_HIP(hipMemcpy(nb.data(), s->d.vnbr, sizeof(int) * s->Nv * s->vphys_cap,
-                      hipMemcpyDeviceToHost));
+    if (scopy(s, nb.data(), s->d.vnbr, sizeof(int) * s->Nv * s->vphys_cap,
+              hipMemcpyDeviceToHost))
+        return -1;
     int64_t c = 0;
     for (int64_t v = 0; v < s->Nv; ++v)
         for (int64_t k = 0; k < nn[v] && k < s->vphys_cap; ++k) {
@@ -759,11 +709,11 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) 
     if (scopy(s, s->d.svidx, id.data(), sizeof(int) * s->Nv, hipMemcpyHostToDevice)) return -1;
     s->rv = rv;
     s->rotation = rotation != 0;
-    hipLaunchKernelGGL(k_bumpy_init_vertices, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, rv);
+    s->gen++;
+    hipLaunchKernelGGL(k_bumpy_init_vertices, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, rv);
     if (check_launch()) return -1;
     DPM_HIP(hipStreamSynchronize(s->stream));
-    s->gen++;
-    return 0;
+    return refresh_rmax(s);
 }
 
 }  // extern "C"

@@ -1,18 +1,26 @@
 // dpm_core.hip -- gfx950 kernels and host control for one particle system
 // (disk path, neighbour lists, reductions, NVE / FIRE / Adam loops, jamming).
 //
-// Reference correspondence is cited per kernel.  Layout in HBM (one system):
-//   pos, vel, frc, lref, cref : double2[N]   (x,y interleaved: one 16 B gather)
-//   rad, mass, pe, ke         : double[N]
-//   nbr                       : int32 ELL, column-major nbr[k*N + i] (lanes coalesce)
-//   nnum, sidx, cell_of       : int32[N]
-//   cell_start                : int32[ncells+1] counting-sort offsets
-//   pmax_nd, pmax_cd          : double[ceil(N/256)] per-block displacement maxima
-//   part                      : double[4*ceil(N/64)] canonical-sum chunk partials
+// Layout in HBM (one system):
+//   P0, P1      : double4[N] {x, y, radius, mass}; P0 is current between calls,
+//                 fused NVE chunks ping-pong positions P0 <-> P1 (one 32 B gather
+//                 per neighbour)
+//   vel, frc, lref, cref : double2[N];  pe, ke : double[N]
+//   nbr         : int32 ELL, column-major nbr[k*N + i] (lanes coalesce); nnum[N]
+//   sidx, cell_of : int32[N];  cell_start : int32[ncells+1] counting-sort offsets
+//   pmax_nd/cd  : double[ceil(N/256)] per-block displacement maxima
+//   part        : double[4*ceil(N/64) + ...] canonical-sum chunk partials
+//
+// Execution model of the hot loops (NVE, FIRE): chunks of steps are captured in a
+// hipGraph.  A step whose drift crossed the rebuild threshold halts itself (and the
+// rest of the chunk); a gated rebuild chain at the end of every chunk performs the
+// rebuild and finishes the halted step on the device, so the host only polls a
+// pinned copy of the control block per chunk.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <type_traits>
 
 #include "dpm_kernels.hpp"
 
@@ -20,6 +28,16 @@ namespace dpm {
 
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
+
+constexpr long long STAMP_STRIDE = 8192;  // > any chunk length
+__device__ __forceinline__ long long stamp(long long epoch, int pos) {
+    return epoch * STAMP_STRIDE + pos + 1;
+}
+
+__device__ __forceinline__ bool resumable(const Dev& d) {
+    const Ctl* c = d.ctl;
+    return c->max_nnum <= d.cap && (d.vnbr == nullptr || c->max_vnum <= d.vcap);
+}
 
 // =========================================================================
 //                                  kernels
@@ -30,115 +48,275 @@ __device__ __forceinline__ int64_t particle_of_thread(int& tile) {
     return (int64_t)tile * BLOCK + threadIdx.x;
 }
 
-// Per-block maxima of the two displacement arrays (the reference keeps the
-// full arrays, particle.cu:1094-1100; only their maxima are ever read) and the
-// rebuild request of Particle::checkFor{Cell,Verlet}ListUpdate
-// (particle.cu:1141-1161): 3*max > threshold.  par < 0: no request.
-__device__ __forceinline__ void disp_epilogue(const Dev& d, int tile, double nd, double cd,
-                                              int method, double thr, double cthr, int par) {
+// Per-block maxima of the two displacement arrays (the reference keeps the full
+// arrays, particle.cu:1094-1100; only their maxima are ever read) and the rebuild
+// request of Particle::checkFor{Cell,Verlet}ListUpdate (particle.cu:1141-1161):
+// 3*max > threshold.  wr < 0: no request.
+__device__ __forceinline__ void disp_epilogue(const Dev& d, const NbrParams& nb, int tile,
+                                              double nd, double cd, int wr, long long st) {
     double bn = block_max(nd);
     double bc = block_max(cd);
     if (threadIdx.x == 0) {
         d.pmax_nd[tile] = bn;
         d.pmax_cd[tile] = bc;
-        if (par >= 0) {
-            bool need = false;
-            if (method == DPM_NBR_CELL) need = (3.0 * bc > cthr) || (3.0 * bn > thr);
-            else if (method == DPM_NBR_VERLET) need = 3.0 * bn > thr;
-            if (need) d.ctl->flag[par] = 1;
-            if (blockIdx.x == 0) d.ctl->flag[par ^ 1] = 0;
+        if (wr >= 0 && (nb.method == DPM_NBR_CELL || nb.method == DPM_NBR_VERLET)) {
+            if (nb.method == DPM_NBR_CELL && 3.0 * bc > nb.cthr) d.ctl->cflag[wr] = st;
+            if (3.0 * bn > nb.thr) d.ctl->flag[wr] = st;
         }
     }
 }
 
-// NVE first half: kernelUpdateVelocities(dt/2) + kernelUpdatePositions(dt)
-// (src/particles/base/kernels.cu:117-139, :48-81) fused.
-__global__ __launch_bounds__(BLOCK) void k_nve_kick_drift(Dev d, double half_dt, double dt,
-                                                          int method, double thr,
-                                                          double cthr, int par) {
-    if (d.ctl->halt) return;
-    int tile;
-    int64_t i = particle_of_thread(tile);
-    double nd = 0.0, cd = 0.0;
-    if (i < d.N) {
-        double2 v = d.vel[i], f = d.frc[i], x = d.pos[i];
-        double dtm = half_dt / d.mass[i];
-        v.x += f.x * dtm;
-        v.y += f.y * dtm;
-        x.x += v.x * dt;
-        x.y += v.y * dt;
-        d.vel[i] = v;
-        d.pos[i] = x;
-        double2 ln = d.lref[i], lc = d.cref[i];
-        double dxn = x.x - ln.x, dyn = x.y - ln.y;
-        nd = dxn * dxn + dyn * dyn;
-        double dxc = x.x - lc.x, dyc = x.y - lc.y;
-        cd = dxc * dxc + dyc * dyc;
-    }
-    disp_epilogue(d, tile, nd, cd, method, thr, cthr, par);
+__device__ __forceinline__ void disp_of(const Dev& d, int64_t i, double x, double y, double& nd,
+                                        double& cd) {
+    double2 ln = d.lref[i], lc = d.cref[i];
+    double dxn = x - ln.x, dyn = y - ln.y;
+    nd = dxn * dxn + dyn * dyn;
+    double dxc = x - lc.x, dyc = y - lc.y;
+    cd = dxc * dxc + dyc * dyc;
 }
 
-// Sum of harmonic pair forces/energies over particle i's ELL row:
-// kernelCalcDiskForces, src/particles/disk/kernels.cu:11-57.
-__device__ __forceinline__ void disk_row(const Dev& d, const PairParams& p, int64_t i, double& ax,
-                                         double& ay, double& e) {
-    double2 xi = d.pos[i];
-    double ri = d.rad[i];
+// The rebuild a step at (epoch, pos) must perform before its forces:
+// 0 none, 1 list, 2 cells + list (the cell test is the reference's first,
+// particle.cu:1141-1161), 3 the list of the last in-kernel rebuild again, from its
+// stored build positions, with a larger capacity.  ovf holds the stamp of the step
+// whose in-kernel rebuild overflowed; that step itself ignores it, so every block of
+// a launch reaches the same decision.
+__device__ __forceinline__ bool ovf_pending(const Ctl* c, long long st) {
+    return c->ovf != 0 && c->ovf != st;
+}
+
+__device__ __forceinline__ int pending_rebuild(const Ctl* c, long long epoch, int pos) {
+    long long st = stamp(epoch, pos);
+    if (c->cflag[pos & 1] == st) return 2;
+    bool list = c->flag[pos & 1] == st;
+    if (ovf_pending(c, st)) return list ? 1 : 3;
+    return list ? 1 : 0;
+}
+
+__device__ __forceinline__ int wrap_cell(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// In-kernel list rebuild of particle i's row (kernelUpdateCellNeighborList,
+// base/kernels.cu:440-505: 3x3 stencil, dy outer, dx inner, ascending sorted index),
+// summing the pair forces of the accepted neighbours in the same pass.  The row is
+// accepted in exactly the order the stored list is later traversed, so the sums are
+// bit-identical to a separate build followed by disk_row.  Returns the row length.
+__device__ __forceinline__ int stencil_row(const Dev& d, const double4* P, const PairParams& p,
+                                           const NbrParams& nb, int64_t i, double4 qi,
+                                           double& ax, double& ay) {
+    const int ncd = nb.ncd;
+    int cid = d.cell_of[i];
+    int cx = cid % ncd, cy = cid / ncd;
+    int cnt = 0;
+    const int cap = d.cap;
+    ax = 0.0;
+    ay = 0.0;
+    for (int oy = -1; oy <= 1; ++oy) {
+        int yy = wrap_cell(cy + oy, ncd);
+        for (int ox = -1; ox <= 1; ++ox) {
+            int c = wrap_cell(cx + ox, ncd) + yy * ncd;
+            int s = d.cell_start[c], e = d.cell_start[c + 1];
+            for (int j = s; j < e; ++j) {
+                if (j == i) continue;
+                double4 qj = P[j];
+                double dx = pbc(qi.x, qj.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+                double dy = pbc(qi.y, qj.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+                if (dx * dx + dy * dy < nb.rc2) {
+                    if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
+                    cnt++;
+                    double fx, fy;
+                    pair_core(dx, dy, qi.z, qj.z, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                }
+            }
+        }
+    }
+    d.nnum[i] = cnt;
+    return cnt;
+}
+
+// Sum of harmonic pair forces/energies over particle i's ELL row, in row order:
+// kernelCalcDiskForces, src/particles/disk/kernels.cu:11-57.  The next neighbour's
+// index and position are in flight while the current pair is evaluated.
+__device__ __forceinline__ void disk_row(const Dev& d, const double4* P, const PairParams& p,
+                                         int64_t i, double4 qi, double& ax, double& ay,
+                                         double& e) {
     int nn = min(d.nnum[i], d.cap);
     const int* col = d.nbr + i;
+    const int64_t N = d.N;
     ax = 0.0;
     ay = 0.0;
     e = 0.0;
+    if (nn == 0) return;
+    int j1 = nn > 1 ? col[N] : (int)i;
+    double4 qa = P[col[0]];
     for (int k = 0; k < nn; ++k) {
-        int j = col[(int64_t)k * d.N];
-        double2 xj = d.pos[j];
-        double rj = d.rad[j];
+        int j2 = (k + 2 < nn) ? col[(int64_t)(k + 2) * N] : (int)i;
+        double4 qb = P[j1];
         double fx, fy;
-        double ie = pair_law(xi.x, xi.y, ri, xj.x, xj.y, rj, p, fx, fy);
+        double ie = pair_law(qi.x, qi.y, qi.z, qa.x, qa.y, qa.z, p, fx, fy);
         ax += fx;
         ay += fy;
         e += ie;
+        qa = qb;
+        j1 = j2;
     }
 }
 
-// NVE second half: zero + kernelCalcDiskForces + kernelUpdateVelocities(dt/2)
-// fused (src/integrator/nve.cu:21-24).  Gated on the rebuild request of this
-// step: when raised, the step halts here and the host rebuilds first.
-__global__ __launch_bounds__(BLOCK) void k_nve_force_kick(Dev d, PairParams p, double half_dt,
-                                                          int par, int cpos) {
+// ---- NVE (src/integrator/nve.cu:15-25) ---------------------------------------
+// One launch per step: zero + forces + closing kick(dt/2) of step s, then the
+// opening kick(dt/2) + drift(dt) of step s+1 (kernelCalcDiskForces,
+// kernelUpdateVelocities twice, kernelUpdatePositions; base/kernels.cu:48-139).
+// Positions are read from P[cur] and written to P[1-cur], so neighbours are read
+// at their old positions.  The forces and energies of these intermediate steps are
+// consumed in registers and never stored: the step that ends a run (k_nve_force)
+// writes the arrays the caller can observe.
+__global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrParams nb,
+                                                     double hdt, double dt, int cur, int pos,
+                                                     int last) {
     Ctl* c = d.ctl;
     if (c->halt) return;
-    if (par >= 0 && c->flag[par]) {
+    long long e = c->epoch;
+    int need = pending_rebuild(c, e, pos);
+    // list rebuilds of a cell list are done here, row by row; the rest halts the chunk
+    bool in_kernel = need == 1 && nb.method == DPM_NBR_CELL && !ovf_pending(c, stamp(e, pos));
+    if (need != 0 && !in_kernel) {
         if (threadIdx.x == 0) {
             c->halt = 1;
-            c->halt_pos = cpos;
+            c->halt_pos = pos;
+            c->halt_mode = H_NVE_FUSED;
+            c->halt_kind = need;
         }
         return;
+    }
+    const double4* Pc = cur ? d.P1 : d.P0;
+    double4* Pn = cur ? d.P0 : d.P1;
+    int tile;
+    int64_t i = particle_of_thread(tile);
+    double nd = 0.0, cd = 0.0;
+    int cnt = 0;
+    if (i < d.N) {
+        double4 qi = Pc[i];
+        double ax, ay;
+        double2 ln;
+        if (in_kernel) {
+            cnt = stencil_row(d, Pc, p, nb, i, qi, ax, ay);
+            ln = make_double2(qi.x, qi.y);
+            d.lref[i] = ln;
+            if (cnt > d.cap) c->ovf = stamp(e, pos);
+        } else {
+            double en;
+            disk_row(d, Pc, p, i, qi, ax, ay, en);
+            ln = d.lref[i];
+        }
+        double fx = 0.0 + ax, fy = 0.0 + ay;
+        double dtm = hdt / qi.w;
+        double2 v = d.vel[i];
+        v.x += fx * dtm;  // closing half-kick of step s
+        v.y += fy * dtm;
+        v.x += fx * dtm;  // opening half-kick of step s+1
+        v.y += fy * dtm;
+        double x = qi.x + v.x * dt, y = qi.y + v.y * dt;
+        d.vel[i] = v;
+        Pn[i] = make_double4(x, y, qi.z, qi.w);
+        double dxn = x - ln.x, dyn = y - ln.y;
+        nd = dxn * dxn + dyn * dyn;
+        double2 lc = d.cref[i];
+        double dxc = x - lc.x, dyc = y - lc.y;
+        cd = dxc * dxc + dyc * dyc;
+    }
+    if (in_kernel) {  // row-length statistics of the rebuild (Particle::max_neighbors)
+        int m = wave_max_i(cnt);
+        if (threadIdx.x % WAVE == 0 && m > 0) atomicMax(&c->max_nnum_all, m);
+        if (blockIdx.x == 0 && threadIdx.x == 0) c->n_rebuilds += 1;
+    }
+    long long st = last ? stamp(e + 1, 0) : stamp(e, pos + 1);
+    disp_epilogue(d, nb, tile, nd, cd, (pos + 1) & 1, st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->progress = pos + 1;
+}
+
+// Zero + forces + closing kick(dt/2) on P0 (the end of a run, the first half of a
+// split step, or the resumption of a halted step).
+__global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double hdt, int pos,
+                                                     int hmode, int gate) {
+    Ctl* c = d.ctl;
+    if (gate == G_NORMAL) {
+        if (c->halt) return;
+        int need = pending_rebuild(c, c->epoch, pos);
+        if (need) {
+            if (threadIdx.x == 0) {
+                c->halt = 1;
+                c->halt_pos = pos;
+                c->halt_mode = hmode;
+                c->halt_kind = need;
+            }
+            return;
+        }
+    } else if (gate == G_RESUME) {
+        if (!c->halt || !resumable(d) || c->halt_mode == H_FIRE) return;
     }
     int tile;
     int64_t i = particle_of_thread(tile);
     if (i >= d.N) return;
-    double ax, ay, e;
-    disk_row(d, p, i, ax, ay, e);
+    double4 qi = d.P0[i];
+    double ax, ay, en;
+    disk_row(d, d.P0, p, i, qi, ax, ay, en);
     double fx = 0.0 + ax, fy = 0.0 + ay;
     d.frc[i] = make_double2(fx, fy);
-    d.pe[i] = 0.0 + e;
-    double dtm = half_dt / d.mass[i];
+    d.pe[i] = 0.0 + en;
+    double dtm = hdt / qi.w;
     double2 v = d.vel[i];
     v.x += fx * dtm;
     v.y += fy * dtm;
     d.vel[i] = v;
 }
 
-// Disk::calculateForces as called stand-alone: forces are accumulated (+=),
-// exactly as the reference kernel does into caller-zeroed arrays.
+// Opening kick(dt/2) + drift(dt) in place on P0.  (wr_pos, epoch + wr_eoff) names
+// the step that must check the rebuild request; progress >= 0 is recorded
+// (resume: halt_pos + 1).
+__global__ __launch_bounds__(BLOCK) void k_nve_kick_drift(Dev d, NbrParams nb, double hdt,
+                                                          double dt, int wr_pos, int wr_eoff,
+                                                          int progress, int gate) {
+    Ctl* c = d.ctl;
+    if (gate == G_NORMAL) {
+        if (c->halt) return;
+    } else if (gate == G_RESUME) {
+        if (!c->halt || !resumable(d) || c->halt_mode != H_NVE_FUSED) return;
+    }
+    long long e = c->epoch;
+    int tile;
+    int64_t i = particle_of_thread(tile);
+    double nd = 0.0, cd = 0.0;
+    if (i < d.N) {
+        double4 q = d.P0[i];
+        double2 v = d.vel[i], f = d.frc[i];
+        double dtm = hdt / q.w;
+        v.x += f.x * dtm;
+        v.y += f.y * dtm;
+        q.x += v.x * dt;
+        q.y += v.y * dt;
+        d.vel[i] = v;
+        d.P0[i] = q;
+        disp_of(d, i, q.x, q.y, nd, cd);
+    }
+    disp_epilogue(d, nb, tile, nd, cd, wr_pos & 1, stamp(e + wr_eoff, wr_pos));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (gate == G_RESUME) c->progress = c->halt_pos + 1;
+        else if (progress >= 0) c->progress = progress;
+    }
+}
+
+// ---- stand-alone pieces of the Particle interface -------------------------------
+// Disk::calculateForces as called stand-alone: forces are accumulated (+=), exactly
+// as the reference kernel does into caller-zeroed arrays.
 __global__ __launch_bounds__(BLOCK) void k_disk_forces_acc(Dev d, PairParams p) {
     int tile;
     int64_t i = particle_of_thread(tile);
     if (i >= d.N) return;
     if (d.nnum[i] == 0) return;
+    double4 qi = d.P0[i];
     double ax, ay, e;
-    disk_row(d, p, i, ax, ay, e);
+    disk_row(d, d.P0, p, i, qi, ax, ay, e);
     double2 f = d.frc[i];
     f.x += ax;
     f.y += ay;
@@ -166,29 +344,26 @@ __global__ void k_kick(Dev d, double dt) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
     double2 v = d.vel[i], f = d.frc[i];
-    double dtm = dt / d.mass[i];
+    double dtm = dt / d.P0[i].w;
     v.x += f.x * dtm;
     v.y += f.y * dtm;
     d.vel[i] = v;
 }
 
 // kernelUpdatePositions, src/particles/base/kernels.cu:48-81
-__global__ __launch_bounds__(BLOCK) void k_drift(Dev d, double dt) {
+__global__ __launch_bounds__(BLOCK) void k_drift(Dev d, NbrParams nb, double dt) {
     int tile;
     int64_t i = particle_of_thread(tile);
     double nd = 0.0, cd = 0.0;
     if (i < d.N) {
-        double2 x = d.pos[i], v = d.vel[i];
-        x.x += v.x * dt;
-        x.y += v.y * dt;
-        d.pos[i] = x;
-        double2 ln = d.lref[i], lc = d.cref[i];
-        double dxn = x.x - ln.x, dyn = x.y - ln.y;
-        nd = dxn * dxn + dyn * dyn;
-        double dxc = x.x - lc.x, dyc = x.y - lc.y;
-        cd = dxc * dxc + dyc * dyc;
+        double4 q = d.P0[i];
+        double2 v = d.vel[i];
+        q.x += v.x * dt;
+        q.y += v.y * dt;
+        d.P0[i] = q;
+        disp_of(d, i, q.x, q.y, nd, cd);
     }
-    disp_epilogue(d, tile, nd, cd, 0, 0.0, 0.0, -1);
+    disp_epilogue(d, nb, tile, nd, cd, -1, 0);
 }
 
 // kernelMixVelocitiesAndForces, src/particles/base/kernels.cu:91-112
@@ -215,17 +390,19 @@ __global__ void k_mix(Dev d, double alpha) {
     d.vel[i] = v;
 }
 
-// kernelCalculateTranslationalKineticEnergy, src/particles/base/kernels.cu:163-180
+// kernelCalculateTranslationalKineticEnergy (base/kernels.cu:163-180) and the
+// rigid version (rigid_bumpy/kernels.cu:254-269)
 __global__ void k_kinetic(Dev d, int rigid_rot) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
     double2 v = d.vel[i];
+    double m = d.P0[i].w;
     double v2 = v.x * v.x + v.y * v.y;
     if (rigid_rot) {
         double w = d.omega[i];
-        d.ke[i] = 0.5 * d.mass[i] * v2 + 0.5 * d.moi[i] * w * w;
+        d.ke[i] = 0.5 * m * v2 + 0.5 * d.moi[i] * w * w;
     } else {
-        d.ke[i] = 0.5 * d.mass[i] * v2;
+        d.ke[i] = 0.5 * m * v2;
     }
 }
 
@@ -243,7 +420,10 @@ __device__ __forceinline__ double sum_value(const Dev& d, int what, int64_t i) {
         case S_SYX: return d.syx[i];
         case S_VX: return d.vel[i].x;
         case S_VY: return d.vel[i].y;
-        case S_R2: return d.rad[i] * d.rad[i];
+        case S_R2: {
+            double r = d.P0[i].z;
+            return r * r;
+        }
         case S_TQW: return d.torque[i] * d.omega[i];
         case S_COUNT: return (double)d.contacts[i];
     }
@@ -266,8 +446,14 @@ __global__ void k_final_sum(const double* part, int64_t nc, double* out) {
 
 __device__ __forceinline__ double max_value(const Dev& d, int what, int64_t i) {
     switch (what) {
-        case M_F2: { double2 f = d.frc[i]; return f.x * f.x + f.y * f.y; }
-        case M_T2: { double t = d.torque[i]; return t * t; }
+        case M_F2: {
+            double2 f = d.frc[i];
+            return f.x * f.x + f.y * f.y;
+        }
+        case M_T2: {
+            double t = d.torque[i];
+            return t * t;
+        }
     }
     return 0.0;
 }
@@ -290,17 +476,18 @@ __global__ __launch_bounds__(BLOCK) void k_final_max(const double* part, int64_t
 __global__ __launch_bounds__(BLOCK) void k_disk_stress(Dev d, PairParams p, double area) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
-    double2 xi = d.pos[i], v = d.vel[i];
-    double m = d.mass[i], ri = d.rad[i];
+    double4 qi = d.P0[i];
+    double2 v = d.vel[i];
+    double m = qi.w;
     double axx = m * v.x * v.x, axy = m * v.x * v.y, ayx = axy, ayy = m * v.y * v.y;
     int nn = min(d.nnum[i], d.cap);
     for (int k = 0; k < nn; ++k) {
         int j = d.nbr[(int64_t)k * d.N + i];
-        double2 xj = d.pos[j];
-        double xd = pbc_mul(xi.x, xj.x, p.Lx, p.invLx);
-        double yd = pbc_mul(xi.y, xj.y, p.Ly, p.invLy);
+        double4 qj = d.P0[j];
+        double xd = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, p.fast);
+        double yd = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, p.fast);
         double fx, fy;
-        pair_law(xi.x, xi.y, ri, xj.x, xj.y, d.rad[j], p, fx, fy);
+        pair_law(qi.x, qi.y, qi.z, qj.x, qj.y, qj.z, p, fx, fy);
         axx += xd * fx / 2;
         axy += xd * fy / 2;
         ayx += yd * fx / 2;
@@ -316,41 +503,42 @@ __global__ __launch_bounds__(BLOCK) void k_disk_stress(Dev d, PairParams p, doub
 __global__ __launch_bounds__(BLOCK) void k_disk_contacts(Dev d, PairParams p) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
-    double2 xi = d.pos[i];
-    double ri = d.rad[i];
+    double4 qi = d.P0[i];
     int c = 0;
     int nn = min(d.nnum[i], d.cap);
     for (int k = 0; k < nn; ++k) {
         int j = d.nbr[(int64_t)k * d.N + i];
-        double2 xj = d.pos[j];
-        double dx = pbc_mul(xi.x, xj.x, p.Lx, p.invLx);
-        double dy = pbc_mul(xi.y, xj.y, p.Ly, p.invLy);
+        double4 qj = d.P0[j];
+        double dx = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, p.fast);
+        double dy = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, p.fast);
         double dist = sqrt(dx * dx + dy * dy);
-        if (dist < ri + d.rad[j]) c++;
+        if (dist < qi.z + qj.z) c++;
     }
     d.contacts[i] = c;
 }
 
-// positions.scale (Particle::scalePositionsFull, particle.cu:1070-1072)
+// positions.scale (Particle::scalePositionsFull, particle.cu:1070-1072); rigid:
+// kernelScalePositions, rigid_bumpy/kernels.cu:1308-1333
 __global__ void k_scale_positions(Dev d, double f) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
-    double2 x = d.pos[i];
-    if (d.vpos) {  // kernelScalePositions, rigid_bumpy/kernels.cu:1308-1333
-        double nx = x.x * f, ny = x.y * f;
+    double4 q = d.P0[i];
+    if (d.vpos) {
+        double nx = q.x * f, ny = q.y * f;
         int st = d.vstart[i], nv = d.nverts[i];
         for (int v = st; v < st + nv; ++v) {
-            double2 q = d.vpos[v];
-            q.x += (nx - x.x);
-            q.y += (ny - x.y);
-            d.vpos[v] = q;
+            double2 w = d.vpos[v];
+            w.x += (nx - q.x);
+            w.y += (ny - q.y);
+            d.vpos[v] = w;
         }
-        d.pos[i] = make_double2(nx, ny);
-        return;
+        q.x = nx;
+        q.y = ny;
+    } else {
+        q.x *= f;
+        q.y *= f;
     }
-    x.x *= f;
-    x.y *= f;
-    d.pos[i] = x;
+    d.P0[i] = q;
 }
 
 __global__ void k_add_velocity(Dev d, double ax, double ay, double scale) {
@@ -368,8 +556,9 @@ __global__ void k_add_velocity(Dev d, double ax, double ay, double scale) {
 }
 
 // kernelAdamStep, src/particles/base/kernels.cu:550-611
-__global__ __launch_bounds__(BLOCK) void k_adam_update(Dev d, double alpha, double b1, double b2,
-                                                       double omb1, double omb2, double eps) {
+__global__ __launch_bounds__(BLOCK) void k_adam_update(Dev d, NbrParams nb, double alpha,
+                                                       double b1, double b2, double omb1,
+                                                       double omb2, double eps) {
     int tile;
     int64_t i = particle_of_thread(tile);
     double nd = 0.0, cd = 0.0;
@@ -383,61 +572,83 @@ __global__ __launch_bounds__(BLOCK) void k_adam_update(Dev d, double alpha, doub
         double vhx = m2.x / omb2, vhy = m2.y / omb2;
         double ux = -alpha * mhx / (sqrt(vhx) + eps);
         double uy = -alpha * mhy / (sqrt(vhy) + eps);
-        double2 x = d.pos[i];
-        x.x += ux;
-        x.y += uy;
-        d.pos[i] = x;
-        double2 ln = d.lref[i], lc = d.cref[i];
-        double dxn = x.x - ln.x, dyn = x.y - ln.y;
-        nd = dxn * dxn + dyn * dyn;
-        double dxc = x.x - lc.x, dyc = x.y - lc.y;
-        cd = dxc * dxc + dyc * dyc;
+        double4 q = d.P0[i];
+        q.x += ux;
+        q.y += uy;
+        d.P0[i] = q;
+        disp_of(d, i, q.x, q.y, nd, cd);
         d.m1[i] = m1;
         d.m2[i] = m2;
     }
-    disp_epilogue(d, tile, nd, cd, 0, 0.0, 0.0, -1);
+    disp_epilogue(d, nb, tile, nd, cd, -1, 0);
 }
 
-__global__ void k_fill_int(int* a, int v, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-__global__ void k_fill_double(double* a, double v, int64_t n) {
+// gate != nullptr: runs only for a cell rebuild (kind 2)
+__global__ void k_fill_int(int* a, int v, int64_t n, const Ctl* gate) {
+    if (gate && gate->kind != 2) return;
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) a[i] = v;
 }
 
 // ------------------------------------------------------------- neighbours
-__device__ __forceinline__ int64_t cell_coord(double pos, double cs, int64_t n) {
+// Every rebuild kernel takes `gated`: when set it runs only if the chain chose a
+// rebuild that includes it: the cell kernels for kind 2 (cells + list), the list
+// kernels for kinds 1 (list), 2 and 3 (list from the stored build positions).
+#define GATE_CELL \
+    if (gated && d.ctl->kind != 2) return;
+#define GATE_LIST \
+    if (gated && d.ctl->kind < 1) return;
+
+__device__ __forceinline__ int cell_coord(double pos, double cs, int n) {
     // getPBCCellIndex, include/particles/base/kernels.cuh:285-301
-    int64_t idx = (int64_t)floor(pos / cs);
-    return idx >= 0 ? idx % n : (idx % n + n) % n;
+    long long idx = (long long)floor(pos / cs);
+    return (int)(idx >= 0 ? idx % n : (idx % n + n) % n);
 }
 
 // kernelGetCellIndexForParticle (base/kernels.cu:309-330) + counting-sort histogram
-__global__ void k_cell_bin(Dev d, double cs, int64_t ncd) {
+__global__ void k_cell_bin(Dev d, NbrParams nb, int gated) {
+    GATE_CELL;
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
-    double2 x = d.pos[i];
-    int64_t c = cell_coord(x.x, cs, ncd) + cell_coord(x.y, cs, ncd) * ncd;
-    if (!(c >= 0 && c < ncd * ncd)) {  // non-finite position: keep the index in range, report
+    double4 q = d.P0[i];
+    long long c = (long long)cell_coord(q.x, nb.cs, nb.ncd) +
+                  (long long)cell_coord(q.y, nb.cs, nb.ncd) * nb.ncd;
+    if (!(c >= 0 && c < nb.ncells)) {  // non-finite position: keep the index in range, report
         c = 0;
-        d.ctl->overflow = 1;
+        d.ctl->bad_pos = 1;
     }
     d.cell_tmp[i] = (int)c;
     atomicAdd(&d.cell_cnt[c], 1);
 }
 
-// Exclusive scan of the cell histogram into cell_start (one 1024-thread block).
-// Replaces kernelGetFirstParticleIndexForCell's O(n_cells * N) scan
-// (base/kernels.cu:332-438); empty cells get start == end instead of -1.
-__global__ __launch_bounds__(1024) void k_cell_scan(Dev d, int64_t nc) {
+// Exclusive scan of the histogram into cell_start, in three parallel passes
+// (block sums, scan of the block sums, per-block scan + offset).  Replaces
+// kernelGetFirstParticleIndexForCell's O(n_cells*N) search (base/kernels.cu:332-438);
+// empty cells get start == end instead of -1, which the stencil loop handles
+// identically (base/kernels.cu:470-478).
+constexpr int SCAN_TILE = 1024;  // cells per block (4 per thread)
+
+__global__ __launch_bounds__(BLOCK) void k_scan_a(Dev d, long long nc, int gated) {
+    GATE_CELL;
+    __shared__ int sm[BLOCK / WAVE];
+    long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int s = 0;
+    for (int t = 0; t < 4; ++t)
+        if (base + t < nc) s += d.cell_cnt[base + t];
+    for (int o = WAVE / 2; o > 0; o >>= 1) s += __shfl_down(s, o, WAVE);
+    if (threadIdx.x % WAVE == 0) sm[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) d.scan_bsum[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_b(Dev d, long long nb_blocks, int gated) {
+    GATE_CELL;
     __shared__ int sums[1024];
     int t = threadIdx.x;
-    int64_t per = (nc + 1023) / 1024;
-    int64_t lo = t * per, hi = min(lo + per, nc);
+    long long per = (nb_blocks + 1023) / 1024;
+    long long lo = t * per, hi = min(lo + per, nb_blocks);
     int loc = 0;
-    for (int64_t c = lo; c < hi; ++c) loc += d.cell_cnt[c];
+    for (long long b = lo; b < hi; ++b) loc += d.scan_bsum[b];
     sums[t] = loc;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
@@ -446,16 +657,44 @@ __global__ __launch_bounds__(1024) void k_cell_scan(Dev d, int64_t nc) {
         sums[t] += v;
         __syncthreads();
     }
-    int base = sums[t] - loc;
-    for (int64_t c = lo; c < hi; ++c) {
-        d.cell_start[c] = base;
-        d.cell_cur[c] = base;
-        base += d.cell_cnt[c];
+    int run = sums[t] - loc;
+    for (long long b = lo; b < hi; ++b) {
+        int x = d.scan_bsum[b];
+        d.scan_bsum[b] = run;
+        run += x;
     }
-    if (t == 1023) d.cell_start[nc] = (int)d.N;
 }
 
-__global__ void k_cell_scatter(Dev d) {
+__global__ __launch_bounds__(BLOCK) void k_scan_c(Dev d, long long nc, int gated) {
+    GATE_CELL;
+    __shared__ int sm[BLOCK];
+    long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int t = 0; t < 4; ++t) {
+        v[t] = (base + t < nc) ? d.cell_cnt[base + t] : 0;
+        s += v[t];
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+        int x = ((int)threadIdx.x >= off) ? sm[threadIdx.x - off] : 0;
+        __syncthreads();
+        sm[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = d.scan_bsum[blockIdx.x] + sm[threadIdx.x] - s;
+    for (int t = 0; t < 4; ++t) {
+        if (base + t < nc) {
+            d.cell_start[base + t] = run;
+            d.cell_cur[base + t] = run;
+        }
+        run += v[t];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.cell_start[nc] = (int)d.N;
+}
+
+__global__ void k_cell_scatter(Dev d, int gated) {
+    GATE_CELL;
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
     int c = d.cell_tmp[i];
@@ -465,8 +704,9 @@ __global__ void k_cell_scatter(Dev d) {
 
 // Restore stability inside each cell: ascending previous index, which makes the
 // order identical to thrust::sort_by_key's stable radix sort (particle.cu:1293).
-__global__ void k_cell_sort(Dev d, int64_t nc) {
-    int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ void k_cell_sort(Dev d, long long nc, int gated) {
+    GATE_CELL;
+    long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
     int s = d.cell_start[c], e = d.cell_start[c + 1];
     for (int a = s + 1; a < e; ++a) {
@@ -480,19 +720,20 @@ __global__ void k_cell_sort(Dev d, int64_t nc) {
     }
 }
 
-// kernelReorderParticleData, base/kernels.cu:508-543 (+ Adam moments, static ids)
-__global__ void k_reorder_gather(Dev d, Spare sp) {
+// kernelReorderParticleData, base/kernels.cu:508-543 (+ Adam moments, static ids,
+// rigid-body particle arrays of kernelReorderRigidBumpyParticleData,
+// rigid_bumpy/kernels.cu:796-842)
+__global__ void k_reorder_gather(Dev d, Spare sp, int gated) {
+    GATE_CELL;
     int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (k >= d.N) return;
     int o = d.perm[k];
-    double2 x = d.pos[o];
-    sp.pos[k] = x;
+    double4 q = d.P0[o];
+    sp.P[k] = q;
     sp.vel[k] = d.vel[o];
     sp.frc[k] = d.frc[o];
-    sp.rad[k] = d.rad[o];
-    sp.mass[k] = d.mass[o];
     sp.sidx[k] = d.sidx[o];
-    d.cref[k] = x;
+    d.cref[k] = make_double2(q.x, q.y);
     d.cell_of[k] = d.cell_tmp[o];
     if (d.m1) {
         sp.m1[k] = d.m1[o];
@@ -508,14 +749,16 @@ __global__ void k_reorder_gather(Dev d, Spare sp) {
     }
 }
 
-__global__ void k_reorder_copyback(Dev d, Spare sp) {
+__global__ __launch_bounds__(BLOCK) void k_reorder_copyback(Dev d, Spare sp, int gated) {
+    GATE_CELL;
     int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (threadIdx.x == 0) d.pmax_cd[blockIdx.x] = 0.0;  // the grid has one block per tile
     if (k >= d.N) return;
-    d.pos[k] = sp.pos[k];
+    double4 q = sp.P[k];
+    d.P0[k] = q;
+    d.P1[k] = q;
     d.vel[k] = sp.vel[k];
     d.frc[k] = sp.frc[k];
-    d.rad[k] = sp.rad[k];
-    d.mass[k] = sp.mass[k];
     d.sidx[k] = sp.sidx[k];
     if (d.m1) {
         d.m1[k] = sp.m1[k];
@@ -530,65 +773,87 @@ __global__ void k_reorder_copyback(Dev d, Spare sp) {
     }
 }
 
-__device__ __forceinline__ bool within_div(double2 a, double2 b, double c2, double Lx, double Ly) {
+__device__ __forceinline__ bool within(double ax, double ay, double bx, double by,
+                                       const NbrParams& nb, double c2) {
     // isWithinCutoffSquared, include/particles/base/kernels.cuh:249-260
-    double dx = pbc_div(a.x, b.x, Lx);
-    double dy = pbc_div(a.y, b.y, Ly);
+    double dx = pbc(ax, bx, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+    double dy = pbc(ay, by, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
     return dx * dx + dy * dy < c2;
 }
 
 // kernelUpdateCellNeighborList, base/kernels.cu:440-505: 3x3 stencil in the
-// reference's order (dy outer, dx inner, ascending sorted index), written to
-// the column-major ELL.
-__global__ __launch_bounds__(BLOCK) void k_build_cell(Dev d, double rc2, double Lx, double Ly,
-                                                      int64_t ncd, int64_t cap) {
+// reference's order (dy outer, dx inner, ascending sorted index), written to the
+// column-major ELL.  The nine cell ranges are loaded up front.
+// kind 3 rebuilds the list of the last in-kernel rebuild from its build positions
+// (lref) without touching the displacement state.
+__global__ __launch_bounds__(BLOCK) void k_build_cell(Dev d, NbrParams nb, int gated) {
+    GATE_LIST;
+    const bool from_ref = d.ctl->kind == 3;
     int tile;
     int64_t i = particle_of_thread(tile);
-    if (threadIdx.x == 0) d.pmax_nd[tile] = 0.0;
+    if (threadIdx.x == 0 && !from_ref) d.pmax_nd[tile] = 0.0;
     if (i >= d.N) return;
-    double2 xi = d.pos[i];
+    double2 xi = from_ref ? d.lref[i] : make_double2(d.P0[i].x, d.P0[i].y);
+    int ncd = nb.ncd;
     int cid = d.cell_of[i];
-    int64_t cx = cid % ncd, cy = cid / ncd;
+    int cx = cid % ncd, cy = cid / ncd;
+    int s[9], e[9];
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+        int yy = cy + oy;
+        yy = yy < 0 ? yy + ncd : (yy >= ncd ? yy - ncd : yy);
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+            int xx = cx + ox;
+            xx = xx < 0 ? xx + ncd : (xx >= ncd ? xx - ncd : xx);
+            int c = xx + yy * ncd;
+            int t = (oy + 1) * 3 + (ox + 1);
+            s[t] = d.cell_start[c];
+            e[t] = d.cell_start[c + 1];
+        }
+    }
     int cnt = 0;
-    for (int64_t oy = -1; oy <= 1; ++oy) {
-        int64_t yi = ((cy + oy) % ncd + ncd) % ncd;
-        for (int64_t ox = -1; ox <= 1; ++ox) {
-            int64_t xx = ((cx + ox) % ncd + ncd) % ncd;
-            int64_t c = xx + yi * ncd;
-            int s = d.cell_start[c], e = d.cell_start[c + 1];
-            for (int j = s; j < e; ++j) {
-                if (j == i) continue;
-                if (within_div(xi, d.pos[j], rc2, Lx, Ly)) {
-                    if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
-                    cnt++;
-                }
+    const int cap = d.cap;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        for (int j = s[t]; j < e[t]; ++j) {
+            if (j == i) continue;
+            double2 xj = from_ref ? d.lref[j] : make_double2(d.P0[j].x, d.P0[j].y);
+            if (within(xi.x, xi.y, xj.x, xj.y, nb, nb.rc2)) {
+                if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
+                cnt++;
             }
         }
     }
     d.nnum[i] = cnt;
-    d.lref[i] = xi;
+    if (!from_ref) d.lref[i] = xi;
     atomicMax(&d.ctl->max_nnum, cnt);
+    atomicMax(&d.ctl->max_nnum_all, cnt);
 }
 
 // kernelUpdateNeighborList, base/kernels.cu:256-293: all pairs, ascending j,
 // j positions staged through LDS.
-__global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, double rc2, double Lx, double Ly,
-                                                        int64_t cap) {
+__global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, NbrParams nb, int gated) {
+    GATE_LIST;
     __shared__ double2 tp[BLOCK];
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (threadIdx.x == 0) d.pmax_nd[blockIdx.x] = 0.0;
-    double2 xi = (i < d.N) ? d.pos[i] : make_double2(0.0, 0.0);
+    double4 qi = (i < d.N) ? d.P0[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     int cnt = 0;
+    const int cap = d.cap;
     for (int64_t base = 0; base < d.N; base += BLOCK) {
         __syncthreads();
-        if (base + threadIdx.x < d.N) tp[threadIdx.x] = d.pos[base + threadIdx.x];
+        if (base + threadIdx.x < d.N) {
+            double4 q = d.P0[base + threadIdx.x];
+            tp[threadIdx.x] = make_double2(q.x, q.y);
+        }
         __syncthreads();
         int lim = (int)min((int64_t)BLOCK, d.N - base);
         if (i < d.N) {
             for (int t = 0; t < lim; ++t) {
                 int64_t j = base + t;
                 if (j == i) continue;
-                if (within_div(xi, tp[t], rc2, Lx, Ly)) {
+                if (within(qi.x, qi.y, tp[t].x, tp[t].y, nb, nb.rc2)) {
                     if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = (int)j;
                     cnt++;
                 }
@@ -597,8 +862,103 @@ __global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, double rc2, doubl
     }
     if (i < d.N) {
         d.nnum[i] = cnt;
-        d.lref[i] = xi;
+        d.lref[i] = make_double2(qi.x, qi.y);
         atomicMax(&d.ctl->max_nnum, cnt);
+        atomicMax(&d.ctl->max_nnum_all, cnt);
+    }
+}
+
+// ------------------------------------------------------------- rebuild chain
+// mode 0: end of a chunk (acts only if a step halted); 1: check (decide from the
+// current displacement maxima); 2: forced full rebuild (initialisation, restore).
+__global__ __launch_bounds__(BLOCK) void k_chain_decide(Dev d, NbrParams nb, int mode) {
+    Ctl* c = d.ctl;
+    if (c->overflow || (mode == 0 && !c->halt)) {
+        if (threadIdx.x == 0) c->kind = 0;
+        return;
+    }
+    int64_t nblk = (d.N + BLOCK - 1) / BLOCK;
+    double mn = 0.0, mc = 0.0;
+    for (int64_t k = threadIdx.x; k < nblk; k += BLOCK) {
+        mn = fmax(mn, d.pmax_nd[k]);
+        mc = fmax(mc, d.pmax_cd[k]);
+    }
+    mn = block_max(mn);
+    mc = block_max(mc);
+    if (threadIdx.x != 0) return;
+    int kind = 0;
+    if (mode == 0) {
+        kind = c->halt_kind;
+    } else if (mode == 2) {
+        kind = nb.method == DPM_NBR_CELL ? 2 : 1;
+    } else if (nb.method == DPM_NBR_CELL) {
+        if (3.0 * mc > nb.cthr) kind = 2;
+        else if (3.0 * mn > nb.thr) kind = 1;
+    } else if (nb.method == DPM_NBR_VERLET) {
+        if (3.0 * mn > nb.thr) kind = 1;
+    }
+    c->kind = kind;
+    if (kind >= 1) {
+        c->max_nnum = 0;
+        c->max_vnum = 0;
+    }
+    if (kind == 1 || kind == 2) c->n_rebuilds += 1;
+    if (kind == 2) c->n_cell_rebuilds += 1;
+}
+
+// A fused NVE step halted at an odd position left the current positions in P1.
+__global__ void k_chain_copy_cur(Dev d) {
+    const Ctl* c = d.ctl;
+    if (!c->halt || c->halt_mode != H_NVE_FUSED || !(c->halt_pos & 1)) return;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double4 a = d.P1[i], b = d.P0[i];
+    d.P0[i] = make_double4(a.x, a.y, b.z, b.w);
+}
+
+__global__ void k_chain_end(Dev d, int mode) {
+    Ctl* c = d.ctl;
+    bool over = c->max_nnum > d.cap || (d.vnbr != nullptr && c->max_vnum > d.vcap);
+    if (mode != 0) {
+        if (c->kind >= 1 && over) c->overflow = 1;
+        return;
+    }
+    if (c->halt) {
+        if (c->overflow || over) {
+            c->overflow = 1;
+        } else {
+            c->halt = 0;
+            c->ovf = 0;  // the whole list was rebuilt with enough room
+        }
+    }
+    c->last_progress = c->progress;
+    c->progress = 0;
+    c->epoch += 1;
+}
+
+// Host fix after a capacity overflow: rebuild only the list, without counting a
+// new rebuild (the reference regrows inside the same update call).
+__global__ void k_overflow_begin(Dev d) {
+    Ctl* c = d.ctl;
+    c->overflow = 0;
+    if (c->kind != 3) c->kind = 1;
+    c->max_nnum = 0;
+    c->max_vnum = 0;
+}
+
+__global__ void k_ctl_reset(Dev d, int config) {
+    Ctl* c = d.ctl;
+    c->halt = 0;
+    c->progress = 0;
+    c->last_progress = 0;
+    c->done = 0;
+    c->overflow = 0;
+    c->kind = 0;
+    if (config) {  // a new neighbour configuration restarts the capacity statistics
+        c->max_nnum = 0;
+        c->max_nnum_all = 0;
+        c->max_vnum = 0;
+        c->max_vnum_all = 0;
     }
 }
 
@@ -608,69 +968,74 @@ __global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, double rc2, doubl
 //   k_fire_force  : zero + forces + kick(dt/2) + chunk partials of P, PE, |F|max
 //   k_fire_ctl    : canonical sums, minimizeFire's stop test for this step and
 //                   Fire::step's decision for the next (one wave)
-__global__ __launch_bounds__(BLOCK) void k_fire_move(Dev d, int method, double thr, double cthr,
-                                                     int par) {
-    if (d.ctl->halt) return;
+__global__ __launch_bounds__(BLOCK) void k_fire_move(Dev d, NbrParams nb, int pos) {
+    Ctl* c = d.ctl;
+    if (c->halt) return;
     const FireCtl* F = d.fire;
     if (F->done) return;
+    long long e = c->epoch;
     double dt = F->dt, alpha = F->alpha;
     int uphill = F->uphill;
     int tile;
     int64_t i = particle_of_thread(tile);
     double nd = 0.0, cd = 0.0;
     if (i < d.N) {
-        double2 x = d.pos[i], v = d.vel[i], f = d.frc[i];
+        double4 q = d.P0[i];
+        double2 v = d.vel[i], f = d.frc[i];
         if (uphill) {  // updatePositions(-dt/2) with the decreased dt; setVelocitiesToZero
             double bdt = -dt / 2.0;
-            x.x += v.x * bdt;
-            x.y += v.y * bdt;
+            q.x += v.x * bdt;
+            q.y += v.y * bdt;
             v.x = 0.0;
             v.y = 0.0;
         }
-        double dtm = (dt / 2.0) / d.mass[i];
+        double dtm = (dt / 2.0) / q.w;
         v.x += f.x * dtm;
         v.y += f.y * dtm;
         mix_one(v, f, alpha);
-        x.x += v.x * dt;
-        x.y += v.y * dt;
-        d.pos[i] = x;
+        q.x += v.x * dt;
+        q.y += v.y * dt;
+        d.P0[i] = q;
         d.vel[i] = v;
-        double2 ln = d.lref[i], lc = d.cref[i];
-        double dxn = x.x - ln.x, dyn = x.y - ln.y;
-        nd = dxn * dxn + dyn * dyn;
-        double dxc = x.x - lc.x, dyc = x.y - lc.y;
-        cd = dxc * dxc + dyc * dyc;
+        disp_of(d, i, q.x, q.y, nd, cd);
     }
-    disp_epilogue(d, tile, nd, cd, method, thr, cthr, par);
+    disp_epilogue(d, nb, tile, nd, cd, pos & 1, stamp(e, pos));
 }
 
-// kick_mode: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
-__global__ __launch_bounds__(BLOCK) void k_fire_force(Dev d, PairParams p, int par, int cpos,
-                                                      int kick_mode) {
+// kick: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
+__global__ __launch_bounds__(BLOCK) void k_fire_force(Dev d, PairParams p, int pos, int kick,
+                                                      int gate) {
     Ctl* c = d.ctl;
-    if (c->halt) return;
     const FireCtl* F = d.fire;
-    if (F->done) return;
-    if (par >= 0 && c->flag[par]) {
-        if (threadIdx.x == 0) {
-            c->halt = 1;
-            c->halt_pos = cpos;
+    if (gate == G_NORMAL) {
+        if (c->halt || F->done) return;
+        int need = pending_rebuild(c, c->epoch, pos);
+        if (need) {
+            if (threadIdx.x == 0) {
+                c->halt = 1;
+                c->halt_pos = pos;
+                c->halt_mode = H_FIRE;
+                c->halt_kind = need;
+            }
+            return;
         }
-        return;
+    } else if (gate == G_RESUME) {
+        if (!c->halt || !resumable(d) || c->halt_mode != H_FIRE) return;
     }
     double half = F->dt / 2.0;
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double pxv = 0.0, pyv = 0.0, pe = 0.0, f2 = 0.0;
     if (i < d.N) {
+        double4 qi = d.P0[i];
         double ax, ay, e;
-        disk_row(d, p, i, ax, ay, e);
+        disk_row(d, d.P0, p, i, qi, ax, ay, e);
         double fx = 0.0 + ax, fy = 0.0 + ay;
         d.frc[i] = make_double2(fx, fy);
         pe = 0.0 + e;
         d.pe[i] = pe;
         double2 v = d.vel[i];
-        if (kick_mode) {
-            double dtm = half / d.mass[i];
+        if (kick) {
+            double dtm = half / qi.w;
             v.x += fx * dtm;
             v.y += fy * dtm;
             d.vel[i] = v;
@@ -699,20 +1064,26 @@ __device__ __forceinline__ void fire_finish(const Dev& d, FireCtl* F) {
     d.ctl->done = 1;
 }
 
-__global__ void k_fire_ctl(Dev d, int init) {
-    if (d.ctl->halt) return;
+// init: 1 = minimizeFire's prologue (records the first average energy)
+__global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
+    Ctl* c = d.ctl;
     FireCtl* F = d.fire;
-    if (F->done) return;
+    if (gate == G_NORMAL) {
+        if (c->halt || F->done) return;
+    } else if (gate == G_RESUME) {
+        if (!c->halt || !resumable(d) || c->halt_mode != H_FIRE) return;
+    }
     int64_t nc = (d.N + CHUNK - 1) / CHUNK;
-    int64_t nb = (d.N + BLOCK - 1) / BLOCK;
+    int64_t nblk = (d.N + BLOCK - 1) / BLOCK;
     const double* P = d.part;
     double px = wave_final_sum(P, nc);
     double py = wave_final_sum(P + nc, nc);
     double pe = wave_final_sum(P + 2 * nc, nc);
     double fm = 0.0;
-    for (int64_t k = threadIdx.x; k < nb; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
+    for (int64_t k = threadIdx.x; k < nblk; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
     fm = wave_max(fm);
     if (threadIdx.x != 0) return;
+    if (!init) c->progress = (gate == G_RESUME) ? c->halt_pos + 1 : pos + 1;
     double avg = pe / F->dof;
     F->fmax_sq = fm;
     F->avg_pe = avg;
@@ -769,18 +1140,14 @@ __global__ void k_fire_ctl(Dev d, int init) {
 // =========================================================================
 //                             host-side helpers
 // =========================================================================
-using namespace dpm;
+namespace dpm {
 
-namespace {
-
-template <class T>
-int dalloc(dpm_system* s, T** p, int64_t n) {
-    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+int dalloc_bytes(dpm_system* s, void** p, size_t bytes) {
     void* q = nullptr;
     DPM_HIP(hipMalloc(&q, bytes));
     DPM_HIP(hipMemset(q, 0, bytes));
     s->allocs.push_back(q);
-    *p = (T*)q;
+    *p = q;
     return 0;
 }
 
@@ -791,9 +1158,9 @@ void dfree(dpm_system* s, void* p) {
     (void)hipFree(p);
 }
 
-int64_t pow2_at_least(int64_t m) { return (int64_t)std::pow(2.0, std::ceil(std::log2((double)m))); }
-
-inline dim3 g1(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK)); }
+int64_t pow2_at_least(int64_t m) {
+    return (int64_t)std::pow(2.0, std::ceil(std::log2((double)m)));
+}
 
 int check_launch() {
     hipError_t e = hipGetLastError();
@@ -804,17 +1171,56 @@ int check_launch() {
     return 0;
 }
 
-PairParams params_of(dpm_system* s) {
-    if (s->kind == DPM_BUMPY) return pair_params(s, s->e_c, s->n_c);
-    return pair_params(s, s->e_c, s->n_c);
+// Host<->device copies ordered on the system stream (the stream is non-blocking,
+// so the legacy null stream would not wait for kernels queued on it).
+int scopy(dpm_system* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    DPM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s->stream));
+    DPM_HIP(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
-}  // namespace
+// The fast minimum image is exact when L/2 exceeds every contact and list cutoff.
+static bool fast_ok(const dpm_system* s) {
+    double lim = 2.0 * s->rmax;
+    if (s->method == DPM_NBR_ALL) return false;  // its cutoff is 10 box diagonals
+    if (s->method == DPM_NBR_CELL || s->method == DPM_NBR_VERLET) lim = std::max(lim, s->rc);
+    if (s->kind == DPM_BUMPY) lim = std::max({lim, 2.0 * s->rv, s->rc_v, s->pcut_v});
+    double half = 0.5 * std::min(s->box[0], s->box[1]);
+    return half > lim * (1.0 + 1e-9);
+}
 
-namespace dpm {
+PairParams params_of(const dpm_system* s) {
+    PairParams p = pair_params(s, s->e_c, s->n_c);
+    p.fast = fast_ok(s) ? 1 : 0;
+    if (s->method == DPM_NBR_ALL)  // pair forces still only need L/2 > largest contact
+        p.fast = 0.5 * std::min(s->box[0], s->box[1]) > 2.0 * s->rmax * (1.0 + 1e-9) ? 1 : 0;
+    return p;
+}
+
+NbrParams nbr_params(const dpm_system* s) {
+    NbrParams nb{};
+    nb.method = s->method;
+    nb.ncd = (int)std::max<int64_t>(s->ncd, 1);
+    nb.ncells = s->ncells;
+    nb.fast = fast_ok(s) ? 1 : 0;
+    nb.rc2 = s->rc * s->rc;
+    nb.thr = s->thr_sq;
+    nb.cthr = s->cell_thr_sq;
+    nb.cs = s->cell_size;
+    nb.Lx = s->box[0];
+    nb.Ly = s->box[1];
+    nb.invLx = 1.0 / s->box[0];
+    nb.invLy = 1.0 / s->box[1];
+    nb.halfLx = 0.5 * s->box[0];
+    nb.halfLy = 0.5 * s->box[1];
+    nb.vc2 = s->rc_v * s->rc_v;
+    nb.vp2 = s->pcut_v * s->pcut_v;
+    return nb;
+}
 
 int launch_chunk_sum(dpm_system* s, int what, double* out_dev) {
-    hipLaunchKernelGGL(k_chunk_sum, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, what, s->d.part);
+    hipLaunchKernelGGL(k_chunk_sum, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, what,
+                       s->d.part);
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(WAVE), 0, s->stream, s->d.part, s->nchunk,
                        out_dev);
     return check_launch();
@@ -823,9 +1229,7 @@ int launch_chunk_sum(dpm_system* s, int what, double* out_dev) {
 int sum_to_host(dpm_system* s, int what, double* out) {
     double* dv = &s->d.ctl->scal[0];
     if (launch_chunk_sum(s, what, dv)) return -1;
-    DPM_HIP(hipMemcpyAsync(out, dv, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
+    return scopy(s, out, dv, sizeof(double), hipMemcpyDeviceToHost);
 }
 
 int max_to_host(dpm_system* s, int what, double* out) {
@@ -836,144 +1240,18 @@ int max_to_host(dpm_system* s, int what, double* out) {
     else if (what == M_CD) src = s->d.pmax_cd;
     else {
         double* tmp = s->d.part;
-        hipLaunchKernelGGL(k_block_max, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, what, tmp);
+        hipLaunchKernelGGL(k_block_max, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, what, tmp);
         src = tmp;
     }
     hipLaunchKernelGGL(k_final_max, dim3(1), dim3(BLOCK), 0, s->stream, src, n, dv);
     if (check_launch()) return -1;
-    DPM_HIP(hipMemcpyAsync(out, dv, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
+    return scopy(s, out, dv, sizeof(double), hipMemcpyDeviceToHost);
 }
-
-}  // namespace dpm
-
-// defined in dpm_bumpy.hip
-namespace dpm {
-int bumpy_alloc(dpm_system* s);
-int bumpy_forces(dpm_system* s, int accumulate);
-int bumpy_vertex_build(dpm_system* s);
-int bumpy_cell_reorder_vertices(dpm_system* s);
-int bumpy_kick(dpm_system* s, double dt);
-int bumpy_drift(dpm_system* s, double dt);
-int bumpy_mix(dpm_system* s, double alpha);
-int bumpy_stress(dpm_system* s);
-int bumpy_contacts(dpm_system* s);
-int bumpy_nve_run(dpm_system* s, double dt, int64_t n);
-int bumpy_minimize_fire(dpm_system* s, const dpm_fire_params* p, dpm_fire_result* out);
-int bumpy_fire_step(dpm_system* s, dpm_fire_state* st);
-int bumpy_snapshot(dpm_system* s, bool save);
-int bumpy_download(dpm_system* s, int field, void* data, int64_t count, const std::vector<int>& sidx);
-int bumpy_upload(dpm_system* s, int field, const void* data, int64_t count, const std::vector<int>& sidx);
-}  // namespace dpm
-
-// =========================================================================
-//                          neighbour management
-// =========================================================================
-namespace dpm {
-
-static int ensure_ell(dpm_system* s, int64_t cap) {
-    if (cap <= s->phys_cap) return 0;
-    dfree(s, s->d.nbr);
-    s->d.nbr = nullptr;
-    if (dalloc(s, &s->d.nbr, s->N * cap)) return -1;
-    s->phys_cap = cap;
-    s->d.cap = (int)cap;
-    s->gen++;
-    return 0;
-}
-
-// Particle::updateVerletList / updateCellNeighborList with the reference's
-// overflow regrowth (particle.cu:1119-1131, :1320-1331).
-int neighbor_build(dpm_system* s) {
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        DPM_HIP(hipMemsetAsync(&s->d.ctl->max_nnum, 0, sizeof(int), s->stream));
-        if (s->method == DPM_NBR_CELL) {
-            hipLaunchKernelGGL(k_build_cell, g1(s->N), dim3(BLOCK), 0, s->stream, s->d,
-                               s->rc * s->rc, s->box[0], s->box[1], s->ncd, s->phys_cap);
-        } else {
-            hipLaunchKernelGGL(k_build_verlet, g1(s->N), dim3(BLOCK), 0, s->stream, s->d,
-                               s->rc * s->rc, s->box[0], s->box[1], s->phys_cap);
-        }
-        if (check_launch()) return -1;
-        int mx = 0;
-        DPM_HIP(hipMemcpyAsync(&mx, &s->d.ctl->max_nnum, sizeof(int), hipMemcpyDeviceToHost,
-                               s->stream));
-        DPM_HIP(hipStreamSynchronize(s->stream));
-        if (mx > s->max_alloc) s->max_alloc = pow2_at_least(mx);
-        if (mx > s->phys_cap) {
-            if (ensure_ell(s, std::max<int64_t>(s->max_alloc, pow2_at_least(mx)))) {
-                s->nbr_ready = false;
-                return -1;
-            }
-            continue;
-        }
-        s->n_rebuilds++;
-        if (s->kind == DPM_BUMPY) return bumpy_vertex_build(s);
-        return 0;
-    }
-    s->nbr_ready = false;
-    set_error("neighbor_build: capacity did not converge");
-    return -1;
-}
-
-// Particle::updateCellList + reorderParticleData (particle.cu:1292-1317)
-int cell_rebuild(dpm_system* s) {
-    const Dev& d = s->d;
-    hipLaunchKernelGGL(k_fill_int, g1(s->ncells), dim3(BLOCK), 0, s->stream, d.cell_cnt, 0,
-                       s->ncells);
-    hipLaunchKernelGGL(k_cell_bin, g1(s->N), dim3(BLOCK), 0, s->stream, d, s->cell_size, s->ncd);
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s->stream, d, s->ncells);
-    hipLaunchKernelGGL(k_cell_scatter, g1(s->N), dim3(BLOCK), 0, s->stream, d);
-    hipLaunchKernelGGL(k_cell_sort, g1(s->ncells), dim3(BLOCK), 0, s->stream, d, s->ncells);
-    hipLaunchKernelGGL(k_reorder_gather, g1(s->N), dim3(BLOCK), 0, s->stream, d, s->sp);
-    if (s->kind == DPM_BUMPY && bumpy_cell_reorder_vertices(s)) return -1;
-    hipLaunchKernelGGL(k_reorder_copyback, g1(s->N), dim3(BLOCK), 0, s->stream, d, s->sp);
-    hipLaunchKernelGGL(k_fill_double, g1(s->nblk), dim3(BLOCK), 0, s->stream, d.pmax_cd, 0.0,
-                       s->nblk);
-    s->n_cell_rebuilds++;
-    return check_launch();
-}
-
-// Particle::checkForNeighborUpdate (particle.cu:1137-1161): returns 0 none,
-// 1 neighbour rebuild, 2 cell + neighbour rebuild.
-int rebuild_kind(dpm_system* s, int* kind) {
-    *kind = 0;
-    if (s->method != DPM_NBR_VERLET && s->method != DPM_NBR_CELL) return 0;
-    double mn = 0.0, mc = 0.0;
-    if (max_to_host(s, M_ND, &mn)) return -1;
-    if (s->method == DPM_NBR_CELL) {
-        if (max_to_host(s, M_CD, &mc)) return -1;
-        if (3.0 * mc > s->cell_thr_sq) *kind = 2;
-        else if (3.0 * mn > s->thr_sq) *kind = 1;
-    } else if (3.0 * mn > s->thr_sq) {
-        *kind = 1;
-    }
-    return 0;
-}
-
-int do_rebuild(dpm_system* s, int kind) {
-    if (kind == 2 && cell_rebuild(s)) return -1;
-    if (kind >= 1) return neighbor_build(s);
-    return 0;
-}
-
-int check_and_rebuild(dpm_system* s, int* kind_out) {
-    int kind = 0;
-    if (rebuild_kind(s, &kind)) return -1;
-    if (do_rebuild(s, kind)) return -1;
-    if (kind_out) *kind_out = kind;
-    return 0;
-}
-
-}  // namespace dpm
 
 // =========================================================================
 //                         timing (per-launch HIP events)
 // =========================================================================
-namespace {
-
-hipEvent_t pool_event(dpm_system* s) {
+static hipEvent_t pool_event(dpm_system* s) {
     if (s->ev_next >= s->ev_pool.size()) {
         hipEvent_t e;
         (void)hipEventCreate(&e);
@@ -982,22 +1260,18 @@ hipEvent_t pool_event(dpm_system* s) {
     return s->ev_pool[s->ev_next++];
 }
 
-struct Timed {
-    dpm_system* s;
-    int cls;
-    hipEvent_t a = nullptr, b = nullptr;
-    Timed(dpm_system* s_, int cls_) : s(s_), cls(cls_) {
-        if (!s->timing) return;
-        a = pool_event(s);
-        b = pool_event(s);
-        (void)hipEventRecord(a, s->stream);
-    }
-    ~Timed() {
-        if (!s->timing) return;
-        (void)hipEventRecord(b, s->stream);
-        s->ev_pending.push_back({cls, {a, b}});
-    }
-};
+Timed::Timed(dpm_system* s_, int cls_) : s(s_), cls(cls_) {
+    if (!s->timing) return;
+    a = pool_event(s);
+    b = pool_event(s);
+    (void)hipEventRecord(a, s->stream);
+}
+
+Timed::~Timed() {
+    if (!s->timing) return;
+    (void)hipEventRecord(b, s->stream);
+    s->ev_pending.push_back({cls, {a, b}});
+}
 
 void collect_timing(dpm_system* s) {
     if (s->ev_pending.empty()) return;
@@ -1013,29 +1287,144 @@ void collect_timing(dpm_system* s) {
     s->ev_next = 0;
 }
 
-}  // namespace
-
 // =========================================================================
-//                               NVE loop
+//                 neighbour management (chain, rebuilds, capacity)
 // =========================================================================
-namespace dpm {
+int ensure_ell(dpm_system* s, int64_t cap) {
+    if (cap <= s->phys_cap) return 0;
+    dfree(s, s->d.nbr);
+    s->d.nbr = nullptr;
+    if (dalloc(s, &s->d.nbr, s->N * cap)) return -1;
+    s->phys_cap = cap;
+    s->d.cap = (int)cap;
+    s->gen++;
+    return 0;
+}
 
-static void nve_step_launch(dpm_system* s, const PairParams& p, double dt, int par, int cpos) {
-    {
-        Timed t(s, DPM_K_DRIFT);
-        hipLaunchKernelGGL(k_nve_kick_drift, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, 0.5 * dt,
-                           dt, s->method, s->thr_sq, s->cell_thr_sq, par);
+// The rebuild kernels (cell binning, scan, scatter, stable fix-up, reorder, list
+// build) in stream order; with `gated` each one consults the chain's decision.
+static void enqueue_rebuild(dpm_system* s, int gated) {
+    const Dev& d = s->d;
+    NbrParams nb = nbr_params(s);
+    hipStream_t st = s->stream;
+    if (s->method == DPM_NBR_CELL) {
+        long long nc = s->ncells;
+        long long nsb = (nc + SCAN_TILE - 1) / SCAN_TILE;
+        const Ctl* g = gated ? d.ctl : nullptr;
+        hipLaunchKernelGGL(k_fill_int, pgrid(nc), dim3(BLOCK), 0, st, d.cell_cnt, 0,
+                           (int64_t)nc, g);
+        hipLaunchKernelGGL(k_cell_bin, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
+        hipLaunchKernelGGL(k_scan_a, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, gated);
+        hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, st, d, nsb, gated);
+        hipLaunchKernelGGL(k_scan_c, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, gated);
+        hipLaunchKernelGGL(k_cell_scatter, pgrid(s->N), dim3(BLOCK), 0, st, d, gated);
+        hipLaunchKernelGGL(k_cell_sort, pgrid(nc), dim3(BLOCK), 0, st, d, nc, gated);
+        hipLaunchKernelGGL(k_reorder_gather, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
+        if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_reorder(s, gated);
+        hipLaunchKernelGGL(k_reorder_copyback, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
+        hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
+    } else {
+        hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
     }
-    {
-        Timed t(s, DPM_K_FORCE);
-        hipLaunchKernelGGL(k_nve_force_kick, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p,
-                           0.5 * dt, par, cpos);
+    if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_build(s, gated);
+}
+
+// The halted step's remaining work. resume: 0 none, 1 NVE, 2 FIRE.
+static void enqueue_resume(dpm_system* s, int resume, double dt) {
+    const Dev& d = s->d;
+    NbrParams nb = nbr_params(s);
+    PairParams p = params_of(s);
+    hipStream_t st = s->stream;
+    if (resume == 1) {
+        hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, 0, 0,
+                           (int)G_RESUME);
+        hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 0.5 * dt, dt,
+                           0, 1, -1, (int)G_RESUME);
+    } else if (resume == 2) {
+        hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 1,
+                           (int)G_RESUME);
+        hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, 0, (int)G_RESUME);
     }
 }
 
-static int ensure_graph(dpm_system* s, Graph& g, double dt, int steps,
-                        void (*fill)(dpm_system*, double, int)) {
-    if (g.exec && g.gen == s->gen && g.dt == dt && g.steps == steps) return 0;
+// End of a chunk: decide, rebuild and finish a halted step, all on the device.
+static void enqueue_chain(dpm_system* s, int resume, double dt) {
+    const Dev& d = s->d;
+    hipStream_t st = s->stream;
+    if (s->method != DPM_NBR_CELL && s->method != DPM_NBR_VERLET) {
+        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, 0);
+        return;
+    }
+    Timed t(s, DPM_K_REBUILD);
+    hipLaunchKernelGGL(k_chain_decide, dim3(1), dim3(BLOCK), 0, st, d, nbr_params(s), 0);
+    if (resume == 1) hipLaunchKernelGGL(k_chain_copy_cur, pgrid(s->N), dim3(BLOCK), 0, st, d);
+    enqueue_rebuild(s, 1);
+    enqueue_resume(s, resume, dt);
+    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, 0);
+}
+
+// Host-side fix after a capacity overflow: regrow the ELL to 2^ceil(log2 max) (the
+// reference's rule, particle.cu:1122-1130), rebuild the list and (for a halted
+// loop) finish the halted step.
+static int fix_overflow(dpm_system* s, int resume, double dt) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        Ctl c;
+        if (scopy(s, &c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+        if (!c.overflow) return 0;
+        if (c.max_nnum > s->phys_cap && ensure_ell(s, pow2_at_least(c.max_nnum))) return -1;
+        if (s->kind == DPM_BUMPY && c.max_vnum > s->vphys_cap &&
+            bumpy_ensure_vcap(s, pow2_at_least(c.max_vnum)))
+            return -1;
+        const Dev& d = s->d;
+        NbrParams nb = nbr_params(s);
+        hipStream_t st = s->stream;
+        hipLaunchKernelGGL(k_overflow_begin, dim3(1), dim3(1), 0, st, d);
+        if (s->method == DPM_NBR_CELL)
+            hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 1);
+        else
+            hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 1);
+        if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_build(s, 1);
+        enqueue_resume(s, resume, dt);
+        // halted loops finish like a chunk end; stand-alone rebuilds only check
+        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, resume ? 0 : 1);
+        if (check_launch()) return -1;
+    }
+    set_error("neighbour capacity did not converge");
+    return -1;
+}
+
+// Stand-alone rebuild: mode 1 = Particle::checkForNeighborUpdate, 2 = forced.
+// kind_out: 0 none, 1 list, 2 cells + list.
+int device_rebuild(dpm_system* s, int mode, int* kind_out) {
+    if (kind_out) *kind_out = 0;
+    if (s->method == DPM_NBR_NONE) return 0;
+    if (s->method == DPM_NBR_ALL && mode == 1) return 0;  // checkForAllToAllUpdate: no-op
+    hipLaunchKernelGGL(k_chain_decide, dim3(1), dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
+                       mode);
+    enqueue_rebuild(s, 1);
+    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, s->stream, s->d, mode);
+    if (check_launch()) return -1;
+    Ctl c;
+    if (scopy(s, &c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+    if (c.overflow && fix_overflow(s, 0, 0.0)) {
+        s->nbr_ready = false;
+        return -1;
+    }
+    if (kind_out) *kind_out = c.kind;
+    return 0;
+}
+
+int check_and_rebuild(dpm_system* s, int* kind_out) { return device_rebuild(s, 1, kind_out); }
+
+// =========================================================================
+//                         chunked device-resident loops
+// =========================================================================
+typedef void (*ChunkFn)(dpm_system*, double, int64_t);
+
+static int ensure_graph(dpm_system* s, std::map<int64_t, Graph>& cache, int64_t steps, double dt,
+                        ChunkFn fill) {
+    Graph& g = cache[steps];
+    if (g.exec && g.gen == s->gen && g.dt == dt) return 0;
     if (g.exec) {
         (void)hipGraphExecDestroy(g.exec);
         (void)hipGraphDestroy(g.graph);
@@ -1047,105 +1436,112 @@ static int ensure_graph(dpm_system* s, Graph& g, double dt, int steps,
     DPM_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     g.gen = s->gen;
     g.dt = dt;
-    g.steps = steps;
     return 0;
 }
 
-static void nve_fill(dpm_system* s, double dt, int steps) {
+// c NVE steps (positions 0..c-1) + the chain.  An odd c splits its last step into
+// force + kick-drift so every chunk ends with the positions in P0.
+static void nve_chunk(dpm_system* s, double dt, int64_t c) {
+    const Dev& d = s->d;
     PairParams p = params_of(s);
-    for (int c = 0; c < steps; ++c) nve_step_launch(s, p, dt, c & 1, c);
+    NbrParams nb = nbr_params(s);
+    hipStream_t st = s->stream;
+    int64_t fused = (c % 2) ? c - 1 : c;
+    for (int64_t k = 0; k < fused; ++k) {
+        Timed t(s, DPM_K_FORCE);
+        hipLaunchKernelGGL(k_nve_fused, pgrid(s->N), dim3(BLOCK), 0, st, d, p, nb, 0.5 * dt, dt,
+                           (int)(k & 1), (int)k, (int)(k == c - 1));
+    }
+    if (c % 2) {
+        int k = (int)(c - 1);
+        {
+            Timed t(s, DPM_K_FORCE);
+            hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, k,
+                               (int)H_NVE_FUSED, (int)G_NORMAL);
+        }
+        Timed t(s, DPM_K_DRIFT);
+        hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 0.5 * dt, dt,
+                           0, 1, k + 1, (int)G_NORMAL);
+    }
+    enqueue_chain(s, 1, dt);
 }
 
-static int clear_ctl(dpm_system* s) {
-    DPM_HIP(hipMemsetAsync(s->d.ctl, 0, offsetof(Ctl, max_nnum), s->stream));
-    return 0;
+static void fire_chunk(dpm_system* s, double, int64_t c) {
+    const Dev& d = s->d;
+    PairParams p = params_of(s);
+    NbrParams nb = nbr_params(s);
+    hipStream_t st = s->stream;
+    for (int64_t k = 0; k < c; ++k) {
+        {
+            Timed t(s, DPM_K_DRIFT);
+            hipLaunchKernelGGL(k_fire_move, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, (int)k);
+        }
+        {
+            Timed t(s, DPM_K_FORCE);
+            hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p,
+                               (int)k, 1, (int)G_NORMAL);
+        }
+        hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, (int)k, (int)G_NORMAL);
+    }
+    enqueue_chain(s, 2, 0.0);
 }
 
-// Drives `n` steps of a chunked, halt-on-rebuild device loop.  `launch`
-// enqueues k steps (graph or direct); `resume` finishes a halted step after the
-// host-side rebuild; `finished` tells whether the device loop ended early
-// (FIRE convergence).
-struct LoopOps {
-    int (*launch)(dpm_system*, double, int64_t);
-    int (*resume)(dpm_system*, double);
-};
-
-static int run_chunked(dpm_system* s, double dt, int64_t n, const LoopOps& ops) {
-    if (clear_ctl(s)) return -1;
+// Runs `total` steps in chunks with up to two chunks in flight.  After each chunk
+// the control block is copied to pinned memory; the host counts the chunk's
+// completed steps and intervenes only on a capacity overflow.  With
+// `stop_on_done` the loop ends when the device reports completion (FIRE).
+static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, ChunkFn chunk_fn,
+                      std::map<int64_t, Graph>& cache, bool stop_on_done) {
     struct Inflight {
-        int64_t k;
+        int64_t c;
         int slot;
     };
-    static thread_local hipEvent_t evs[2] = {nullptr, nullptr};
-    if (!evs[0]) {
-        DPM_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-        DPM_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-    }
     std::deque<Inflight> q;
-    int64_t done = 0, launched = 0;
-    while (done < n) {
-        while (q.size() < 2 && launched < n) {
-            int64_t k = std::min<int64_t>(s->chunk, n - launched);
-            int slot = q.empty() ? 0 : (q.front().slot ^ 1);
-            if (ops.launch(s, dt, k)) return -1;
+    int64_t done = 0, planned = 0;
+    int next_slot = 0;
+    while (done < total) {
+        while (q.size() < 2 && planned < total) {
+            int64_t c = std::min<int64_t>(s->chunk, total - planned);
+            if (s->use_graph && !s->timing && c == s->chunk) {
+                if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
+                DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
+            } else {
+                chunk_fn(s, dt, c);
+                if (check_launch()) return -1;
+            }
+            int slot = next_slot;
+            next_slot = (next_slot + 1) % 4;
             DPM_HIP(hipMemcpyAsync(&s->hctl[slot], s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
                                    s->stream));
-            DPM_HIP(hipEventRecord(evs[slot], s->stream));
-            q.push_back({k, slot});
-            launched += k;
+            DPM_HIP(hipEventRecord(s->poll_ev[slot], s->stream));
+            q.push_back({c, slot});
+            planned += c;
         }
+        if (q.empty()) break;
         Inflight f = q.front();
         q.pop_front();
-        DPM_HIP(hipEventSynchronize(evs[f.slot]));
+        DPM_HIP(hipEventSynchronize(s->poll_ev[f.slot]));
         if (s->timing) collect_timing(s);
         Ctl hc = s->hctl[f.slot];
-        if (hc.halt) {
+        if (hc.overflow) {
+            // the halted step is unfinished and every later chunk in flight was a no-op
             DPM_HIP(hipStreamSynchronize(s->stream));
-            done += hc.halt_pos;
-            int kind = 0;
-            if (rebuild_kind(s, &kind)) return -1;
-            if (kind == 0) kind = 1;  // the flag was raised by the same test
-            {
-                Timed t(s, DPM_K_REBUILD);
-                if (do_rebuild(s, kind)) return -1;
-            }
-            if (clear_ctl(s)) return -1;
-            if (ops.resume(s, dt)) return -1;
-            done += 1;
-            launched = done;
-            q.clear();
+            if (fix_overflow(s, resume, dt)) return -1;
             Ctl after;
-            DPM_HIP(hipMemcpyAsync(&after, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
-                                   s->stream));
-            DPM_HIP(hipStreamSynchronize(s->stream));
-            if (after.done) break;
+            if (scopy(s, &after, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+            done += after.last_progress;
+            q.clear();
+            planned = done;
+            if (stop_on_done && after.done) break;
             continue;
         }
-        done += f.k;
-        if (hc.done) break;
+        done += hc.last_progress;
+        planned -= (f.c - hc.last_progress);
+        if (stop_on_done && hc.done) break;
     }
     DPM_HIP(hipStreamSynchronize(s->stream));
     if (s->timing) collect_timing(s);
     return 0;
-}
-
-static int nve_launch(dpm_system* s, double dt, int64_t k) {
-    if (s->use_graph && !s->timing && k == s->chunk && (k % 2) == 0) {
-        if (ensure_graph(s, s->nve_graph, dt, (int)k, nve_fill)) return -1;
-        DPM_HIP(hipGraphLaunch(s->nve_graph.exec, s->stream));
-        return 0;
-    }
-    PairParams p = params_of(s);
-    for (int64_t c = 0; c < k; ++c) nve_step_launch(s, p, dt, (int)(c & 1), (int)c);
-    return check_launch();
-}
-
-static int nve_resume(dpm_system* s, double dt) {
-    PairParams p = params_of(s);
-    Timed t(s, DPM_K_FORCE);
-    hipLaunchKernelGGL(k_nve_force_kick, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p, 0.5 * dt,
-                       -1, 0);
-    return check_launch();
 }
 
 int nve_run(dpm_system* s, double dt, int64_t n) {
@@ -1154,52 +1550,30 @@ int nve_run(dpm_system* s, double dt, int64_t n) {
         set_error("nve_run: neighbour list not initialised");
         return -1;
     }
-    LoopOps ops{nve_launch, nve_resume};
-    return run_chunked(s, dt, n, ops);
-}
-
-// =========================================================================
-//                               FIRE loop
-// =========================================================================
-static void fire_step_launch(dpm_system* s, const PairParams& p, int par, int cpos) {
-    {
+    if (n <= 0) return 0;
+    const Dev& d = s->d;
+    hipStream_t st = s->stream;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, d, 0);
+    {  // opening kick + drift of the first step; its request is checked at position 0
         Timed t(s, DPM_K_DRIFT);
-        hipLaunchKernelGGL(k_fire_move, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, s->method,
-                           s->thr_sq, s->cell_thr_sq, par);
+        hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nbr_params(s),
+                           0.5 * dt, dt, 0, 0, -1, (int)G_NORMAL);
     }
-    {
+    if (check_launch()) return -1;
+    if (n > 1 && run_chunks(s, n - 1, dt, 1, nve_chunk, s->nve_graphs, false)) return -1;
+    {  // closing force + kick of the last step
         Timed t(s, DPM_K_FORCE);
-        hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d,
-                           p, par, cpos, 1);
+        hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, params_of(s),
+                           0.5 * dt, 0, (int)H_NVE_FORCE, (int)G_NORMAL);
     }
-    hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, s->stream, s->d, 0);
-}
-
-static void fire_fill(dpm_system* s, double, int steps) {
-    PairParams p = params_of(s);
-    for (int c = 0; c < steps; ++c) fire_step_launch(s, p, c & 1, c);
-}
-
-static int fire_launch(dpm_system* s, double dt, int64_t k) {
-    if (s->use_graph && !s->timing && k == s->chunk && (k % 2) == 0) {
-        if (ensure_graph(s, s->fire_graph, 0.0, (int)k, fire_fill)) return -1;
-        DPM_HIP(hipGraphLaunch(s->fire_graph.exec, s->stream));
-        return 0;
-    }
-    PairParams p = params_of(s);
-    for (int64_t c = 0; c < k; ++c) fire_step_launch(s, p, (int)(c & 1), (int)c);
-    return check_launch();
-}
-
-static int fire_resume(dpm_system* s, double) {
-    PairParams p = params_of(s);
-    {
-        Timed t(s, DPM_K_FORCE);
-        hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d,
-                           p, -1, 0, 1);
-    }
-    hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, s->stream, s->d, 0);
-    return check_launch();
+    enqueue_chain(s, 1, dt);
+    if (check_launch()) return -1;
+    Ctl c;
+    if (scopy(s, &c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+    if (c.overflow && fix_overflow(s, 1, dt)) return -1;
+    DPM_HIP(hipStreamSynchronize(st));
+    if (s->timing) collect_timing(s);
+    return 0;
 }
 
 int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
@@ -1220,20 +1594,21 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
     f.fmax_target_sq = prm->fmax_target > 0 ? prm->fmax_target * prm->fmax_target : 0.0;
     f.dof = (double)s->N;
     f.max_steps = prm->max_steps;
-    DPM_HIP(hipMemcpyAsync(s->d.fire, &f, sizeof(FireCtl), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_zero_velocities, g1(s->N), dim3(BLOCK), 0, s->stream, s->d);
+    const Dev& d = s->d;
+    hipStream_t st = s->stream;
+    DPM_HIP(hipMemcpyAsync(d.fire, &f, sizeof(FireCtl), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, d, 0);
+    hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (check_and_rebuild(s, nullptr)) return -1;
-    if (clear_ctl(s)) return -1;
-    PairParams p = params_of(s);
-    hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d, p,
-                       -1, 0, 0);
-    hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, s->stream, s->d, 1);
+    hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, params_of(s),
+                       0, 0, (int)G_ALWAYS);
+    hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 1, 0, (int)G_ALWAYS);
     if (check_launch()) return -1;
-    LoopOps ops{fire_launch, fire_resume};
-    if (prm->max_steps > 0 && run_chunked(s, 0.0, prm->max_steps, ops)) return -1;
-    hipLaunchKernelGGL(k_zero_velocities, g1(s->N), dim3(BLOCK), 0, s->stream, s->d);
-    DPM_HIP(hipMemcpyAsync(&f, s->d.fire, sizeof(FireCtl), hipMemcpyDeviceToHost, s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
+    if (prm->max_steps > 0 &&
+        run_chunks(s, prm->max_steps, 0.0, 2, fire_chunk, s->fire_graphs, true))
+        return -1;
+    hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
+    if (scopy(s, &f, d.fire, sizeof(FireCtl), hipMemcpyDeviceToHost)) return -1;
     if (out) {
         out->steps = std::min<int64_t>(f.step, prm->max_steps);
         out->stopped = f.stopped;
@@ -1248,9 +1623,9 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
     return check_launch();
 }
 
-// One Fire::step with host-held state (src/integrator/fire.cu:13-46), built from
-// the single kernels; used by state-machine trace tests.
-int fire_step_host(dpm_system* s, dpm_fire_state* st);
+void reset_ctl(dpm_system* s, int config) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s->stream, s->d, config);
+}
 
 }  // namespace dpm
 
@@ -1261,20 +1636,9 @@ using namespace dpm;
 
 namespace {
 
-// Host<->device copies ordered on the system stream (the stream is non-blocking,
-// so the legacy null stream would not wait for kernels queued on it).
-int scopy(dpm_system* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
-    DPM_HIP(hipMemcpyAsync(dst, src, bytes, kind, s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
-}
-
 int fetch_sidx(dpm_system* s, std::vector<int>& sidx) {
     sidx.resize(s->N);
-    DPM_HIP(hipMemcpyAsync(sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost,
-                           s->stream));
-    DPM_HIP(hipStreamSynchronize(s->stream));
-    return 0;
+    return scopy(s, sidx.data(), s->d.sidx, sizeof(int) * s->N, hipMemcpyDeviceToHost);
 }
 
 int require_nbr(dpm_system* s, const char* who) {
@@ -1304,7 +1668,49 @@ int ensure_stress(dpm_system* s) {
     return 0;
 }
 
+int64_t ndof_of(dpm_system* s) {
+    if (s->kind == DPM_BUMPY) return s->rotation ? 3 * s->N : 2 * s->N;
+    return 2 * s->N;
+}
+
+// Per-particle fields: 'P' = component of P0 (radius and mass also mirrored in P1),
+// 'V'/'F' = component of vel/frc, 'D' = a plain double array.
+bool field_slot(dpm_system* s, int f, char* kind, int* comp, double** plain) {
+    *plain = nullptr;
+    *comp = 0;
+    switch (f) {
+        case DPM_F_X: *kind = 'P'; *comp = 0; return true;
+        case DPM_F_Y: *kind = 'P'; *comp = 1; return true;
+        case DPM_F_RAD: *kind = 'P'; *comp = 2; return true;
+        case DPM_F_MASS: *kind = 'P'; *comp = 3; return true;
+        case DPM_F_VX: *kind = 'V'; *comp = 0; return true;
+        case DPM_F_VY: *kind = 'V'; *comp = 1; return true;
+        case DPM_F_FX: *kind = 'F'; *comp = 0; return true;
+        case DPM_F_FY: *kind = 'F'; *comp = 1; return true;
+        case DPM_F_PE: *kind = 'D'; *plain = s->d.pe; return true;
+        case DPM_F_KE: *kind = 'D'; *plain = s->d.ke; return true;
+        case DPM_F_STRESS_XX: *kind = 'D'; *plain = s->d.sxx; return *plain != nullptr;
+        case DPM_F_STRESS_XY: *kind = 'D'; *plain = s->d.sxy; return *plain != nullptr;
+        case DPM_F_STRESS_YX: *kind = 'D'; *plain = s->d.syx; return *plain != nullptr;
+        case DPM_F_STRESS_YY: *kind = 'D'; *plain = s->d.syy; return *plain != nullptr;
+    }
+    return false;
+}
+
 }  // namespace
+
+namespace dpm {
+// Largest radius in P0 (host copy), used to validate the fast minimum image.
+int refresh_rmax(dpm_system* s) {
+    std::vector<double4> h(s->N);
+    if (scopy(s, h.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost)) return -1;
+    double m = 0.0;
+    for (const auto& q : h) m = std::max(m, q.z);
+    s->rmax = m;
+    s->gen++;
+    return 0;
+}
+}  // namespace dpm
 
 extern "C" {
 
@@ -1322,8 +1728,7 @@ dpm_system* dpm_create(int kind, int64_t n, int64_t nv) {
         set_error("dpm_create: bad arguments");
         return nullptr;
     }
-    int ndev = dpm_device_count();
-    if (ndev <= 0) {
+    if (dpm_device_count() <= 0) {
         set_error("dpm_create: no HIP device");
         return nullptr;
     }
@@ -1338,22 +1743,25 @@ dpm_system* dpm_create(int kind, int64_t n, int64_t nv) {
     Dev& d = s->d;
     d.N = n;
     d.Nv = s->Nv;
-    ok = ok && !dalloc(s, &d.pos, n) && !dalloc(s, &d.vel, n) && !dalloc(s, &d.frc, n) &&
-         !dalloc(s, &d.lref, n) && !dalloc(s, &d.cref, n) && !dalloc(s, &d.rad, n) &&
-         !dalloc(s, &d.mass, n) && !dalloc(s, &d.pe, n) && !dalloc(s, &d.ke, n) &&
-         !dalloc(s, &d.sidx, n) && !dalloc(s, &d.nnum, n) && !dalloc(s, &d.cell_of, n) &&
-         !dalloc(s, &d.cell_tmp, n) && !dalloc(s, &d.perm, n) && !dalloc(s, &d.pmax_nd, s->nblk) &&
-         !dalloc(s, &d.pmax_cd, s->nblk) &&
-         !dalloc(s, &d.part, 4 * s->nchunk + s->nblk + 64) && !dalloc(s, &d.ctl, 1) &&
-         !dalloc(s, &d.fire, 1) && !dalloc(s, &s->sp.pos, n) && !dalloc(s, &s->sp.vel, n) &&
-         !dalloc(s, &s->sp.frc, n) && !dalloc(s, &s->sp.rad, n) && !dalloc(s, &s->sp.mass, n) &&
-         !dalloc(s, &s->sp.sidx, n);
+    ok = ok && !dalloc(s, &d.P0, n) && !dalloc(s, &d.P1, n) && !dalloc(s, &d.vel, n) &&
+         !dalloc(s, &d.frc, n) && !dalloc(s, &d.lref, n) && !dalloc(s, &d.cref, n) &&
+         !dalloc(s, &d.pe, n) && !dalloc(s, &d.ke, n) && !dalloc(s, &d.sidx, n) &&
+         !dalloc(s, &d.nnum, n) && !dalloc(s, &d.cell_of, n) && !dalloc(s, &d.cell_tmp, n) &&
+         !dalloc(s, &d.perm, n) && !dalloc(s, &d.pmax_nd, s->nblk) &&
+         !dalloc(s, &d.pmax_cd, s->nblk) && !dalloc(s, &d.part, 4 * s->nchunk + s->nblk + 64) &&
+         !dalloc(s, &d.ctl, 1) && !dalloc(s, &d.fire, 1) && !dalloc(s, &s->sp.P, n) &&
+         !dalloc(s, &s->sp.vel, n) && !dalloc(s, &s->sp.frc, n) && !dalloc(s, &s->sp.sidx, n);
     if (ok) {
         std::vector<int> id(n);
         for (int64_t k = 0; k < n; ++k) id[k] = (int)k;
         ok = hipMemcpy(d.sidx, id.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess;
+        Ctl c{};
+        c.epoch = 1;
+        ok = ok && hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) == hipSuccess;
     }
-    ok = ok && hipHostMalloc((void**)&s->hctl, 2 * sizeof(Ctl), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&s->hctl, 4 * sizeof(Ctl), hipHostMallocDefault) == hipSuccess;
+    for (int k = 0; ok && k < 4; ++k)
+        ok = hipEventCreateWithFlags(&s->poll_ev[k], hipEventDisableTiming) == hipSuccess;
     if (ok && kind == DPM_BUMPY) ok = bumpy_alloc(s) == 0;
     if (!ok) {
         std::string e = dpm::g_err;
@@ -1367,14 +1775,17 @@ dpm_system* dpm_create(int kind, int64_t n, int64_t nv) {
 void dpm_destroy(dpm_system* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (auto* g : {&s->nve_graph, &s->fire_graph})
-        if (g->exec) {
-            (void)hipGraphExecDestroy(g->exec);
-            (void)hipGraphDestroy(g->graph);
-        }
+    for (auto* cache : {&s->nve_graphs, &s->fire_graphs})
+        for (auto& kv : *cache)
+            if (kv.second.exec) {
+                (void)hipGraphExecDestroy(kv.second.exec);
+                (void)hipGraphDestroy(kv.second.graph);
+            }
     for (void* p : s->allocs) (void)hipFree(p);
     for (void* p : s->snap_allocs) (void)hipFree(p);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : s->poll_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->hctl) (void)hipHostFree(s->hctl);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1405,105 +1816,99 @@ int dpm_set_energy(dpm_system* s, double e_c, double n_c) {
     return 0;
 }
 
-static int field_d2(dpm_system* s, int f, double2** p, int* comp) {
-    switch (f) {
-        case DPM_F_X: *p = s->d.pos; *comp = 0; return 1;
-        case DPM_F_Y: *p = s->d.pos; *comp = 1; return 1;
-        case DPM_F_VX: *p = s->d.vel; *comp = 0; return 1;
-        case DPM_F_VY: *p = s->d.vel; *comp = 1; return 1;
-        case DPM_F_FX: *p = s->d.frc; *comp = 0; return 1;
-        case DPM_F_FY: *p = s->d.frc; *comp = 1; return 1;
-    }
-    return 0;
-}
-
-static double* field_d(dpm_system* s, int f) {
-    switch (f) {
-        case DPM_F_RAD: return s->d.rad;
-        case DPM_F_MASS: return s->d.mass;
-        case DPM_F_PE: return s->d.pe;
-        case DPM_F_KE: return s->d.ke;
-        case DPM_F_STRESS_XX: return s->d.sxx;
-        case DPM_F_STRESS_XY: return s->d.sxy;
-        case DPM_F_STRESS_YX: return s->d.syx;
-        case DPM_F_STRESS_YY: return s->d.syy;
-    }
-    return nullptr;
-}
-
 int dpm_upload(dpm_system* s, int f, const void* data, int64_t count) {
     std::vector<int> sidx;
     if (fetch_sidx(s, sidx)) return -1;
-    if (s->kind == DPM_BUMPY &&
-        ((f >= DPM_F_ANGLE && f <= DPM_F_MOI) || f >= DPM_F_VERTEX_X))
+    if (s->kind == DPM_BUMPY && ((f >= DPM_F_ANGLE && f <= DPM_F_MOI) || f >= DPM_F_VERTEX_X))
         return bumpy_upload(s, f, data, count, sidx);
     if (count != s->N) {
         set_error("dpm_upload: count mismatch");
         return -1;
     }
     const double* src = (const double*)data;
-    double2* p2;
+    char kind;
     int comp;
-    if (field_d2(s, f, &p2, &comp)) {
-        std::vector<double2> h(s->N);
-        if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
-        for (int64_t k = 0; k < s->N; ++k) (comp ? h[k].y : h[k].x) = src[sidx[k]];
-        if (scopy(s, p2, h.data(), sizeof(double2) * s->N, hipMemcpyHostToDevice)) return -1;
-        return 0;
+    double* plain;
+    if (!field_slot(s, f, &kind, &comp, &plain) || f == DPM_F_KE) {
+        set_error("dpm_upload: field not uploadable");
+        return -1;
     }
-    if (double* p = field_d(s, f)) {
+    if (kind == 'D') {
         std::vector<double> h(s->N);
         for (int64_t k = 0; k < s->N; ++k) h[k] = src[sidx[k]];
-        if (scopy(s, p, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice)) return -1;
-        return 0;
+        return scopy(s, plain, h.data(), sizeof(double) * s->N, hipMemcpyHostToDevice);
     }
-    set_error("dpm_upload: field not uploadable");
-    return -1;
+    if (kind == 'P') {
+        std::vector<double4> h(s->N);
+        if (scopy(s, h.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost)) return -1;
+        for (int64_t k = 0; k < s->N; ++k) (&h[k].x)[comp] = src[sidx[k]];
+        if (f == DPM_F_RAD) {
+            double m = 0.0;
+            for (int64_t k = 0; k < s->N; ++k) m = std::max(m, h[k].z);
+            s->rmax = m;
+            s->gen++;
+        }
+        if (scopy(s, s->d.P0, h.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice)) return -1;
+        return scopy(s, s->d.P1, h.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice);
+    }
+    double2* p2 = kind == 'V' ? s->d.vel : s->d.frc;
+    std::vector<double2> h(s->N);
+    if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
+    for (int64_t k = 0; k < s->N; ++k) (comp ? h[k].y : h[k].x) = src[sidx[k]];
+    return scopy(s, p2, h.data(), sizeof(double2) * s->N, hipMemcpyHostToDevice);
 }
 
 int dpm_download(dpm_system* s, int f, void* data, int64_t count) {
     std::vector<int> sidx;
     if (fetch_sidx(s, sidx)) return -1;
-    if (s->kind == DPM_BUMPY &&
-        ((f >= DPM_F_ANGLE && f <= DPM_F_MOI) || f >= DPM_F_VERTEX_X))
+    if (s->kind == DPM_BUMPY && ((f >= DPM_F_ANGLE && f <= DPM_F_MOI) || f >= DPM_F_VERTEX_X))
         return bumpy_download(s, f, data, count, sidx);
     if (count != s->N) {
         set_error("dpm_download: count mismatch");
         return -1;
     }
-    double2* p2;
-    int comp;
-    if (field_d2(s, f, &p2, &comp)) {
-        std::vector<double2> h(s->N);
-        if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
-        double* dst = (double*)data;
-        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = comp ? h[k].y : h[k].x;
-        return 0;
-    }
-    if (double* p = field_d(s, f)) {
-        std::vector<double> h(s->N);
-        if (scopy(s, h.data(), p, sizeof(double) * s->N, hipMemcpyDeviceToHost)) return -1;
-        double* dst = (double*)data;
-        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
-        return 0;
-    }
-    int* pi = nullptr;
-    if (f == DPM_F_NUM_NEIGHBORS) pi = s->d.nnum;
-    if (f == DPM_F_CONTACTS) pi = s->d.contacts;
     if (f == DPM_F_STATIC_INDEX) {
         int64_t* dst = (int64_t*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[k] = sidx[k];
         return 0;
     }
-    if (pi) {
+    if (f == DPM_F_NUM_NEIGHBORS || f == DPM_F_CONTACTS) {
+        int* src = f == DPM_F_NUM_NEIGHBORS ? s->d.nnum : s->d.contacts;
+        if (!src) {
+            set_error("dpm_download: field not computed");
+            return -1;
+        }
         std::vector<int> h(s->N);
-        if (scopy(s, h.data(), pi, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
+        if (scopy(s, h.data(), src, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
         int64_t* dst = (int64_t*)data;
         for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
         return 0;
     }
-    set_error("dpm_download: unknown or unallocated field");
-    return -1;
+    char kind;
+    int comp;
+    double* plain;
+    if (!field_slot(s, f, &kind, &comp, &plain)) {
+        set_error("dpm_download: unknown or unallocated field");
+        return -1;
+    }
+    double* dst = (double*)data;
+    if (kind == 'D') {
+        std::vector<double> h(s->N);
+        if (scopy(s, h.data(), plain, sizeof(double) * s->N, hipMemcpyDeviceToHost)) return -1;
+        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = h[k];
+        return 0;
+    }
+    if (kind == 'P') {
+        std::vector<double4> h(s->N);
+        if (scopy(s, h.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost)) return -1;
+        for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = (&h[k].x)[comp];
+        return 0;
+    }
+    double2* p2 = kind == 'V' ? s->d.vel : s->d.frc;
+    std::vector<double2> h(s->N);
+    if (scopy(s, h.data(), p2, sizeof(double2) * s->N, hipMemcpyDeviceToHost)) return -1;
+    for (int64_t k = 0; k < s->N; ++k) dst[sidx[k]] = comp ? h[k].y : h[k].x;
+    return 0;
 }
 
 int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq, int64_t ncd,
@@ -1513,8 +1918,6 @@ int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq,
         return -1;
     }
     s->method = method;
-    s->max_alloc = 4;
-    s->vmax_alloc = 4;
     s->rc = cutoff;
     s->thr_sq = thr_sq;
     s->nbr_ready = false;
@@ -1537,8 +1940,11 @@ int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq,
             dfree(s, s->d.cell_start);
             dfree(s, s->d.cell_cnt);
             dfree(s, s->d.cell_cur);
+            dfree(s, s->d.scan_bsum);
+            int64_t nsb = (s->ncells + SCAN_TILE - 1) / SCAN_TILE;
             if (dalloc(s, &s->d.cell_start, s->ncells + 1) ||
-                dalloc(s, &s->d.cell_cnt, s->ncells) || dalloc(s, &s->d.cell_cur, s->ncells))
+                dalloc(s, &s->d.cell_cnt, s->ncells) || dalloc(s, &s->d.cell_cur, s->ncells) ||
+                dalloc(s, &s->d.scan_bsum, nsb + 1))
                 return -1;
             s->ncells_alloc = s->ncells;
         }
@@ -1546,15 +1952,15 @@ int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq,
     int64_t cap = 16;
     if (method == DPM_NBR_ALL) {
         // Particle::initAllToAllList (particle.cu:1186-1200)
-        s->max_alloc = std::max<int64_t>(1, s->N - 1);
-        cap = s->max_alloc;
+        cap = std::max<int64_t>(1, s->N - 1);
         double diag = std::sqrt(s->box[0] * s->box[0] + s->box[1] * s->box[1]);
         s->rc = 10.0 * diag;
         s->thr_sq = s->rc * s->rc;
     }
     if (ensure_ell(s, cap)) return -1;
+    reset_ctl(s, 1);
     s->gen++;
-    return 0;
+    return check_launch();
 }
 
 int dpm_neighbor_init(dpm_system* s) {
@@ -1563,14 +1969,15 @@ int dpm_neighbor_init(dpm_system* s) {
         return -1;
     }
     // initVerletListVariables (particle.cu:1171-1178): references and displacements zeroed
-    DPM_HIP(hipMemsetAsync(s->d.lref, 0, sizeof(double2) * s->N, s->stream));
-    DPM_HIP(hipMemsetAsync(s->d.cref, 0, sizeof(double2) * s->N, s->stream));
-    DPM_HIP(hipMemsetAsync(s->d.pmax_nd, 0, sizeof(double) * s->nblk, s->stream));
-    DPM_HIP(hipMemsetAsync(s->d.pmax_cd, 0, sizeof(double) * s->nblk, s->stream));
-    if (s->method == DPM_NBR_CELL && cell_rebuild(s)) return -1;
-    if (neighbor_build(s)) return -1;
+    hipStream_t st = s->stream;
+    DPM_HIP(hipMemsetAsync(s->d.lref, 0, sizeof(double2) * s->N, st));
+    DPM_HIP(hipMemsetAsync(s->d.cref, 0, sizeof(double2) * s->N, st));
+    DPM_HIP(hipMemsetAsync(s->d.pmax_nd, 0, sizeof(double) * s->nblk, st));
+    DPM_HIP(hipMemsetAsync(s->d.pmax_cd, 0, sizeof(double) * s->nblk, st));
+    reset_ctl(s, 0);
+    if (device_rebuild(s, 2, nullptr)) return -1;
     s->nbr_ready = true;
-    DPM_HIP(hipStreamSynchronize(s->stream));
+    DPM_HIP(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -1584,8 +1991,8 @@ int dpm_neighbor_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int
     std::vector<int> sidx, nn(s->N), nb((size_t)(s->N * s->phys_cap));
     if (fetch_sidx(s, sidx)) return -1;
     if (scopy(s, nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
-    DPM_HIP(hipMemcpy(nb.data(), s->d.nbr, sizeof(int) * s->N * s->phys_cap,
-                      hipMemcpyDeviceToHost));
+    if (scopy(s, nb.data(), s->d.nbr, sizeof(int) * s->N * s->phys_cap, hipMemcpyDeviceToHost))
+        return -1;
     int64_t c = 0;
     for (int64_t i = 0; i < s->N; ++i)
         for (int64_t k = 0; k < nn[i] && k < s->phys_cap; ++k) {
@@ -1605,7 +2012,14 @@ int dpm_neighbor_pairs(dpm_system* s, int64_t* oi, int64_t* oj, int64_t cap, int
 }
 
 int dpm_neighbor_stats(dpm_system* s, int64_t* ma, int64_t* tot, int64_t* nr, int64_t* ncr) {
-    if (ma) *ma = s->max_alloc;
+    Ctl c;
+    if (scopy(s, &c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+    if (ma) {
+        // Particle::updateVerletList's capacity rule (particle.cu:1122-1130): starts at
+        // 4 and becomes 2^ceil(log2 max) whenever a build exceeds it
+        if (s->method == DPM_NBR_ALL) *ma = std::max<int64_t>(1, s->N - 1);
+        else *ma = c.max_nnum_all > 4 ? pow2_at_least(c.max_nnum_all) : 4;
+    }
     if (tot) {
         std::vector<int> nn(s->N);
         if (scopy(s, nn.data(), s->d.nnum, sizeof(int) * s->N, hipMemcpyDeviceToHost)) return -1;
@@ -1613,13 +2027,13 @@ int dpm_neighbor_stats(dpm_system* s, int64_t* ma, int64_t* tot, int64_t* nr, in
         for (int v : nn) t += v;
         *tot = t;
     }
-    if (nr) *nr = s->n_rebuilds;
-    if (ncr) *ncr = s->n_cell_rebuilds;
+    if (nr) *nr = c.n_rebuilds;
+    if (ncr) *ncr = c.n_cell_rebuilds;
     return 0;
 }
 
 int dpm_zero_forces(dpm_system* s) {
-    hipLaunchKernelGGL(k_zero_forces, g1(s->N), dim3(BLOCK), 0, s->stream, s->d);
+    hipLaunchKernelGGL(k_zero_forces, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d);
     if (s->kind == DPM_BUMPY) return bumpy_forces(s, -1);
     return check_launch();
 }
@@ -1627,31 +2041,31 @@ int dpm_zero_forces(dpm_system* s) {
 int dpm_compute_forces(dpm_system* s) {
     if (require_nbr(s, "dpm_compute_forces")) return -1;
     if (s->kind == DPM_BUMPY) return bumpy_forces(s, 1);
-    PairParams p = params_of(s);
-    hipLaunchKernelGGL(k_disk_forces_acc, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p);
+    hipLaunchKernelGGL(k_disk_forces_acc, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
+                       params_of(s));
     return check_launch();
 }
 
 int dpm_kick(dpm_system* s, double dt) {
     if (s->kind == DPM_BUMPY) return bumpy_kick(s, dt);
-    hipLaunchKernelGGL(k_kick, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
+    hipLaunchKernelGGL(k_kick, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
     return check_launch();
 }
 
 int dpm_drift(dpm_system* s, double dt) {
     if (s->kind == DPM_BUMPY) return bumpy_drift(s, dt);
-    hipLaunchKernelGGL(k_drift, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
+    hipLaunchKernelGGL(k_drift, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nbr_params(s), dt);
     return check_launch();
 }
 
 int dpm_mix(dpm_system* s, double alpha) {
     if (s->kind == DPM_BUMPY) return bumpy_mix(s, alpha);
-    hipLaunchKernelGGL(k_mix, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha);
+    hipLaunchKernelGGL(k_mix, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha);
     return check_launch();
 }
 
 int dpm_zero_velocities(dpm_system* s) {
-    hipLaunchKernelGGL(k_zero_velocities, g1(s->N), dim3(BLOCK), 0, s->stream, s->d);
+    hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d);
     return check_launch();
 }
 
@@ -1663,17 +2077,12 @@ int dpm_nve_run(dpm_system* s, double dt, int64_t n) {
     return nve_run(s, dt, n);
 }
 
-static int64_t ndof_of(dpm_system* s) {
-    if (s->kind == DPM_BUMPY) return s->rotation ? 3 * s->N : 2 * s->N;
-    return 2 * s->N;
-}
-
 int dpm_reduce(dpm_system* s, int what, double* out) {
     switch (what) {
         case DPM_RED_PE: return sum_to_host(s, S_PE, out);
         case DPM_RED_KE:
         case DPM_RED_TEMPERATURE: {
-            hipLaunchKernelGGL(k_kinetic, g1(s->N), dim3(BLOCK), 0, s->stream, s->d,
+            hipLaunchKernelGGL(k_kinetic, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
                                (int)(s->kind == DPM_BUMPY && s->rotation));
             double ke;
             if (sum_to_host(s, S_KE, &ke)) return -1;
@@ -1721,9 +2130,10 @@ int dpm_reduce(dpm_system* s, int what, double* out) {
         case DPM_RED_STRESS_YX:
         case DPM_RED_STRESS_YY: {
             if (dpm_compute_stress(s)) return -1;
-            int w = what == DPM_RED_STRESS_XX ? S_SXX
+            int w = what == DPM_RED_STRESS_XX   ? S_SXX
                     : what == DPM_RED_STRESS_XY ? S_SXY
-                    : what == DPM_RED_STRESS_YX ? S_SYX : S_SYY;
+                    : what == DPM_RED_STRESS_YX ? S_SYX
+                                                : S_SYY;
             return sum_to_host(s, w, out);
         }
         case DPM_RED_CONTACTS:
@@ -1810,8 +2220,8 @@ int dpm_adam_step(dpm_system* s, int64_t step, double alpha, double b1, double b
     if (dpm_zero_forces(s) || check_and_rebuild(s, nullptr) || dpm_compute_forces(s)) return -1;
     double omb1 = 1 - std::pow(b1, step + 1);
     double omb2 = 1 - std::pow(b2, step + 1);
-    hipLaunchKernelGGL(k_adam_update, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha, b1, b2,
-                       omb1, omb2, eps);
+    hipLaunchKernelGGL(k_adam_update, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
+                       nbr_params(s), alpha, b1, b2, omb1, omb2, eps);
     return check_launch();
 }
 
@@ -1842,8 +2252,7 @@ int dpm_compute_stress(dpm_system* s) {
     if (require_nbr(s, "dpm_compute_stress")) return -1;
     if (ensure_stress(s)) return -1;
     if (s->kind == DPM_BUMPY) return bumpy_stress(s);
-    PairParams p = params_of(s);
-    hipLaunchKernelGGL(k_disk_stress, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p,
+    hipLaunchKernelGGL(k_disk_stress, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, params_of(s),
                        s->box[0] * s->box[1]);
     return check_launch();
 }
@@ -1852,13 +2261,13 @@ int dpm_count_contacts(dpm_system* s) {
     if (require_nbr(s, "dpm_count_contacts")) return -1;
     if (ensure_stress(s)) return -1;
     if (s->kind == DPM_BUMPY) return bumpy_contacts(s);
-    PairParams p = params_of(s);
-    hipLaunchKernelGGL(k_disk_contacts, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, p);
+    hipLaunchKernelGGL(k_disk_contacts, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
+                       params_of(s));
     return check_launch();
 }
 
 int dpm_scale_positions(dpm_system* s, double f) {
-    hipLaunchKernelGGL(k_scale_positions, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, f);
+    hipLaunchKernelGGL(k_scale_positions, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, f);
     return check_launch();
 }
 
@@ -1893,8 +2302,7 @@ static int snap_alloc(dpm_system* s) {
         return 0;
     };
     int64_t n = s->N;
-    if (al(&s->snap.pos, n) || al(&s->snap.vel, n) || al(&s->snap.frc, n) ||
-        al(&s->snap.rad, n) || al(&s->snap.mass, n) || al(&s->snap.sidx, n)) {
+    if (al(&s->snap.P, n) || al(&s->snap.vel, n) || al(&s->snap.frc, n) || al(&s->snap.sidx, n)) {
         set_error("snapshot allocation failed");
         return -1;
     }
@@ -1918,11 +2326,9 @@ int dpm_snapshot_save(dpm_system* s) {
         return hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s->stream);
     };
     int64_t n = s->N;
-    DPM_HIP(cp(s->snap.pos, s->d.pos, sizeof(double2) * n));
+    DPM_HIP(cp(s->snap.P, s->d.P0, sizeof(double4) * n));
     DPM_HIP(cp(s->snap.vel, s->d.vel, sizeof(double2) * n));
     DPM_HIP(cp(s->snap.frc, s->d.frc, sizeof(double2) * n));
-    DPM_HIP(cp(s->snap.rad, s->d.rad, sizeof(double) * n));
-    DPM_HIP(cp(s->snap.mass, s->d.mass, sizeof(double) * n));
     DPM_HIP(cp(s->snap.sidx, s->d.sidx, sizeof(int) * n));
     if (s->kind == DPM_BUMPY && bumpy_snapshot(s, true)) return -1;
     s->snap_box[0] = s->box[0];
@@ -1931,7 +2337,7 @@ int dpm_snapshot_save(dpm_system* s) {
     return 0;
 }
 
-// Particle::revertToLastState (particle.cu:977-991) + a full cell rebuild.
+// Particle::revertToLastState (particle.cu:977-991) + a full neighbour rebuild.
 int dpm_snapshot_restore(dpm_system* s) {
     if (!s->snap_valid) {
         set_error("dpm_snapshot_restore: no snapshot");
@@ -1941,18 +2347,14 @@ int dpm_snapshot_restore(dpm_system* s) {
         return hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s->stream);
     };
     int64_t n = s->N;
-    DPM_HIP(cp(s->d.pos, s->snap.pos, sizeof(double2) * n));
+    DPM_HIP(cp(s->d.P0, s->snap.P, sizeof(double4) * n));
+    DPM_HIP(cp(s->d.P1, s->snap.P, sizeof(double4) * n));
     DPM_HIP(cp(s->d.vel, s->snap.vel, sizeof(double2) * n));
     DPM_HIP(cp(s->d.frc, s->snap.frc, sizeof(double2) * n));
-    DPM_HIP(cp(s->d.rad, s->snap.rad, sizeof(double) * n));
-    DPM_HIP(cp(s->d.mass, s->snap.mass, sizeof(double) * n));
     DPM_HIP(cp(s->d.sidx, s->snap.sidx, sizeof(int) * n));
     if (s->kind == DPM_BUMPY && bumpy_snapshot(s, false)) return -1;
     if (dpm_set_box(s, s->snap_box[0], s->snap_box[1])) return -1;
-    if (s->nbr_ready) {
-        if (s->method == DPM_NBR_CELL && cell_rebuild(s)) return -1;
-        if (neighbor_build(s)) return -1;
-    }
+    if (s->nbr_ready && device_rebuild(s, 2, nullptr)) return -1;
     return 0;
 }
 
@@ -1961,7 +2363,7 @@ int dpm_remove_mean_velocity(dpm_system* s) {
     double a, b;
     if (sum_to_host(s, S_VX, &a) || sum_to_host(s, S_VY, &b)) return -1;
     double mx = a / s->N, my = b / s->N;
-    hipLaunchKernelGGL(k_add_velocity, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, mx, my, 0.0);
+    hipLaunchKernelGGL(k_add_velocity, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, mx, my, 0.0);
     return check_launch();
 }
 
@@ -1974,11 +2376,10 @@ int dpm_scale_to_temperature(dpm_system* s, double T) {
         return -1;
     }
     double f = std::sqrt(T / cur);
-    hipLaunchKernelGGL(k_add_velocity, g1(s->N), dim3(BLOCK), 0, s->stream, s->d, 0.0, 0.0, f);
+    hipLaunchKernelGGL(k_add_velocity, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, 0.0, 0.0, f);
     return check_launch();
 }
-
-// Pressure-bisection jamming, scripts/runArthurJamWithFire.cpp:36-106 (see oracle jam()).
+// Pressure-bisection jamming, scripts/runArthurJamWithFire.cpp:36-106.
 int dpm_jam(dpm_system* s, const dpm_jam_params* P, dpm_jam_result* out) {
     if (require_nbr(s, "dpm_jam")) return -1;
     double P_low = P->pressure_target, P_high = P->pressure_target * (1.0 + P->window_rel);

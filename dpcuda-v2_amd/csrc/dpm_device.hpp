@@ -22,21 +22,30 @@ __device__ __forceinline__ double pbc_div(double a, double b, double L) {
     return d - L * round(d / L);
 }
 
-// Same, with d/L replaced by d*(1/L).  For any pair whose minimum image is
-// shorter than L/2 by more than a few ulps, round(d*invL) == round(d/L), so the
-// returned difference is bit-identical; pairs near L/2 are beyond every contact
-// distance (L/2 > r_i + r_j), where both forms give zero force.  Used in force,
-// stress and contact kernels.
-__device__ __forceinline__ double pbc_mul(double a, double b, double L, double invL) {
+// Minimum image used by the force, stress, contact and list-build kernels.
+// With `fast` set (the host sets it when L/2 exceeds every contact and list
+// cutoff), |d| < L/2 returns d itself and other d use d*(1/L) for the rounding.
+// Both are bit-identical to pbc_div for every pair closer than L/2 minus a few
+// ulps; the only pairs where they can differ have |dx| within ulps of L/2, which
+// is beyond every cutoff, so forces, energies, stresses, contacts and neighbour
+// sets are unchanged.  Without `fast` the reference's division form is used.
+__device__ __forceinline__ double pbc(double a, double b, double L, double invL, double halfL,
+                                      int fast) {
     double d = a - b;
-    return d - L * round(d * invL);
+    if (fast) {
+        if (fabs(d) < halfL) return d;
+        return d - L * round(d * invL);
+    }
+    return d - L * round(d / L);
 }
 
 struct PairParams {
-    double Lx, Ly, invLx, invLy;
+    double Lx, Ly, invLx, invLy, halfLx, halfLy;
     double e_c, n_c, ncm1;  // ncm1 = n_c - 1
     double inv2nc;          // 1/(2 n_c) when 2 n_c is a power of two (exact), else 0
     double two_nc;
+    int fast;
+    int pad;
 };
 
 // overlap^(n_c-1): n_c == 2 gives exactly overlap.
@@ -44,13 +53,11 @@ __device__ __forceinline__ double pow_ov(double ov, const PairParams& p) {
     return p.ncm1 == 1.0 ? ov : pow(ov, p.ncm1);
 }
 
-// calcPointPointInteraction, include/particles/base/kernels.cuh:154-183.
-// Returns the half pair energy; writes the force on the first point.
-__device__ __forceinline__ double pair_law(double px, double py, double r, double ox, double oy,
-                                           double orad, const PairParams& p, double& fx,
-                                           double& fy) {
-    double dx = pbc_mul(px, ox, p.Lx, p.invLx);
-    double dy = pbc_mul(py, oy, p.Ly, p.invLy);
+// calcPointPointInteraction, include/particles/base/kernels.cuh:154-183, for a
+// given minimum-image separation.  Returns the half pair energy; writes the force
+// on the first point.
+__device__ __forceinline__ double pair_core(double dx, double dy, double r, double orad,
+                                            const PairParams& p, double& fx, double& fy) {
     double rs = r + orad;
     double d2 = dx * dx + dy * dy;
     if (d2 >= rs * rs) {
@@ -67,6 +74,14 @@ __device__ __forceinline__ double pair_law(double px, double py, double r, doubl
     fx = fm * dx;
     fy = fm * dy;
     return e;
+}
+
+__device__ __forceinline__ double pair_law(double px, double py, double r, double ox, double oy,
+                                           double orad, const PairParams& p, double& fx,
+                                           double& fy) {
+    double dx = pbc(px, ox, p.Lx, p.invLx, p.halfLx, p.fast);
+    double dy = pbc(py, oy, p.Ly, p.invLy, p.halfLy, p.fast);
+    return pair_core(dx, dy, r, orad, p, fx, fy);
 }
 
 // Pairwise-halving tree over the 64 lanes (lane t += lane t+s, s = 32..1);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -22,17 +23,37 @@ void set_error(const std::string& m);
         }                                                                      \
     } while (0)
 
-// Device control block of one system.  Written by kernels with plain stores
-// (never atomics in the hot loop); read by the host through a pinned copy.
+// What a halted step still has to do once the rebuild chain has run.
+enum HaltMode { H_NVE_FUSED = 0, H_NVE_FORCE = 1, H_FIRE = 2 };
+
+// Device control block of one system.  Hot kernels only ever write it with plain
+// stores (no atomics, no fences); the host reads it through a pinned copy taken at
+// the end of each chunk.
+//
+// Rebuild requests are "stamped": the kernel that moves particles writes, into
+// flag[slot] (list rebuild) and cflag[slot] (cell + list rebuild), the stamp of the
+// step that must act on it (epoch*8192 + position + 1).  A reader acts only on an
+// exact stamp match, so stale requests never need resets.
 struct Ctl {
-    int flag[2];     // rebuild request raised by the drift kernel of step parity p
-    int halt;        // set by the force kernel when a rebuild must happen first
-    int halt_pos;    // chunk position of the halted step
-    int done;        // the device-resident loop has finished (FIRE convergence)
-    int max_nnum;    // max neighbours of the last build (atomicMax)
-    int max_vnum;    // max vertex neighbours of the last build
-    int overflow;    // a build row did not fit the physical capacity
-    double scal[16]; // reduction results
+    long long flag[2];
+    long long cflag[2];
+    long long epoch;          // bumped by the chain at the end of every chunk
+    long long ovf;            // stamp of a step whose in-kernel list rebuild overflowed
+    int halt;                 // a step stopped because it needs the rebuild chain
+    int halt_pos;             // chunk position of the halted step
+    int halt_mode;            // HaltMode of the halted step
+    int halt_kind;            // rebuild it needs: 1 list, 2 cells + list, 3 list from lref
+    int progress;             // steps completed so far in the running chunk
+    int last_progress;        // steps completed in the previous chunk
+    int done;                 // device-resident FIRE loop finished
+    int kind;                 // rebuild chosen by the chain: 0 none, 1 list, 2 cells+list
+    int overflow;             // a row exceeded the physical capacity: host must regrow
+    int max_nnum, max_nnum_all;  // last build / running max over all builds
+    int max_vnum, max_vnum_all;
+    int n_rebuilds, n_cell_rebuilds;
+    int bad_pos;              // a non-finite position was binned
+    int pad[2];
+    double scal[16];          // small reduction results
 };
 
 // Device control block of the device-resident FIRE loop.
@@ -42,57 +63,64 @@ struct FireCtl {
     double last_avg_pe, avg_pe, power, fmax_sq;
     long long n_good, n_bad, step, max_steps;
     int stopped, done, converged, uphill;
-    int started;     // minimisation has begun (first step taken)
-    int pad[3];
+    int pad[4];
 };
 
 // Device pointers of one system (passed by value to kernels).
 struct Dev {
     int64_t N, Nv;
-    double2 *pos, *vel, *frc, *lref, *cref;
-    double *rad, *mass, *pe, *ke;
-    int *sidx;                  // static id of internal slot k
-    // neighbour list, ELL column-major: nbr[k*N + i]
-    int *nbr, *nnum;
+    // particle state: P0/P1 hold {x, y, radius, mass}; P0 is current outside fused
+    // NVE chunks, which ping-pong the positions between the two.
+    double4 *P0, *P1;
+    double2 *vel, *frc, *lref, *cref;
+    double *pe, *ke;
+    int* sidx;                  // static id of internal slot k
+    int *nbr, *nnum;            // ELL column-major: nbr[k*N + i]
     int cap;                    // physical ELL slots per particle
     int *cell_of;               // cell of each particle at the last cell rebuild
-    int *cell_start, *cell_cnt, *cell_cur, *cell_tmp, *perm;
+    int *cell_start, *cell_cnt, *cell_cur, *cell_tmp, *perm, *scan_bsum;
     double *pmax_nd, *pmax_cd;  // per-block max displacement partials
-    double *part;               // reduction partials scratch (chunk partials)
+    double* part;               // canonical-sum chunk partials
     Ctl* ctl;
     FireCtl* fire;
-    // Adam moments (allocated on first use)
-    double2 *m1, *m2;
-    // stress / contacts (allocated on first use)
+    double2 *m1, *m2;           // Adam moments (allocated on first use)
     double *sxx, *sxy, *syx, *syy;
     int *contacts, *vcontacts;
     // rigid bumpy
     double *angle, *omega, *torque, *moi;
-    double2 *vpos;
-    double *vmass;
+    double2* vpos;
+    double* vmass;
     int *vpart, *vstart, *nverts, *svidx;
     int *vnbr, *vnum;           // vertex ELL: vnbr[k*Nv + v]
-    int vcap;                   // physical vertex ELL slots
-    double2 *vfrc;              // per-vertex force (two-kernel path scratch)
+    int vcap;
+    double2* vfrc;
     double *vtq, *vpe;
-    double2 *lpos;              // COM before the last drift (bumpy rotate)
-    double2 *dpos;              // COM displacement of the last drift
-    double *dth;                // angle change of the last drift
+    double2* lpos;              // COM before the last drift (vertex rotation)
+    double* dth;                // angle change of the last drift
 };
 
 // Reorder scratch: one spare copy of every permuted array.
 struct Spare {
-    double2 *pos, *vel, *frc, *m1, *m2;
-    double *rad, *mass, *angle, *omega, *torque, *moi, *vmass;
-    double2 *vpos;
+    double4* P;
+    double2 *vel, *frc, *m1, *m2;
+    double *angle, *omega, *torque, *moi, *vmass;
+    double2* vpos;
     int *sidx, *nverts, *vstart, *vpart, *svidx, *old_to_new;
+};
+
+// Numbers the neighbour kernels need (by value).
+struct NbrParams {
+    int method, ncd, fast;
+    int pad;
+    long long ncells;
+    double rc2, thr, cthr, cs, Lx, Ly, invLx, invLy, halfLx, halfLy;
+    double vc2, vp2;            // bumpy vertex lists
 };
 
 struct Graph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t gen = ~0ull;
-    int steps = 0;
     double dt = 0.0;
 };
 
@@ -105,19 +133,18 @@ struct dpm_system {
     hipStream_t stream = nullptr;
     double box[2] = {1.0, 1.0};
     double e_c = 1.0, n_c = 2.0;
+    double rmax = 0.0;          // largest radius (decides the fast minimum image)
     dpm::Dev d{};
     dpm::Spare sp{};
-    std::vector<void*> allocs;  // everything hipMalloc'd for this system
+    std::vector<void*> allocs;
     int64_t nblk = 0;           // ceil(N/256)
     int64_t nchunk = 0;         // ceil(N/64)
     // neighbours
     int method = DPM_NBR_NONE;
     double rc = -1, thr_sq = -1, cell_thr_sq = -1, cell_size = -1;
     int64_t ncd = -1, ncells = -1;
-    int64_t max_alloc = 4;      // reference-semantics logical capacity
-    int64_t phys_cap = 0;       // allocated ELL slots per particle
-    int64_t vmax_alloc = 4, vphys_cap = 0;
-    int64_t n_rebuilds = 0, n_cell_rebuilds = 0;
+    int64_t phys_cap = 0;
+    int64_t vphys_cap = 0;
     int64_t ncells_alloc = 0;
     bool nbr_ready = false;
     // rigid bumpy
@@ -133,12 +160,12 @@ struct dpm_system {
     int64_t chunk = 64;
     bool timing = false;
     uint64_t gen = 0;           // bumped whenever baked kernel arguments change
-    dpm::Graph nve_graph, fire_graph;
-    dpm::Ctl* hctl = nullptr;   // pinned host mirrors (2 slots)
-    dpm::FireCtl* hfire = nullptr;
+    std::map<int64_t, dpm::Graph> nve_graphs, fire_graphs;
+    dpm::Ctl* hctl = nullptr;   // pinned host mirrors of the control block (4 slots)
     double ktime_ms[3] = {0, 0, 0};
     int64_t klaunch[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     size_t ev_next = 0;
+    hipEvent_t poll_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };

@@ -126,24 +126,74 @@ def test_nve_trajectory_matches_oracle(n, phi, T, steps):
     assert e_g == e_o
 
 
-def test_nve_chunking_and_graphs_do_not_change_results():
+@pytest.mark.parametrize("steps", [333, 334])
+def test_nve_chunking_and_graphs_do_not_change_results(steps):
+    # 334 steps leave an odd remainder for every even chunk length (split last step)
     st = W.disk_config(3000, 0.8, seed=44)
     spec = W.neighbor_spec(st.side)
     vx, vy = W.gaussian_velocities(st.n, 0.3, 4)
-    outs = []
-    for graph, chunk in ((1, 64), (0, 64), (1, 2), (0, 10)):
+    o = W.make_disk_system(P.System, st, spec, backend=oracle())
+    o.upload(P.F_VX, vx)
+    o.upload(P.F_VY, vy)
+    forces(o)
+    o.nve_run(0.01, steps)
+    so = o.neighbor_stats()
+    assert so["n_rebuilds"] > 1 and so["n_cell_rebuilds"] > 1  # the run crossed rebuilds
+    for graph, chunk in ((1, 64), (0, 64), (1, 2), (0, 10), (1, 256)):
+        say("chunking", steps, graph, chunk)
         g = W.make_disk_system(P.System, st, spec)
         g.set_option(P.OPT_GRAPH, graph)
         g.set_option(P.OPT_CHUNK, chunk)
         g.upload(P.F_VX, vx)
         g.upload(P.F_VY, vy)
         forces(g)
-        g.nve_run(0.01, 333)
-        outs.append((g.download(P.F_X), g.download(P.F_VX), g.neighbor_stats()["n_rebuilds"]))
-    for o in outs[1:]:
-        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
-        assert o[2] == outs[0][2]
-    assert outs[0][2] > 1  # the run crossed rebuilds
+        g.nve_run(0.01, steps)
+        assert_same_state(g, o)
+        sg = g.neighbor_stats()
+        assert (sg["n_rebuilds"], sg["n_cell_rebuilds"]) == (so["n_rebuilds"],
+                                                             so["n_cell_rebuilds"])
+
+
+def _converging_ring(n_ring=20, speed=1.0, radius=0.28):
+    """A centre disk and a ring of disks closing in on it: every ring disk enters the
+    centre's cutoff at the same step, so the centre's row jumps from 0 to n_ring
+    (beyond the initial 16 slots) inside one list rebuild, while neighbouring ring
+    disks press on each other.  Three cells per side make the stencil cover the
+    whole box, and the cell threshold is never reached."""
+    side = 9.0
+    ang = 2 * np.pi * np.arange(n_ring) / n_ring
+    x = np.concatenate([[4.5], 4.5 + 2.5 * np.cos(ang)])
+    y = np.concatenate([[4.5], 4.5 + 2.5 * np.sin(ang)])
+    n = n_ring + 1
+    st = W.DiskState(n, side, x, y, np.full(n, radius), np.ones(n))
+    spec = W.NeighborSpec(3, 1.7, 3 * 0.15 ** 2, 3, 1e30)
+    vx = np.concatenate([[0.0], -speed * np.cos(ang)])
+    vy = np.concatenate([[0.0], -speed * np.sin(ang)])
+    return st, spec, vx, vy
+
+
+def test_list_rebuild_overflow_inside_nve():
+    st, spec, vx, vy = _converging_ring()
+    res = []
+    for backend in (None, oracle()):
+        s = W.make_disk_system(P.System, st, spec, backend=backend)
+        s.upload(P.F_VX, vx)
+        s.upload(P.F_VY, vy)
+        forces(s)
+        assert s.neighbor_stats()["max_alloc"] == 4
+        s.nve_run(0.01, 120)
+        res.append(s)
+    g, o = res
+    sg, so = g.neighbor_stats(), o.neighbor_stats()
+    say("ring overflow", sg, so)
+    assert so["max_alloc"] == 32 and sg["max_alloc"] == 32
+    assert sg["n_rebuilds"] == so["n_rebuilds"] and sg["n_cell_rebuilds"] == so["n_cell_rebuilds"]
+    assert np.array_equal(g.download(P.F_NUM_NEIGHBORS), o.download(P.F_NUM_NEIGHBORS))
+    assert_same_state(g, o)
+    assert np.max(np.abs(g.download(P.F_FX))) > 0.01  # the ring is in contact
+    gi, gj = g.neighbor_pairs()
+    oi, oj = o.neighbor_pairs()
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
 
 
 # ------------------------------------------------------------------ FIRE / Adam
