@@ -95,39 +95,39 @@ __device__ __forceinline__ int pending_rebuild(const Ctl* c, long long epoch, in
 
 __device__ __forceinline__ int wrap_cell(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// In-kernel list rebuild of particle i's row (kernelUpdateCellNeighborList,
-// base/kernels.cu:440-505: 3x3 stencil, dy outer, dx inner, ascending sorted index),
-// summing the pair forces of the accepted neighbours in the same pass.  The row is
-// accepted in exactly the order the stored list is later traversed, so the sums are
-// bit-identical to a separate build followed by disk_row.  Returns the row length.
-__device__ __forceinline__ int stencil_row(const Dev& d, const double4* P, const PairParams& p,
-                                           const NbrParams& nb, int64_t i, double4 qi,
-                                           double& ax, double& ay) {
+// The list row of particle i (kernelUpdateCellNeighborList, base/kernels.cu:440-505):
+// 3x3 stencil, dy outer, dx inner, ascending sorted index within a cell, written to
+// the column-major ELL.  `pos(j)` supplies the positions the list is built from.
+template <class Pos>
+__device__ __forceinline__ int stencil_build(const Dev& d, const NbrParams& nb, int64_t i,
+                                             double2 xi, Pos pos) {
     const int ncd = nb.ncd;
     int cid = d.cell_of[i];
     int cx = cid % ncd, cy = cid / ncd;
-    int cnt = 0;
-    const int cap = d.cap;
-    ax = 0.0;
-    ay = 0.0;
+    int s[9], e[9];
+#pragma unroll
     for (int oy = -1; oy <= 1; ++oy) {
         int yy = wrap_cell(cy + oy, ncd);
+#pragma unroll
         for (int ox = -1; ox <= 1; ++ox) {
             int c = wrap_cell(cx + ox, ncd) + yy * ncd;
-            int s = d.cell_start[c], e = d.cell_start[c + 1];
-            for (int j = s; j < e; ++j) {
-                if (j == i) continue;
-                double4 qj = P[j];
-                double dx = pbc(qi.x, qj.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
-                double dy = pbc(qi.y, qj.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
-                if (dx * dx + dy * dy < nb.rc2) {
-                    if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
-                    cnt++;
-                    double fx, fy;
-                    pair_core(dx, dy, qi.z, qj.z, p, fx, fy);
-                    ax += fx;
-                    ay += fy;
-                }
+            int t = (oy + 1) * 3 + (ox + 1);
+            s[t] = d.cell_start[c];
+            e[t] = d.cell_start[c + 1];
+        }
+    }
+    int cnt = 0;
+    const int cap = d.cap;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        for (int j = s[t]; j < e[t]; ++j) {
+            if (j == i) continue;
+            double2 xj = pos(j);
+            double dx = pbc(xi.x, xj.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+            double dy = pbc(xi.y, xj.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+            if (dx * dx + dy * dy < nb.rc2) {  // isWithinCutoffSquared, kernels.cuh:249-260
+                if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
+                cnt++;
             }
         }
     }
@@ -135,31 +135,54 @@ __device__ __forceinline__ int stencil_row(const Dev& d, const double4* P, const
     return cnt;
 }
 
-// Sum of harmonic pair forces/energies over particle i's ELL row, in row order:
-// kernelCalcDiskForces, src/particles/disk/kernels.cu:11-57.  The next neighbour's
-// index and position are in flight while the current pair is evaluated.
-__device__ __forceinline__ void disk_row(const Dev& d, const double4* P, const PairParams& p,
-                                         int64_t i, double4 qi, double& ax, double& ay,
-                                         double& e) {
-    int nn = min(d.nnum[i], d.cap);
-    const int* col = d.nbr + i;
+// Sum of harmonic pair forces (and energies) over the ELL row of particle il, in row
+// order: kernelCalcDiskForces, src/particles/disk/kernels.cu:11-57.  Every lane of the
+// wave calls this (nn = 0 for lanes without a particle); the wave walks its longest
+// row in batches of ROW_BATCH neighbours, gathering a batch's records together and
+// loading the next batch's indices while the current batch is summed, strictly in
+// row order.  Index loads use a column clamped into the ELL (every slot holds a valid
+// particle index: the array starts zeroed and is only written with indices); slots
+// past a lane's row end contribute +0.0, which leaves its sums unchanged.
+constexpr int ROW_BATCH = 4;
+
+// HOT selects pair_hot (the host passes it when p.fast && p.fastdiv); each kernel is
+// instantiated for both so the hot loop carries no registers for the general path.
+template <bool ENERGY, bool HOT>
+__device__ __forceinline__ void row_forces(const Dev& d, const double4* P, const PairParams& p,
+                                           int64_t il, double4 qi, int nn, double& ax,
+                                           double& ay, double& e) {
+    const int cap = d.cap;
+    const int* col = d.nbr + il;
     const int64_t N = d.N;
+    const int wmax = __builtin_amdgcn_readfirstlane(wave_max_i(nn));
     ax = 0.0;
     ay = 0.0;
     e = 0.0;
-    if (nn == 0) return;
-    int j1 = nn > 1 ? col[N] : (int)i;
-    double4 qa = P[col[0]];
-    for (int k = 0; k < nn; ++k) {
-        int j2 = (k + 2 < nn) ? col[(int64_t)(k + 2) * N] : (int)i;
-        double4 qb = P[j1];
-        double fx, fy;
-        double ie = pair_law(qi.x, qi.y, qi.z, qa.x, qa.y, qa.z, p, fx, fy);
-        ax += fx;
-        ay += fy;
-        e += ie;
-        qa = qb;
-        j1 = j2;
+    int jn[ROW_BATCH];
+#pragma unroll
+    for (int t = 0; t < ROW_BATCH; ++t) jn[t] = col[(int64_t)min(t, cap - 1) * N];
+    for (int k = 0; k < wmax; k += ROW_BATCH) {
+        double4 q[ROW_BATCH];
+#pragma unroll
+        for (int t = 0; t < ROW_BATCH; ++t) q[t] = P[jn[t]];
+#pragma unroll
+        for (int t = 0; t < ROW_BATCH; ++t)
+            jn[t] = col[(int64_t)min(k + ROW_BATCH + t, cap - 1) * N];
+#pragma unroll
+        for (int t = 0; t < ROW_BATCH; ++t) {
+            if (k + t < wmax) {
+                const bool valid = k + t < nn;
+                double fx = 0.0, fy = 0.0, ie = 0.0;
+                if (HOT) {
+                    ie = pair_hot<ENERGY>(qi, q[t], valid, p, fx, fy);
+                } else if (valid) {
+                    ie = pair_law(qi.x, qi.y, qi.z, q[t].x, q[t].y, q[t].z, p, fx, fy);
+                }
+                ax += fx;
+                ay += fy;
+                if (ENERGY) e += ie;
+            }
+        }
     }
 }
 
@@ -171,6 +194,7 @@ __device__ __forceinline__ void disk_row(const Dev& d, const double4* P, const P
 // at their old positions.  The forces and energies of these intermediate steps are
 // consumed in registers and never stored: the step that ends a run (k_nve_force)
 // writes the arrays the caller can observe.
+template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrParams nb,
                                                      double hdt, double dt, int cur, int pos,
                                                      int last) {
@@ -192,23 +216,29 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
     const double4* Pc = cur ? d.P1 : d.P0;
     double4* Pn = cur ? d.P0 : d.P1;
     int tile;
-    int64_t i = particle_of_thread(tile);
-    double nd = 0.0, cd = 0.0;
+    const int64_t i = particle_of_thread(tile);
+    const bool live = i < d.N;
+    const int64_t il = live ? i : 0;
+    const double4 qi = Pc[il];
+    double2 ln = make_double2(qi.x, qi.y);
     int cnt = 0;
-    if (i < d.N) {
-        double4 qi = Pc[i];
-        double ax, ay;
-        double2 ln;
-        if (in_kernel) {
-            cnt = stencil_row(d, Pc, p, nb, i, qi, ax, ay);
-            ln = make_double2(qi.x, qi.y);
+    if (in_kernel) {
+        if (live) {  // rebuild this particle's row, then sum it like any other row
+            cnt = stencil_build(d, nb, i, ln, [&](int j) {
+                double4 q = Pc[j];
+                return make_double2(q.x, q.y);
+            });
             d.lref[i] = ln;
             if (cnt > d.cap) c->ovf = stamp(e, pos);
-        } else {
-            double en;
-            disk_row(d, Pc, p, i, qi, ax, ay, en);
-            ln = d.lref[i];
         }
+    } else {
+        ln = d.lref[il];
+    }
+    const int nn = live ? min(d.nnum[il], d.cap) : 0;
+    double ax, ay, en;
+    row_forces<false, HOT>(d, Pc, p, il, qi, nn, ax, ay, en);
+    double nd = 0.0, cd = 0.0;
+    if (live) {
         double fx = 0.0 + ax, fy = 0.0 + ay;
         double dtm = hdt / qi.w;
         double2 v = d.vel[i];
@@ -225,9 +255,9 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
         double dxc = x - lc.x, dyc = y - lc.y;
         cd = dxc * dxc + dyc * dyc;
     }
-    if (in_kernel) {  // row-length statistics of the rebuild (Particle::max_neighbors)
-        int m = wave_max_i(cnt);
-        if (threadIdx.x % WAVE == 0 && m > 0) atomicMax(&c->max_nnum_all, m);
+    if (in_kernel) {  // row-length statistics of the rebuild, one atomic per block
+        int m = block_max_int(cnt);
+        if (threadIdx.x == 0) atomicMax(&c->max_nnum_all, m);
         if (blockIdx.x == 0 && threadIdx.x == 0) c->n_rebuilds += 1;
     }
     long long st = last ? stamp(e + 1, 0) : stamp(e, pos + 1);
@@ -237,6 +267,7 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
 
 // Zero + forces + closing kick(dt/2) on P0 (the end of a run, the first half of a
 // split step, or the resumption of a halted step).
+template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double hdt, int pos,
                                                      int hmode, int gate) {
     Ctl* c = d.ctl;
@@ -256,11 +287,14 @@ __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double
         if (!c->halt || !resumable(d) || c->halt_mode == H_FIRE) return;
     }
     int tile;
-    int64_t i = particle_of_thread(tile);
-    if (i >= d.N) return;
-    double4 qi = d.P0[i];
+    const int64_t i = particle_of_thread(tile);
+    const bool live = i < d.N;
+    const int64_t il = live ? i : 0;
+    const double4 qi = d.P0[il];
+    const int nn = live ? min(d.nnum[il], d.cap) : 0;
     double ax, ay, en;
-    disk_row(d, d.P0, p, i, qi, ax, ay, en);
+    row_forces<true, HOT>(d, d.P0, p, il, qi, nn, ax, ay, en);
+    if (!live) return;
     double fx = 0.0 + ax, fy = 0.0 + ay;
     d.frc[i] = make_double2(fx, fy);
     d.pe[i] = 0.0 + en;
@@ -272,8 +306,8 @@ __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double
 }
 
 // Opening kick(dt/2) + drift(dt) in place on P0.  (wr_pos, epoch + wr_eoff) names
-// the step that must check the rebuild request; progress >= 0 is recorded
-// (resume: halt_pos + 1).
+// the step that must check the rebuild request; progress >= 0 is recorded (a resumed
+// step adds one to the count).
 __global__ __launch_bounds__(BLOCK) void k_nve_kick_drift(Dev d, NbrParams nb, double hdt,
                                                           double dt, int wr_pos, int wr_eoff,
                                                           int progress, int gate) {
@@ -301,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_nve_kick_drift(Dev d, NbrParams nb, d
     }
     disp_epilogue(d, nb, tile, nd, cd, wr_pos & 1, stamp(e + wr_eoff, wr_pos));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (gate == G_RESUME) c->progress = c->halt_pos + 1;
+        if (gate == G_RESUME) c->progress += 1;  // the finished halted step
         else if (progress >= 0) c->progress = progress;
     }
 }
@@ -309,14 +343,17 @@ __global__ __launch_bounds__(BLOCK) void k_nve_kick_drift(Dev d, NbrParams nb, d
 // ---- stand-alone pieces of the Particle interface -------------------------------
 // Disk::calculateForces as called stand-alone: forces are accumulated (+=), exactly
 // as the reference kernel does into caller-zeroed arrays.
+template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_disk_forces_acc(Dev d, PairParams p) {
     int tile;
-    int64_t i = particle_of_thread(tile);
-    if (i >= d.N) return;
-    if (d.nnum[i] == 0) return;
-    double4 qi = d.P0[i];
+    const int64_t i = particle_of_thread(tile);
+    const bool live = i < d.N;
+    const int64_t il = live ? i : 0;
+    const double4 qi = d.P0[il];
+    const int nn = live ? min(d.nnum[il], d.cap) : 0;
     double ax, ay, e;
-    disk_row(d, d.P0, p, i, qi, ax, ay, e);
+    row_forces<true, HOT>(d, d.P0, p, il, qi, nn, ax, ay, e);
+    if (!live || nn == 0) return;  // the reference kernel skips empty rows
     double2 f = d.frc[i];
     f.x += ax;
     f.y += ay;
@@ -784,6 +821,7 @@ __device__ __forceinline__ bool within(double ax, double ay, double bx, double b
 // kernelUpdateCellNeighborList, base/kernels.cu:440-505: 3x3 stencil in the
 // reference's order (dy outer, dx inner, ascending sorted index), written to the
 // column-major ELL.  The nine cell ranges are loaded up front.
+
 // kind 3 rebuilds the list of the last in-kernel rebuild from its build positions
 // (lref) without touching the displacement state.
 __global__ __launch_bounds__(BLOCK) void k_build_cell(Dev d, NbrParams nb, int gated) {
@@ -792,44 +830,28 @@ __global__ __launch_bounds__(BLOCK) void k_build_cell(Dev d, NbrParams nb, int g
     int tile;
     int64_t i = particle_of_thread(tile);
     if (threadIdx.x == 0 && !from_ref) d.pmax_nd[tile] = 0.0;
-    if (i >= d.N) return;
-    double2 xi = from_ref ? d.lref[i] : make_double2(d.P0[i].x, d.P0[i].y);
-    int ncd = nb.ncd;
-    int cid = d.cell_of[i];
-    int cx = cid % ncd, cy = cid / ncd;
-    int s[9], e[9];
-#pragma unroll
-    for (int oy = -1; oy <= 1; ++oy) {
-        int yy = cy + oy;
-        yy = yy < 0 ? yy + ncd : (yy >= ncd ? yy - ncd : yy);
-#pragma unroll
-        for (int ox = -1; ox <= 1; ++ox) {
-            int xx = cx + ox;
-            xx = xx < 0 ? xx + ncd : (xx >= ncd ? xx - ncd : xx);
-            int c = xx + yy * ncd;
-            int t = (oy + 1) * 3 + (ox + 1);
-            s[t] = d.cell_start[c];
-            e[t] = d.cell_start[c + 1];
-        }
-    }
     int cnt = 0;
-    const int cap = d.cap;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        for (int j = s[t]; j < e[t]; ++j) {
-            if (j == i) continue;
-            double2 xj = from_ref ? d.lref[j] : make_double2(d.P0[j].x, d.P0[j].y);
-            if (within(xi.x, xi.y, xj.x, xj.y, nb, nb.rc2)) {
-                if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = j;
-                cnt++;
-            }
+    if (i < d.N) {
+        if (from_ref) {
+            cnt = stencil_build(d, nb, i, d.lref[i], [&](int j) { return d.lref[j]; });
+        } else {
+            double4 q = d.P0[i];
+            double2 xi = make_double2(q.x, q.y);
+            cnt = stencil_build(d, nb, i, xi, [&](int j) {
+                double4 qj = d.P0[j];
+                return make_double2(qj.x, qj.y);
+            });
+            d.lref[i] = xi;
         }
     }
-    d.nnum[i] = cnt;
-    if (!from_ref) d.lref[i] = xi;
-    atomicMax(&d.ctl->max_nnum, cnt);
-    atomicMax(&d.ctl->max_nnum_all, cnt);
+    // row-length statistics: one atomic per block (same-address atomics serialise)
+    int m = block_max_int(cnt);
+    if (threadIdx.x == 0) {
+        atomicMax(&d.ctl->max_nnum, m);
+        atomicMax(&d.ctl->max_nnum_all, m);
+    }
 }
+
 
 // kernelUpdateNeighborList, base/kernels.cu:256-293: all pairs, ascending j,
 // j positions staged through LDS.
@@ -863,8 +885,11 @@ __global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, NbrParams nb, int
     if (i < d.N) {
         d.nnum[i] = cnt;
         d.lref[i] = make_double2(qi.x, qi.y);
-        atomicMax(&d.ctl->max_nnum, cnt);
-        atomicMax(&d.ctl->max_nnum_all, cnt);
+    }
+    int m = block_max_int(i < d.N ? cnt : 0);
+    if (threadIdx.x == 0) {
+        atomicMax(&d.ctl->max_nnum, m);
+        atomicMax(&d.ctl->max_nnum_all, m);
     }
 }
 
@@ -931,6 +956,16 @@ __global__ void k_chain_end(Dev d, int mode) {
             c->ovf = 0;  // the whole list was rebuilt with enough room
         }
     }
+    c->last_progress = c->progress;
+    c->progress = 0;
+    c->epoch += 1;
+}
+
+// End of an NVE chunk: the chunk's completed steps are published and the next epoch
+// starts.  A halted step stays halted; the host finishes it (rebuild chain) when it
+// reads the status.
+__global__ void k_chunk_end(Dev d) {
+    Ctl* c = d.ctl;
     c->last_progress = c->progress;
     c->progress = 0;
     c->epoch += 1;
@@ -1003,6 +1038,7 @@ __global__ __launch_bounds__(BLOCK) void k_fire_move(Dev d, NbrParams nb, int po
 }
 
 // kick: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
+template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_fire_force(Dev d, PairParams p, int pos, int kick,
                                                       int gate) {
     Ctl* c = d.ctl;
@@ -1023,12 +1059,15 @@ __global__ __launch_bounds__(BLOCK) void k_fire_force(Dev d, PairParams p, int p
         if (!c->halt || !resumable(d) || c->halt_mode != H_FIRE) return;
     }
     double half = F->dt / 2.0;
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = i < d.N;
+    const int64_t il = live ? i : 0;
+    const double4 qi = d.P0[il];
+    const int nn = live ? min(d.nnum[il], d.cap) : 0;
+    double ax, ay, e;
+    row_forces<true, HOT>(d, d.P0, p, il, qi, nn, ax, ay, e);
     double pxv = 0.0, pyv = 0.0, pe = 0.0, f2 = 0.0;
-    if (i < d.N) {
-        double4 qi = d.P0[i];
-        double ax, ay, e;
-        disk_row(d, d.P0, p, i, qi, ax, ay, e);
+    if (live) {
         double fx = 0.0 + ax, fy = 0.0 + ay;
         d.frc[i] = make_double2(fx, fy);
         pe = 0.0 + e;
@@ -1083,7 +1122,10 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
     for (int64_t k = threadIdx.x; k < nblk; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
     fm = wave_max(fm);
     if (threadIdx.x != 0) return;
-    if (!init) c->progress = (gate == G_RESUME) ? c->halt_pos + 1 : pos + 1;
+    if (!init) {
+        if (gate == G_RESUME) c->progress += 1;  // the finished halted step
+        else c->progress = pos + 1;
+    }
     double avg = pe / F->dof;
     F->fmax_sq = fm;
     F->avg_pe = avg;
@@ -1140,6 +1182,16 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
 // =========================================================================
 //                             host-side helpers
 // =========================================================================
+// Launches the HOT instantiation of a pair-force kernel when its PairParams allow the
+// short path (the PairParams argument must be named p at the call site).
+#define LAUNCH_HOT(kern, ...)                                  \
+    do {                                                       \
+        if (p.fast && p.fastdiv)                               \
+            hipLaunchKernelGGL((kern<true>), __VA_ARGS__);     \
+        else                                                   \
+            hipLaunchKernelGGL((kern<false>), __VA_ARGS__);    \
+    } while (0)
+
 namespace dpm {
 
 int dalloc_bytes(dpm_system* s, void** p, size_t bytes) {
@@ -1189,9 +1241,19 @@ static bool fast_ok(const dpm_system* s) {
     return half > lim * (1.0 + 1e-9);
 }
 
+// The short correctly rounded sqrt/division sequences apply (see sqrt_rn_normal) when
+// n_c == 2 and e_c and every contact radius lie within [2^-100, 2^100].
+static bool fastdiv_ok(const dpm_system* s) {
+    const double lo = 0x1p-100, hi = 0x1p100;
+    double rmin = s->kind == DPM_BUMPY ? s->rv : s->rmin;
+    double rmax = s->kind == DPM_BUMPY ? s->rv : s->rmax;
+    return s->n_c == 2.0 && s->e_c >= lo && s->e_c <= hi && rmin >= lo && rmax <= hi;
+}
+
 PairParams params_of(const dpm_system* s) {
     PairParams p = pair_params(s, s->e_c, s->n_c);
     p.fast = fast_ok(s) ? 1 : 0;
+    p.fastdiv = fastdiv_ok(s) ? 1 : 0;
     if (s->method == DPM_NBR_ALL)  // pair forces still only need L/2 > largest contact
         p.fast = 0.5 * std::min(s->box[0], s->box[1]) > 2.0 * s->rmax * (1.0 + 1e-9) ? 1 : 0;
     return p;
@@ -1336,12 +1398,12 @@ static void enqueue_resume(dpm_system* s, int resume, double dt) {
     PairParams p = params_of(s);
     hipStream_t st = s->stream;
     if (resume == 1) {
-        hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, 0, 0,
+        LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, 0, 0,
                            (int)G_RESUME);
         hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 0.5 * dt, dt,
                            0, 1, -1, (int)G_RESUME);
     } else if (resume == 2) {
-        hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 1,
+        LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 1,
                            (int)G_RESUME);
         hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, 0, (int)G_RESUME);
     }
@@ -1449,21 +1511,21 @@ static void nve_chunk(dpm_system* s, double dt, int64_t c) {
     int64_t fused = (c % 2) ? c - 1 : c;
     for (int64_t k = 0; k < fused; ++k) {
         Timed t(s, DPM_K_FORCE);
-        hipLaunchKernelGGL(k_nve_fused, pgrid(s->N), dim3(BLOCK), 0, st, d, p, nb, 0.5 * dt, dt,
+        LAUNCH_HOT(k_nve_fused, pgrid(s->N), dim3(BLOCK), 0, st, d, p, nb, 0.5 * dt, dt,
                            (int)(k & 1), (int)k, (int)(k == c - 1));
     }
     if (c % 2) {
         int k = (int)(c - 1);
         {
             Timed t(s, DPM_K_FORCE);
-            hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, k,
+            LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, k,
                                (int)H_NVE_FUSED, (int)G_NORMAL);
         }
         Timed t(s, DPM_K_DRIFT);
         hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 0.5 * dt, dt,
                            0, 1, k + 1, (int)G_NORMAL);
     }
-    enqueue_chain(s, 1, dt);
+    hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(1), 0, st, d);
 }
 
 static void fire_chunk(dpm_system* s, double, int64_t c) {
@@ -1478,7 +1540,7 @@ static void fire_chunk(dpm_system* s, double, int64_t c) {
         }
         {
             Timed t(s, DPM_K_FORCE);
-            hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p,
+            LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p,
                                (int)k, 1, (int)G_NORMAL);
         }
         hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, (int)k, (int)G_NORMAL);
@@ -1486,45 +1548,73 @@ static void fire_chunk(dpm_system* s, double, int64_t c) {
     enqueue_chain(s, 2, 0.0);
 }
 
-// Runs `total` steps in chunks with up to two chunks in flight.  After each chunk
-// the control block is copied to pinned memory; the host counts the chunk's
-// completed steps and intervenes only on a capacity overflow.  With
-// `stop_on_done` the loop ends when the device reports completion (FIRE).
+static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 1, dt); }
+
+// Runs `total` steps in chunks with up to two launch units in flight; each unit ends
+// with a copy of the control block to pinned memory, from which the host counts the
+// completed steps.  FIRE chunks carry the gated rebuild chain themselves
+// (host_recovery false).  NVE chunks, whose list rebuilds happen inside the step
+// kernel and halt only for cell rebuilds, end with k_chunk_end; when one reports a
+// halted step the host enqueues the rebuild chain (captured once as a graph) as a
+// recovery unit behind the chunk already in flight, without waiting for the device.
+// A capacity overflow is fixed on the host.  With `stop_on_done` the loop ends when
+// the device reports completion (FIRE).
 static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, ChunkFn chunk_fn,
-                      std::map<int64_t, Graph>& cache, bool stop_on_done) {
-    struct Inflight {
-        int64_t c;
+                      std::map<int64_t, Graph>& cache, bool stop_on_done, bool host_recovery) {
+    struct Unit {
+        int64_t c;       // steps planned (0 for a recovery unit)
         int slot;
+        bool recovery;
+        bool covered;    // launched before a recovery that finishes its halted step
     };
-    std::deque<Inflight> q;
+    std::deque<Unit> q;
     int64_t done = 0, planned = 0;
     int next_slot = 0;
-    while (done < total) {
-        while (q.size() < 2 && planned < total) {
-            int64_t c = std::min<int64_t>(s->chunk, total - planned);
-            if (s->use_graph && !s->timing && c == s->chunk) {
-                if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
-                DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
+    bool need_recovery = false;
+    while (done < total || need_recovery) {
+        while (q.size() < 2 && (need_recovery || planned < total)) {
+            Unit u{0, 0, false, false};
+            if (need_recovery) {
+                if (s->use_graph && !s->timing) {
+                    if (ensure_graph(s, cache, -1, dt, recovery_fill)) return -1;
+                    DPM_HIP(hipGraphLaunch(cache[-1].exec, s->stream));
+                } else {
+                    enqueue_chain(s, resume, dt);
+                    if (check_launch()) return -1;
+                }
+                for (auto& x : q) x.covered = true;
+                u.recovery = true;
+                planned += 1;
+                need_recovery = false;
             } else {
-                chunk_fn(s, dt, c);
-                if (check_launch()) return -1;
+                int64_t c = std::min<int64_t>(s->chunk, total - planned);
+                if (s->use_graph && !s->timing && c == s->chunk) {
+                    if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
+                    DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
+                } else {
+                    chunk_fn(s, dt, c);
+                    if (check_launch()) return -1;
+                }
+                u.c = c;
+                planned += c;
             }
-            int slot = next_slot;
+            u.slot = next_slot;
             next_slot = (next_slot + 1) % 4;
-            DPM_HIP(hipMemcpyAsync(&s->hctl[slot], s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+            DPM_HIP(hipMemcpyAsync(&s->hctl[u.slot], s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
                                    s->stream));
-            DPM_HIP(hipEventRecord(s->poll_ev[slot], s->stream));
-            q.push_back({c, slot});
-            planned += c;
+            DPM_HIP(hipEventRecord(s->poll_ev[u.slot], s->stream));
+            q.push_back(u);
         }
         if (q.empty()) break;
-        Inflight f = q.front();
+        Unit f = q.front();
         q.pop_front();
         DPM_HIP(hipEventSynchronize(s->poll_ev[f.slot]));
         if (s->timing) collect_timing(s);
         Ctl hc = s->hctl[f.slot];
+        done += hc.last_progress;
+        planned -= (f.recovery ? 1 : f.c) - hc.last_progress;
         if (hc.overflow) {
-            // the halted step is unfinished and every later chunk in flight was a no-op
+            // the halted step is unfinished and every later unit in flight was a no-op
             DPM_HIP(hipStreamSynchronize(s->stream));
             if (fix_overflow(s, resume, dt)) return -1;
             Ctl after;
@@ -1532,11 +1622,11 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
             done += after.last_progress;
             q.clear();
             planned = done;
+            need_recovery = false;
             if (stop_on_done && after.done) break;
             continue;
         }
-        done += hc.last_progress;
-        planned -= (f.c - hc.last_progress);
+        if (host_recovery && hc.halt && !f.recovery && !f.covered) need_recovery = true;
         if (stop_on_done && hc.done) break;
     }
     DPM_HIP(hipStreamSynchronize(s->stream));
@@ -1560,11 +1650,12 @@ int nve_run(dpm_system* s, double dt, int64_t n) {
                            0.5 * dt, dt, 0, 0, -1, (int)G_NORMAL);
     }
     if (check_launch()) return -1;
-    if (n > 1 && run_chunks(s, n - 1, dt, 1, nve_chunk, s->nve_graphs, false)) return -1;
+    if (n > 1 && run_chunks(s, n - 1, dt, 1, nve_chunk, s->nve_graphs, false, true)) return -1;
     {  // closing force + kick of the last step
         Timed t(s, DPM_K_FORCE);
-        hipLaunchKernelGGL(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, params_of(s),
-                           0.5 * dt, 0, (int)H_NVE_FORCE, (int)G_NORMAL);
+        PairParams p = params_of(s);
+        LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, 0,
+                   (int)H_NVE_FORCE, (int)G_NORMAL);
     }
     enqueue_chain(s, 1, dt);
     if (check_launch()) return -1;
@@ -1600,12 +1691,13 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, d, 0);
     hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (check_and_rebuild(s, nullptr)) return -1;
-    hipLaunchKernelGGL(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, params_of(s),
-                       0, 0, (int)G_ALWAYS);
+    PairParams p = params_of(s);
+    LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 0,
+               (int)G_ALWAYS);
     hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 1, 0, (int)G_ALWAYS);
     if (check_launch()) return -1;
     if (prm->max_steps > 0 &&
-        run_chunks(s, prm->max_steps, 0.0, 2, fire_chunk, s->fire_graphs, true))
+        run_chunks(s, prm->max_steps, 0.0, 2, fire_chunk, s->fire_graphs, true, false))
         return -1;
     hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (scopy(s, &f, d.fire, sizeof(FireCtl), hipMemcpyDeviceToHost)) return -1;
@@ -1704,9 +1796,13 @@ namespace dpm {
 int refresh_rmax(dpm_system* s) {
     std::vector<double4> h(s->N);
     if (scopy(s, h.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost)) return -1;
-    double m = 0.0;
-    for (const auto& q : h) m = std::max(m, q.z);
+    double m = 0.0, mn = HUGE_VAL;
+    for (const auto& q : h) {
+        m = std::max(m, q.z);
+        mn = std::min(mn, q.z);
+    }
     s->rmax = m;
+    s->rmin = mn;
     s->gen++;
     return 0;
 }
@@ -1843,9 +1939,13 @@ int dpm_upload(dpm_system* s, int f, const void* data, int64_t count) {
         if (scopy(s, h.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost)) return -1;
         for (int64_t k = 0; k < s->N; ++k) (&h[k].x)[comp] = src[sidx[k]];
         if (f == DPM_F_RAD) {
-            double m = 0.0;
-            for (int64_t k = 0; k < s->N; ++k) m = std::max(m, h[k].z);
+            double m = 0.0, mn = HUGE_VAL;
+            for (int64_t k = 0; k < s->N; ++k) {
+                m = std::max(m, h[k].z);
+                mn = std::min(mn, h[k].z);
+            }
             s->rmax = m;
+            s->rmin = mn;
             s->gen++;
         }
         if (scopy(s, s->d.P0, h.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice)) return -1;
@@ -2041,8 +2141,8 @@ int dpm_zero_forces(dpm_system* s) {
 int dpm_compute_forces(dpm_system* s) {
     if (require_nbr(s, "dpm_compute_forces")) return -1;
     if (s->kind == DPM_BUMPY) return bumpy_forces(s, 1);
-    hipLaunchKernelGGL(k_disk_forces_acc, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
-                       params_of(s));
+    PairParams p = params_of(s);
+    LAUNCH_HOT(k_disk_forces_acc, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p);
     return check_launch();
 }
 

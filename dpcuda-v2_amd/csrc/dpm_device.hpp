@@ -45,8 +45,40 @@ struct PairParams {
     double inv2nc;          // 1/(2 n_c) when 2 n_c is a power of two (exact), else 0
     double two_nc;
     int fast;
-    int pad;
+    int fastdiv;            // n_c == 2 and e_c, every contact radius within [2^-100, 2^100]
 };
+
+// Correctly rounded square root and quotient for operands known to be normal numbers
+// of moderate size.  These are the refinement sequences the compiler itself emits for
+// IEEE sqrt and division on this target (hardware reciprocal-square-root / reciprocal
+// seed, Newton refinement of the seed, residual corrections with FMA) without its
+// operand pre-scaling and special-value fix-up stages.  Those stages change the result
+// only for zero, infinite, NaN or subnormal operands, arguments below 2^-767 (sqrt),
+// or quotients within 2^-768 of the overflow or underflow thresholds; pair_core uses
+// these only when 2^-700 <= d2 and the host has bounded e_c and the radii, so every
+// call returns the same bits as the IEEE operation.
+__device__ __forceinline__ double sqrt_rn_normal(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double e = fma(-g, g, x);
+    g = fma(e, h, g);
+    e = fma(-g, g, x);
+    return fma(e, h, g);
+}
+
+__device__ __forceinline__ double div_rn_normal(double a, double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    double q = a * y;
+    double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
 
 // overlap^(n_c-1): n_c == 2 gives exactly overlap.
 __device__ __forceinline__ double pow_ov(double ov, const PairParams& p) {
@@ -65,12 +97,20 @@ __device__ __forceinline__ double pair_core(double dx, double dy, double r, doub
         fy = 0.0;
         return 0.0;
     }
-    double d = sqrt(d2);
-    double ov = 1.0 - (d / rs);
-    double op = pow_ov(ov, p);
+    double ov, op, fm;
+    if (p.fastdiv && d2 >= 0x1p-700) {
+        double d = sqrt_rn_normal(d2);
+        ov = 1.0 - div_rn_normal(d, rs);
+        op = ov;
+        fm = div_rn_normal(p.e_c * op, rs * d);
+    } else {
+        double d = sqrt(d2);
+        ov = 1.0 - (d / rs);
+        op = pow_ov(ov, p);
+        fm = p.e_c * op / (rs * d);
+    }
     double num = p.e_c * ov * op;
     double e = p.inv2nc != 0.0 ? num * p.inv2nc : num / p.two_nc;
-    double fm = p.e_c * op / (rs * d);
     fx = fm * dx;
     fy = fm * dy;
     return e;
@@ -82,6 +122,45 @@ __device__ __forceinline__ double pair_law(double px, double py, double r, doubl
     double dx = pbc(px, ox, p.Lx, p.invLx, p.halfLx, p.fast);
     double dy = pbc(py, oy, p.Ly, p.invLy, p.halfLy, p.fast);
     return pair_core(dx, dy, r, orad, p, fx, fy);
+}
+
+// Minimum image without a branch: d - L*rint(d/L) (round half to even).  It equals the
+// reference's d - L*round(d/L) (half away from zero) except when d/L lies within an
+// ulp of a half-integer, where both results have magnitude L/2, beyond every contact
+// and list cutoff whenever the caller may use it (PairParams::fast).
+__device__ __forceinline__ double pbc_rint(double a, double b, double L, double invL) {
+    double d = a - b;
+    double k = __builtin_rint(d * invL);
+    return d - L * k;
+}
+
+// Pair force (and, with ENERGY, half pair energy) of one listed neighbour for the hot
+// loops: calcPointPointInteraction (include/particles/base/kernels.cuh:154-183) with
+// the branch-free minimum image, the contact test folded into selects and the short
+// sqrt/division sequences; bit-identical to pair_law.  `valid` false (a slot past the
+// row end) contributes +0.0, which leaves the running sums unchanged.  Needs
+// p.fast && p.fastdiv; pair_law covers the other cases.
+template <bool ENERGY>
+__device__ __forceinline__ double pair_hot(double4 qi, double4 q, bool valid, const PairParams& p,
+                                           double& fx, double& fy) {
+    double dx = pbc_rint(qi.x, q.x, p.Lx, p.invLx);
+    double dy = pbc_rint(qi.y, q.y, p.Ly, p.invLy);
+    double rs = qi.z + q.z;
+    double d2 = dx * dx + dy * dy;
+    bool contact = valid && d2 < rs * rs;
+    double d = sqrt_rn_normal(d2);
+    double ov = 1.0 - div_rn_normal(d, rs);
+    double fm = div_rn_normal(p.e_c * ov, rs * d);
+    if (contact && d2 < 0x1p-700) {  // separations the short sequences do not cover
+        d = sqrt(d2);
+        ov = 1.0 - (d / rs);
+        fm = p.e_c * ov / (rs * d);
+    }
+    fx = contact ? fm * dx : 0.0;
+    fy = contact ? fm * dy : 0.0;
+    if (!ENERGY) return 0.0;
+    double num = p.e_c * ov * ov;
+    return contact ? num * p.inv2nc : 0.0;
 }
 
 // Pairwise-halving tree over the 64 lanes (lane t += lane t+s, s = 32..1);
@@ -115,6 +194,22 @@ __device__ __forceinline__ double block_max(double v) {
     if (threadIdx.x == 0) {
         r = sm[0];
         for (int k = 1; k < (int)(blockDim.x / WAVE); ++k) r = fmax(r, sm[k]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Block-wide max of an int (all threads must call); result valid in thread 0.
+__device__ __forceinline__ int block_max_int(int v) {
+    __shared__ int smi[BLOCK / WAVE];
+    v = wave_max_i(v);
+    int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    if (l == 0) smi[w] = v;
+    __syncthreads();
+    int r = 0;
+    if (threadIdx.x == 0) {
+        r = smi[0];
+        for (int k = 1; k < (int)(blockDim.x / WAVE); ++k) r = max(r, smi[k]);
     }
     __syncthreads();
     return r;

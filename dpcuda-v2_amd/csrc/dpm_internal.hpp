@@ -134,6 +134,7 @@ struct dpm_system {
     double box[2] = {1.0, 1.0};
     double e_c = 1.0, n_c = 2.0;
     double rmax = 0.0;          // largest radius (decides the fast minimum image)
+    double rmin = 0.0;          // smallest radius (decides the short sqrt/division path)
     dpm::Dev d{};
     dpm::Spare sp{};
     std::vector<void*> allocs;
