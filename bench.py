@@ -6,20 +6,23 @@ square box, harmonic contacts (e_c=1, n_c=2, m=1), seeded uniform positions rela
 by 200 FIRE steps, Gaussian velocities at T=1e-5, dt=0.01, cell list with
 rc = 1.4 + 0.3 and rebuild when the max displacement exceeds 0.15.
 
-One step = one velocity-Verlet step of the hot path over all N particles (one
-launch of the fused force + kick + kick + drift kernel, plus the gated rebuild
-chain at the end of each chunk of steps, which rebuilds the lists when a step
-needs it).  `value` is particle-steps/s over exactly --steps steps with the state
-resident in HBM, bracketed by device synchronisation.
+One step = one velocity-Verlet step of the hot path over all N particles: one
+launch of the fused force + kick + kick + drift kernel, which also rebuilds the
+neighbour lists in place when a step needs it; a step that needs a cell rebuild
+halts its chunk and is finished by the rebuild chain.  `value` is particle-steps/s
+over exactly --steps steps with the state resident in HBM, bracketed by device
+synchronisation.
 
-The roofline object covers the fused step kernel (the dominant one): algorithmic
-bytes per launch = N * (132 + 4*k_mean): per particle the 32 B {x,y,r,m} record
-read and written (ping-pong), velocity read and written (32 B), both displacement
-references read (32 B), the row length (4 B) and k_mean 4-byte neighbour indices;
-the neighbours' records are gathered from spatially sorted memory and counted
-once, in the particle's own read.  It is divided by the kernel's mean duration
-from per-launch HIP events recorded on the system's own stream during a second
-pass of the same --steps steps.
+The roofline object covers the fused step kernel (the dominant one; one launch is
+one NVE step of all N particles).  Algorithmic bytes per launch use the fixed
+per-particle-step figure for a fused NVE step, 196 + 4*k_mean (kick+drift 104, force
++ closing kick 92 + 4*k_mean; neighbour gathers not counted again), times N.  The
+kernel's own compulsory traffic is lower, 132 + 4*k_mean (it never stores the forces
+of intermediate steps), and is reported beside it.  The time is the kernel's total
+duration from HIP events recorded on the system's own stream around every chunk of
+steps (a hipGraph of fused launches) inside the timed region, divided by the number
+of steps: launches that only skip (the rest of a chunk after a step halted for a
+cell rebuild) are charged to the steps, not averaged in.
 
 cpu_baseline: the CPU oracle (C++ restatement of the reference, OpenMP) timed on a
 bounded sample of the same workload on this host.
@@ -84,7 +87,7 @@ def cpu_baseline(P, W, n, seed, steps):
 
 
 def traffic_from_profiles(n):
-    path = os.path.join(ROOT, "profiles", "nve_fused_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_nve_fused_n262144_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -139,6 +142,10 @@ def main():
         s.nve_run(0.01, args.warmup)
     s.synchronize()
     r0 = s.neighbor_stats()
+    # HIP events around every graph-launched chunk of the timed region (on the
+    # system's stream): the fused step kernel's time, chunk by chunk
+    s.set_option(P.OPT_TIMING, 2)
+    s.reset_kernel_time()
     print(f"[bench] timed region: {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     s.nve_run(0.01, args.steps)
@@ -147,20 +154,16 @@ def main():
     r1 = s.neighbor_stats()
     elapsed = t1 - t0
     value = n * args.steps / elapsed
-
-    # kernel durations: a second pass of the same length with per-launch HIP events
-    print("[bench] kernel-timing pass", file=sys.stderr, flush=True)
-    s.set_option(P.OPT_TIMING, 1)
-    s.reset_kernel_time()
-    s.nve_run(0.01, args.steps)
     f_ms, f_n = s.kernel_time(P.K_FORCE)
     r_ms, r_n = s.kernel_time(P.K_REBUILD)
     s.set_option(P.OPT_TIMING, 0)
     stats = s.neighbor_stats()
     k_mean = stats["total_entries"] / n
-    step_bytes = n * (132.0 + 4.0 * k_mean)
-    f_avg_s = f_ms / max(f_n, 1) / 1e3
-    f_gbs = step_bytes / f_avg_s / 1e9
+    step_bytes = n * (196.0 + 4.0 * k_mean)
+    own_bytes = n * (132.0 + 4.0 * k_mean)
+    chunk_steps = max(args.steps - 1, 1)  # nve_run(n): n - 1 steps run in chunks
+    f_step_s = f_ms / chunk_steps / 1e3
+    f_gbs = step_bytes / f_step_s / 1e9
     r_avg_us = r_ms / max(r_n, 1) * 1e3
 
     cpu = None
@@ -201,10 +204,11 @@ def main():
             "frac": f_gbs / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(n),
             "algorithmic_bytes_per_launch": step_bytes,
-            "mean_launch_us": f_avg_s * 1e6,
-            "launches": f_n,
+            "kernel_compulsory_bytes_per_launch": own_bytes,
+            "kernel_us_per_step": f_step_s * 1e6,
+            "timed_chunks": f_n,
         },
-        "rebuild_chain": {"mean_us_per_chunk": r_avg_us, "chunks": r_n},
+        "cell_rebuild_recovery": {"mean_us": r_avg_us, "count": r_n},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -165,9 +165,11 @@ __device__ __forceinline__ void row_forces(const Dev& d, const double4* P, const
         double4 q[ROW_BATCH];
 #pragma unroll
         for (int t = 0; t < ROW_BATCH; ++t) q[t] = P[jn[t]];
+        if (k + ROW_BATCH < wmax) {  // next batch's indices (wave-uniform test)
 #pragma unroll
-        for (int t = 0; t < ROW_BATCH; ++t)
-            jn[t] = col[(int64_t)min(k + ROW_BATCH + t, cap - 1) * N];
+            for (int t = 0; t < ROW_BATCH; ++t)
+                jn[t] = col[(int64_t)min(k + ROW_BATCH + t, cap - 1) * N];
+        }
 #pragma unroll
         for (int t = 0; t < ROW_BATCH; ++t) {
             if (k + t < wmax) {
@@ -1311,42 +1313,58 @@ int max_to_host(dpm_system* s, int what, double* out) {
 }
 
 // =========================================================================
-//                         timing (per-launch HIP events)
+//                 timing (HIP events per launch or per chunk)
 // =========================================================================
+// Events come from a free list; a pair returns to it once its time is collected.
 static hipEvent_t pool_event(dpm_system* s) {
-    if (s->ev_next >= s->ev_pool.size()) {
+    if (s->ev_free.empty()) {
         hipEvent_t e;
         (void)hipEventCreate(&e);
         s->ev_pool.push_back(e);
+        return e;
     }
-    return s->ev_pool[s->ev_next++];
+    hipEvent_t e = s->ev_free.back();
+    s->ev_free.pop_back();
+    return e;
 }
 
 Timed::Timed(dpm_system* s_, int cls_) : s(s_), cls(cls_) {
-    if (!s->timing) return;
+    if (s->timing != 1) return;
     a = pool_event(s);
     b = pool_event(s);
     (void)hipEventRecord(a, s->stream);
 }
 
 Timed::~Timed() {
-    if (!s->timing) return;
+    if (s->timing != 1) return;
     (void)hipEventRecord(b, s->stream);
     s->ev_pending.push_back({cls, {a, b}});
 }
 
+static void account(dpm_system* s, const std::pair<int, std::pair<hipEvent_t, hipEvent_t>>& pe) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+        s->ktime_ms[pe.first] += ms;
+        s->klaunch[pe.first] += 1;
+    }
+    s->ev_free.push_back(pe.second.first);
+    s->ev_free.push_back(pe.second.second);
+}
+
+// Collects every pending pair (waits for the stream).
 void collect_timing(dpm_system* s) {
     if (s->ev_pending.empty()) return;
     (void)hipStreamSynchronize(s->stream);
-    for (auto& pe : s->ev_pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
-            s->ktime_ms[pe.first] += ms;
-            s->klaunch[pe.first] += 1;
-        }
-    }
+    for (auto& pe : s->ev_pending) account(s, pe);
     s->ev_pending.clear();
-    s->ev_next = 0;
+}
+
+// Collects the pairs whose end event has completed, without waiting (chunk timing).
+static void collect_finished(dpm_system* s) {
+    size_t k = 0;
+    while (k < s->ev_pending.size() && hipEventQuery(s->ev_pending[k].second.second) == hipSuccess)
+        account(s, s->ev_pending[k++]);
+    s->ev_pending.erase(s->ev_pending.begin(), s->ev_pending.begin() + k);
 }
 
 // =========================================================================
@@ -1574,8 +1592,14 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
     while (done < total || need_recovery) {
         while (q.size() < 2 && (need_recovery || planned < total)) {
             Unit u{0, 0, false, false};
+            hipEvent_t ua = nullptr, ub = nullptr;
+            if (s->timing == 2) {
+                ua = pool_event(s);
+                ub = pool_event(s);
+                DPM_HIP(hipEventRecord(ua, s->stream));
+            }
             if (need_recovery) {
-                if (s->use_graph && !s->timing) {
+                if (s->use_graph && s->timing != 1) {
                     if (ensure_graph(s, cache, -1, dt, recovery_fill)) return -1;
                     DPM_HIP(hipGraphLaunch(cache[-1].exec, s->stream));
                 } else {
@@ -1588,7 +1612,7 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
                 need_recovery = false;
             } else {
                 int64_t c = std::min<int64_t>(s->chunk, total - planned);
-                if (s->use_graph && !s->timing && c == s->chunk) {
+                if (s->use_graph && s->timing != 1 && c == s->chunk) {
                     if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
                     DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
                 } else {
@@ -1597,6 +1621,10 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
                 }
                 u.c = c;
                 planned += c;
+            }
+            if (s->timing == 2) {
+                DPM_HIP(hipEventRecord(ub, s->stream));
+                s->ev_pending.push_back({u.recovery ? DPM_K_REBUILD : DPM_K_FORCE, {ua, ub}});
             }
             u.slot = next_slot;
             next_slot = (next_slot + 1) % 4;
@@ -1609,7 +1637,8 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
         Unit f = q.front();
         q.pop_front();
         DPM_HIP(hipEventSynchronize(s->poll_ev[f.slot]));
-        if (s->timing) collect_timing(s);
+        if (s->timing == 1) collect_timing(s);
+        else if (s->timing == 2) collect_finished(s);
         Ctl hc = s->hctl[f.slot];
         done += hc.last_progress;
         planned -= (f.recovery ? 1 : f.c) - hc.last_progress;
@@ -2560,7 +2589,13 @@ int dpm_set_option(dpm_system* s, int opt, int64_t v) {
             }
             s->chunk = v;
             return 0;
-        case DPM_OPT_TIMING: s->timing = v != 0; return 0;
+        case DPM_OPT_TIMING:
+            if (v < 0 || v > 2) {
+                set_error("dpm_set_option: timing must be 0, 1 or 2");
+                return -1;
+            }
+            s->timing = (int)v;
+            return 0;
     }
     set_error("dpm_set_option: unknown option");
     return -1;

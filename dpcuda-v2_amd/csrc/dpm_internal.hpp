@@ -159,7 +159,7 @@ struct dpm_system {
     // execution options
     bool use_graph = true;
     int64_t chunk = 64;
-    bool timing = false;
+    int timing = 0;             // 0 off, 1 per launch, 2 per chunk
     uint64_t gen = 0;           // bumped whenever baked kernel arguments change
     std::map<int64_t, dpm::Graph> nve_graphs, fire_graphs;
     dpm::Ctl* hctl = nullptr;   // pinned host mirrors of the control block (4 slots)
@@ -167,6 +167,6 @@ struct dpm_system {
     int64_t klaunch[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-    size_t ev_next = 0;
+    std::vector<hipEvent_t> ev_free;
     hipEvent_t poll_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };

@@ -191,8 +191,10 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* n_vertices_per_particle, doubl
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
  * DPM_OPT_CHUNK: steps per host poll of the device status (default 64).
- * DPM_OPT_TIMING: record HIP events around every launch of the hot kernels (default 0);
- *   dpm_kernel_time returns the summed milliseconds and launch count per kernel class. */
+ * DPM_OPT_TIMING: 0 off (default); 1 HIP events around every launch of the hot kernels
+ *   (disables graph capture); 2 HIP events around every graph-launched chunk of the NVE and
+ *   FIRE loops (chunk time is charged to DPM_K_FORCE, recovery chains to DPM_K_REBUILD).
+ *   dpm_kernel_time returns the summed milliseconds and the count per kernel class. */
 enum { DPM_OPT_GRAPH = 0, DPM_OPT_CHUNK = 1, DPM_OPT_TIMING = 2 };
 enum { DPM_K_FORCE = 0, DPM_K_DRIFT = 1, DPM_K_REBUILD = 2 };
 int dpm_set_option(dpm_system* s, int option, int64_t value);
