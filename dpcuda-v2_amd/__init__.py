@@ -120,6 +120,7 @@ _SIGS = {
     "minimize_fire": (_I, [_P, C.POINTER(FireParams), C.POINTER(FireResult)]),
     "fire_state_init": (_I, [C.POINTER(FireState), _D, _D]),
     "fire_step": (_I, [_P, C.POINTER(FireState)]),
+    "adam_reset": (_I, [_P]),
     "adam_step": (_I, [_P, _L, _D, _D, _D, _D]),
     "minimize_adam": (_I, [_P, _D, _D, _D, _D, _D, _D, _L, C.POINTER(_L)]),
     "compute_stress": (_I, [_P]),
@@ -339,6 +340,9 @@ class System:
 
     def fire_step(self, st: FireState):
         self._c("fire_step", C.byref(st))
+
+    def adam_reset(self):
+        self._c("adam_reset")
 
     def adam_step(self, step, alpha, beta1=0.9, beta2=0.999, eps=1e-8):
         self._c("adam_step", int(step), float(alpha), float(beta1), float(beta2), float(eps))

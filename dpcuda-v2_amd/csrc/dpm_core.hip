@@ -2354,13 +2354,20 @@ int dpm_adam_step(dpm_system* s, int64_t step, double alpha, double b1, double b
     return check_launch();
 }
 
+int dpm_adam_reset(dpm_system* s) {
+    if (ensure_adam(s)) return -1;
+    // Particle::initAdamVariables (particle.cu:319-322)
+    DPM_HIP(hipMemsetAsync(s->d.m1, 0, sizeof(double2) * s->N, s->stream));
+    DPM_HIP(hipMemsetAsync(s->d.m2, 0, sizeof(double2) * s->N, s->stream));
+    return 0;
+}
+
 int dpm_minimize_adam(dpm_system* s, double alpha, double b1, double b2, double eps, double pe_t,
                       double diff_t, int64_t max_steps, int64_t* steps_out) {
     if (require_nbr(s, "dpm_minimize_adam")) return -1;
     if (ensure_adam(s)) return -1;
     // minimizeAdam (src/routines/minimization.cu:3-33)
-    DPM_HIP(hipMemsetAsync(s->d.m1, 0, sizeof(double2) * s->N, s->stream));
-    DPM_HIP(hipMemsetAsync(s->d.m2, 0, sizeof(double2) * s->N, s->stream));
+    if (dpm_adam_reset(s)) return -1;
     int64_t step = 0;
     double dof = (double)ndof_of(s), last = 0.0;
     while (step < max_steps) {

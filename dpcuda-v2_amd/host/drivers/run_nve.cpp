@@ -1,0 +1,59 @@
+// run_nve -- the flow of scripts/runNVE.cpp:15-55 over the C++ host API: particle,
+// optional Gaussian velocities at `temperature`, NVE with dt = dt_dimless / time unit,
+// num_steps of nve.step() (or one device-resident nve.run with loop=device).
+//
+//   run_nve n_particles=262144 packing_fraction=0.85 num_steps=10000 dt_dimless=0.01
+//           temperature=1e-5 seed=0 fire_steps=200 loop=device
+#include <chrono>
+#include <iostream>
+
+#include "../integrator.hpp"
+#include "../routines.hpp"
+#include "run_args.hpp"
+
+int main(int argc, char** argv) {
+    ConfigDict run_config;
+    run_config["particle_type"] = "Disk";
+    run_config["n_particles"] = 1024L;
+    run_config["packing_fraction"] = 0.8;
+    run_config["seed"] = 0L;
+    run_config["num_steps"] = 10000L;
+    run_config["dt_dimless"] = 0.01;
+    run_config["temperature"] = 1e-4;
+    run_config["fire_steps"] = 200L;
+    run_config["loop"] = "host";
+    run_config = parse_run_args(argc, argv, run_config);
+    try {
+        auto particle = particle_from_args(run_config);
+        long num_steps = run_config["num_steps"].get<long>();
+        double dt_dimless = run_config["dt_dimless"].get<double>();
+        double temperature = run_config["temperature"].get<double>();
+        long fire_steps = run_config["fire_steps"].get<long>();
+        if (fire_steps > 0) minimizeFire(*particle, 0.1, 1e-2, fire_steps, 1000L);
+        if (temperature > 0) particle->setRandomVelocities(temperature);
+
+        ConfigDict nve_config_dict = get_nve_config_dict(dt_dimless / particle->getTimeUnit());
+        NVE nve(*particle, nve_config_dict);
+        double e0 = particle->totalKineticEnergy() + particle->totalPotentialEnergy();
+
+        auto start_time = std::chrono::high_resolution_clock::now();
+        if (run_config["loop"].get<std::string>() == "device") {
+            nve.run(num_steps);
+        } else {
+            for (long step = 0; step < num_steps; ++step) nve.step();
+        }
+        double e1 = particle->totalKineticEnergy() + particle->totalPotentialEnergy();
+        auto end_time = std::chrono::high_resolution_clock::now();
+        double secs = std::chrono::duration<double>(end_time - start_time).count();
+        std::cout.precision(17);
+        std::cout << "steps " << num_steps << " E0/N " << e0 / particle->n_particles << " E1/N "
+                  << e1 / particle->n_particles << " rel drift " << (e1 - e0) / std::abs(e0)
+                  << std::endl;
+        std::cout << "particle-steps/s " << particle->n_particles * num_steps / secs << std::endl;
+        std::cout << "Time taken: " << static_cast<long>(secs) << " seconds" << std::endl;
+    } catch (const std::exception& e) {
+        std::cerr << "run_nve: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

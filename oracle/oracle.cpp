@@ -972,13 +972,18 @@ void adam_step(orc_system* s, long step, double alpha, double b1, double b2, dou
     adam_update(s, step, alpha, b1, b2, eps);
 }
 
-long minimize_adam(orc_system* s, double alpha, double b1, double b2, double eps, double pe_t,
-                   double diff_t, long max_steps) {
-    // minimizeAdam src/routines/minimization.cu:3-33 (initAdamVariables particle.cu:319-322)
+// Particle::initAdamVariables particle.cu:319-322
+void adam_reset(orc_system* s) {
     std::fill(s->m1x.begin(), s->m1x.end(), 0.0);
     std::fill(s->m1y.begin(), s->m1y.end(), 0.0);
     std::fill(s->m2x.begin(), s->m2x.end(), 0.0);
     std::fill(s->m2y.begin(), s->m2y.end(), 0.0);
+}
+
+long minimize_adam(orc_system* s, double alpha, double b1, double b2, double eps, double pe_t,
+                   double diff_t, long max_steps) {
+    // minimizeAdam src/routines/minimization.cu:3-33
+    adam_reset(s);
     long step = 0;
     double dof = (double)s->ndof();
     double last = 0.0;
@@ -1364,6 +1369,10 @@ int orc_minimize_fire(orc_system* s, const orc_fire_params* p, orc_fire_result* 
 int orc_fire_state_init(orc_fire_state* st, double a, double dt) { fire_init(st, a, dt); return 0; }
 int orc_fire_step(orc_system* s, orc_fire_state* st) { fire_step(s, st); return 0; }
 
+int orc_adam_reset(orc_system* s) {
+    adam_reset(s);
+    return 0;
+}
 int orc_adam_step(orc_system* s, int64_t step, double a, double b1, double b2, double eps) {
     adam_step(s, step, a, b1, b2, eps);
     return 0;

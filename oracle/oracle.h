@@ -92,6 +92,7 @@ typedef struct {
 int orc_fire_state_init(orc_fire_state* st, double alpha_init, double dt);
 int orc_fire_step(orc_system* s, orc_fire_state* st);
 
+int orc_adam_reset(orc_system* s);
 int orc_adam_step(orc_system* s, int64_t step, double alpha, double beta1, double beta2, double eps);
 int orc_minimize_adam(orc_system* s, double alpha, double beta1, double beta2, double eps,
                       double avg_pe_target, double avg_pe_diff_target, int64_t max_steps,
