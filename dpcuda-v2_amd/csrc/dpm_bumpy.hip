@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "dpm_kernels.hpp"
+#include "dpm_trig.hpp"
 
 namespace dpm {
 
@@ -149,7 +150,8 @@ __global__ void k_bumpy_drift_vertices(Dev d) {
     double2 o = d.lpos[p], n = make_double2(nq.x, nq.y), x = d.vpos[v];
     double dth = d.dth[p];
     double rx = x.x - o.x, ry = x.y - o.y;
-    double c = cos(dth), sn = sin(dth);
+    double sn, c;
+    dpm_sincos(dth, &sn, &c);
     double rxr = rx * c - ry * sn;
     double ryr = rx * sn + ry * c;
     d.vpos[v] = make_double2(n.x + rxr, n.y + ryr);
@@ -363,7 +365,9 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
     double2 c = make_double2(cq.x, cq.y);
     for (int i = 0; i < nv; ++i) {
         double va = i * inc + a;
-        d.vpos[st + i] = make_double2(c.x + inner * cos(va), c.y + inner * sin(va));
+        double sv, cv;
+        dpm_sincos(va, &sv, &cv);
+        d.vpos[st + i] = make_double2(c.x + inner * cv, c.y + inner * sv);
         d.vpart[st + i] = (int)p;
     }
     double vm = cq.w / static_cast<double>(nv);

@@ -24,7 +24,12 @@
 //   * scalar sums use one fixed reduction tree (canonical_sum below) instead
 //     of CUB's; the HIP kernels use the same tree, so GPU and oracle sums are
 //     bit-identical.  The reference's CUB tree order is not reproducible here.
+//   * sin/cos of the rigid-body rotations come from one shared implementation
+//     (dpm_sincos, Cody-Waite reduction + Taylor polynomials, within 2.5 ulp of
+//     the exact values) compiled into both the HIP kernels and this oracle; the
+//     reference calls CUDA's sin/cos, which no CPU library reproduces bit for bit.
 #include "oracle.h"
+#include "../dpcuda-v2_amd/csrc/dpm_trig.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -560,7 +565,8 @@ void drift(orc_system* s, double dt) {
         for (long v = 0; v < s->Nv; ++v) {
             long p = s->vpart[v];
             double rx = s->vxp[v] - s->lpx[p], ry = s->vyp[v] - s->lpy[p];
-            double c = std::cos(s->dth[p]), sn = std::sin(s->dth[p]);
+            double sn, c;
+            dpm_sincos(s->dth[p], &sn, &c);
             double rxr = rx * c - ry * sn;
             double ryr = rx * sn + ry * c;
             s->vxp[v] = s->x[p] + rxr;
@@ -1369,6 +1375,11 @@ int orc_minimize_fire(orc_system* s, const orc_fire_params* p, orc_fire_result* 
 int orc_fire_state_init(orc_fire_state* st, double a, double dt) { fire_init(st, a, dt); return 0; }
 int orc_fire_step(orc_system* s, orc_fire_state* st) { fire_step(s, st); return 0; }
 
+int orc_sincos(const double* x, double* s_out, double* c_out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) dpm_sincos(x[i], &s_out[i], &c_out[i]);
+    return 0;
+}
+
 int orc_adam_reset(orc_system* s) {
     adam_reset(s);
     return 0;
@@ -1453,8 +1464,10 @@ int orc_bumpy_setup(orc_system* s, const int64_t* nvp, double rv, int rotation) 
         for (long i = 0; i < nv; ++i) {
             s->vpart[st + i] = p;
             double va = i * inc + a;
-            s->vxp[st + i] = s->x[p] + inner * std::cos(va);
-            s->vyp[st + i] = s->y[p] + inner * std::sin(va);
+            double sv, cv;
+            dpm_sincos(va, &sv, &cv);
+            s->vxp[st + i] = s->x[p] + inner * cv;
+            s->vyp[st + i] = s->y[p] + inner * sv;
         }
     }
     for (long v = 0; v < s->Nv; ++v) {

@@ -93,6 +93,8 @@ int orc_fire_state_init(orc_fire_state* st, double alpha_init, double dt);
 int orc_fire_step(orc_system* s, orc_fire_state* st);
 
 int orc_adam_reset(orc_system* s);
+/* the sin/cos pair used for rigid-body rotations (shared with the HIP kernels) */
+int orc_sincos(const double* x, double* s_out, double* c_out, int64_t n);
 int orc_adam_step(orc_system* s, int64_t step, double alpha, double beta1, double beta2, double eps);
 int orc_minimize_adam(orc_system* s, double alpha, double beta1, double beta2, double eps,
                       double avg_pe_target, double avg_pe_diff_target, int64_t max_steps,

@@ -448,3 +448,40 @@ def test_bumpy_torque_balance_and_contact():
     x, y = np.array([5.0, 6.15]), np.array([5.0, 5.05])
     total = (tq + x * fy - y * fx).sum()
     assert abs(total) < 1e-13
+
+
+# ---------------------------------------------------------------- shared sin/cos
+def _sincos(x):
+    import ctypes as C
+
+    from _helpers import ORACLE_SO
+    lib = C.CDLL(ORACLE_SO)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, c = np.empty_like(x), np.empty_like(x)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.orc_sincos.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+    assert lib.orc_sincos(ptr(x), ptr(s), ptr(c), x.size) == 0
+    return s, c
+
+
+@pytest.mark.parametrize("lim,max_ulp", [(math.pi / 4, 1.0), (7.0, 2.0), (100.0, 2.0),
+                                         (1e5, 3.0)])
+def test_shared_sincos_against_libm(lim, max_ulp):
+    # dpm_sincos (rigid-body rotations on both sides) vs the C library's correctly
+    # rounded-to-0.5-ulp sin/cos: together at most max_ulp apart
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-lim, lim, 200000),
+                        [0.0, -0.0, 1e-300, 5e-324, math.pi / 4, math.pi / 2, math.pi,
+                         -math.pi / 2, 3 * math.pi / 2, 2 * math.pi]])
+    x = x[np.abs(x) <= lim]
+    s, c = _sincos(x)
+    for got, ref in ((s, np.sin(x)), (c, np.cos(x))):
+        err = np.abs(got - ref) / np.spacing(np.maximum(np.abs(ref), 2.0 ** -1022))
+        assert err.max() <= max_ulp, (lim, err.max())
+    assert np.all(s * s + c * c == pytest.approx(1.0, abs=4e-16))
+    # exact special values and symmetries of the reduction
+    s0, c0 = _sincos(np.array([0.0, -0.0]))
+    assert s0[0] == 0.0 and c0[0] == 1.0 and math.copysign(1, s0[1]) == -1.0
+    sp, cp = _sincos(x)
+    sn, cn = _sincos(-x)
+    assert np.array_equal(sn, -sp) and np.array_equal(cn, cp)
