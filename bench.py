@@ -86,6 +86,121 @@ def cpu_baseline(P, W, n, seed, steps):
                       f"{dt:.2f} s"}
 
 
+# ---------------------------------------------------------------- rigid bumpy FIRE
+RIGID_N = 65536
+
+
+def prepare_rigid(P, W, n, seed, backend=None):
+    st = W.bumpy_config(n, 0.80, seed=seed)
+    return st, W.make_bumpy_system(P.System, st, backend=backend,
+                                   e_c=W.bumpy_contact_scale(st, 1.0))
+
+
+def rigid_fire(args, P, W):
+    """FIRE of N=65,536 rigid bumpy particles to |F|max < 1e-12 (one device-resident
+    minimizeFire).  value = particle FIRE steps per second over the whole
+    minimisation."""
+    n = args.n if args.n != 262144 else RIGID_N
+    print(f"[bench] rigid bumpy FIRE N={n}", file=sys.stderr, flush=True)
+    st, s = prepare_rigid(P, W, n, args.seed)
+    s.set_option(P.OPT_CHUNK, args.chunk)
+    prm = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=1e-12,
+                        max_steps=args.fire_max_steps)
+    s.synchronize()
+    r0 = s.neighbor_stats()
+    t0 = time.perf_counter()
+    r = s.minimize_fire(prm)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    r1 = s.neighbor_stats()
+    s.compute_stress()
+    pressure = s.reduce(P.RED_PRESSURE)
+    pe = s.reduce(P.RED_PE)
+    phi = s.reduce(P.RED_PACKING_FRACTION)
+    # per-launch events on a converged state (no rebuilds): the force phase of a step
+    # (vertex forces + per-particle sums and closing kick) and the move phase
+    s.set_option(P.OPT_TIMING, 1)
+    s.reset_kernel_time()
+    probe = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=0.0,
+                          max_steps=args.steps_probe)
+    s.minimize_fire(probe)
+    f_ms, f_n = s.kernel_time(P.K_FORCE)
+    m_ms, m_n = s.kernel_time(P.K_DRIFT)
+    s.set_option(P.OPT_TIMING, 0)
+    nv = st.nv
+    kv = len(s.vertex_pairs()[0]) / nv
+    force_us = f_ms / max(f_n, 1) * 1e3
+    move_us = m_ms / max(m_n, 1) * 1e3
+    # algorithmic bytes of the force phase per step: per vertex 16 (position) + 4 (owner)
+    # + 4 (row length) + 32 (force, torque, energy written) + 4 k_v (row), and again 32
+    # read back by the per-particle sums; per particle 8 (vertex range) + 32 (mass) +
+    # 32 (force, torque, energy written) + 24 + 24 (velocity, spin read and written) + 8
+    # (moment of inertia)
+    force_bytes = nv * (56.0 + 4.0 * kv + 32.0) + n * (8 + 32 + 32 + 48 + 8)
+    force_gbs = force_bytes / (force_us * 1e-6) / 1e9 if force_us > 0 else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
+        if os.path.exists(oracle_so):
+            be = P.Backend(oracle_so, "orc_", gpu=False)
+            _, so = prepare_rigid(P, W, n, args.seed, backend=be)
+            k = args.cpu_fire_steps
+            c0 = time.perf_counter()
+            so.minimize_fire(P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                           fmax_target=1e-12, max_steps=k))
+            cdt = time.perf_counter() - c0
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            cpu = {"value": n * k / cdt, "unit": "particle-FIRE-steps/s", "cores": threads,
+                   "kind": "port",
+                   "sample": f"the first {k} FIRE steps of the same N={n} minimisation, C++ "
+                             f"oracle with OpenMP, {threads} threads, {cdt:.2f} s"}
+    out = {
+        "metric": "FP64 rigid bumpy FIRE particle-steps/s to |F|max<1e-12 at N=65,536",
+        "value": n * r.steps / elapsed,
+        "unit": "particle-FIRE-steps/s",
+        "n_gpus": 1,
+        "steps": int(r.steps),
+        "warmup": 0,
+        "ms_per_step": elapsed / max(r.steps, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform positions and orientations)",
+        "config": {
+            "workload": "config 3: FP64 FIRE of N=65536 rigid bumpy particles (diameters "
+                        "1.0 / 1.4 with 20 / 28 vertices, 50/50), phi=0.80, vertex radius "
+                        "from the touching rule, contact scale (2 r_v / d_min)^2, to "
+                        "|F|max < 1e-12, cell list",
+            "n_particles": n,
+            "n_vertices": nv,
+            "converged": bool(r.converged),
+            "time_to_converge_s": elapsed,
+            "final_pe_per_particle": pe / n,
+            "final_pressure": pressure,
+            "final_phi": phi,
+            "final_fmax": r.fmax_sq ** 0.5,
+            "rebuilds": r1["n_rebuilds"] - r0["n_rebuilds"],
+            "cell_rebuilds": r1["n_cell_rebuilds"] - r0["n_cell_rebuilds"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bfire_vforce + k_bfire_reduce (force phase of one FIRE step)",
+            "achieved": force_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": force_gbs / HBM_PEAK_GBS if force_gbs else None,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": force_bytes,
+            "force_phase_us": force_us,
+            "move_phase_us": move_us,
+            "vertex_row_mean": kv,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def traffic_from_profiles(n):
     path = os.path.join(ROOT, "profiles", "k_nve_fused_n262144_traffic.json")
     if not os.path.exists(path):
@@ -110,6 +225,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid"), default="nve-disk",
+                    help="nve-disk: the headline NVE benchmark (default); fire-rigid: FIRE "
+                         "of N=65,536 rigid bumpy particles to |F|max < 1e-12")
+    ap.add_argument("--fire-max-steps", type=int, default=200000)
+    ap.add_argument("--steps-probe", type=int, default=200)
+    ap.add_argument("--cpu-fire-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (torch) five times; its "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -133,6 +254,9 @@ def main():
         sys.exit(2)
 
     P, W = load_pkg()
+    if args.workload == "fire-rigid":
+        rigid_fire(args, P, W)
+        return
     n = args.n
     print(f"[bench] preparing N={n} (FIRE 200 steps, T=1e-5)", file=sys.stderr, flush=True)
     s = prepare(P, W, n, args.seed)

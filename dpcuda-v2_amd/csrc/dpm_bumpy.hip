@@ -17,12 +17,11 @@
 namespace dpm {
 
 // ------------------------------------------------------------------ kernels
-// kernelCalcRigidBumpyForces1, src/particles/rigid_bumpy/kernels.cu:366-417
-// mode 0: write (arrays known zero), 1: accumulate (+=)
-__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams p, double rv,
-                                                               int mode) {
-    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (v >= d.Nv) return;
+// kernelCalcRigidBumpyForces1, src/particles/rigid_bumpy/kernels.cu:366-417: the force,
+// torque and energy of vertex v from its vertex row.  mode 0: write (arrays known zero),
+// 1: accumulate (+=)
+__device__ __forceinline__ void vertex_force_one(const Dev& d, const PairParams& p, double rv,
+                                                 int64_t v, int mode) {
     int nn = min(d.vnum[v], d.vcap);
     if (nn == 0) {
         if (mode == 0) {
@@ -60,6 +59,13 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams
         d.vtq[v] += t;
         d.vpe[v] += e;
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_forces(Dev d, PairParams p, double rv,
+                                                               int mode) {
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    vertex_force_one(d, p, rv, v, mode);
 }
 
 // kernelCalcRigidBumpyParticleForces1, rigid_bumpy/kernels.cu:419-442 (sequential
@@ -158,10 +164,8 @@ __global__ void k_bumpy_drift_vertices(Dev d) {
 }
 
 // kernelMixRigidVelocitiesAndForces, rigid_bumpy/kernels.cu:89-121
-__global__ void k_bumpy_mix(Dev d, double alpha) {
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= d.N) return;
-    double2 f = d.frc[i], v = d.vel[i];
+__device__ __forceinline__ void rigid_mix_one(double2& v, double2 f, double& w, double t,
+                                              double alpha) {
     double fn = sqrt(f.x * f.x + f.y * f.y);
     double vn = sqrt(v.x * v.x + v.y * v.y);
     double mr = 0.0;
@@ -171,7 +175,6 @@ __global__ void k_bumpy_mix(Dev d, double alpha) {
         v.x = 0.0;
         v.y = 0.0;
     }
-    double w = d.omega[i], t = d.torque[i];
     double tn = fabs(t), wn = fabs(w), tmr = 0.0;
     if (tn > 1e-16) {
         tmr = wn / tn * alpha;
@@ -179,8 +182,18 @@ __global__ void k_bumpy_mix(Dev d, double alpha) {
         w = 0.0;
         tmr = 0.0;
     }
-    d.vel[i] = make_double2(v.x * (1 - alpha) + f.x * mr, v.y * (1 - alpha) + f.y * mr);
-    d.omega[i] = w * (1 - alpha) + t * tmr;
+    v = make_double2(v.x * (1 - alpha) + f.x * mr, v.y * (1 - alpha) + f.y * mr);
+    w = w * (1 - alpha) + t * tmr;
+}
+
+__global__ void k_bumpy_mix(Dev d, double alpha) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= d.N) return;
+    double2 v = d.vel[i];
+    double w = d.omega[i];
+    rigid_mix_one(v, d.frc[i], w, d.torque[i], alpha);
+    d.vel[i] = v;
+    d.omega[i] = w;
 }
 
 __device__ __forceinline__ bool within_v(double ax, double ay, double bx, double by,
@@ -380,6 +393,212 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
     d.moi[p] = I;
 }
 
+// ------------------------------------------------------------------ FIRE loop
+// One FIRE iteration of a rigid bumpy system (Fire::step, src/integrator/fire.cu:13-46,
+// with RigidBumpy's updatePositions / updateVelocities / mixVelocitiesAndForces /
+// calculateForces) as a chain of launches that a graph replays; each kernel reads the
+// control block, so a halted or finished loop turns the rest of the chunk into no-ops:
+//   k_bfire_move      particles: [uphill: drift(-dt/2), zero v and w] + kick(dt/2) + mix
+//                     + drift(dt); records both rigid moves and the displacement maxima
+//   k_bfire_vertices  vertices: the rotations of those moves, in the same order
+//   k_bfire_decide    the neighbour check of the step: a list rebuild runs inside the
+//                     chunk, a cell rebuild halts the loop until the chunk's end
+//   k_build_cell / k_build_verlet, k_bumpy_vertex_build   (gated on that decision)
+//   k_bfire_check     an in-chunk rebuild that overflowed the capacity halts the loop
+//   k_bfire_vforce    vertices: forces, torques, energies (kernelCalcRigidBumpyForces1)
+//   k_bfire_reduce    particles: vertex sums (kernelCalcRigidBumpyParticleForces1),
+//                     kick(dt/2), chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2)
+//   k_fire_ctl        stop test of the step and Fire::step's decision for the next
+__device__ __forceinline__ bool bfire_skip(const Dev& d, int gate) {
+    const Ctl* c = d.ctl;
+    if (gate == G_NORMAL) return c->halt || d.fire->done;
+    if (gate == G_RESUME) return !c->halt || !resumable(d) || c->halt_mode != H_FIRE;
+    return false;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bfire_move(Dev d, NbrParams nb, int pos,
+                                                      int rotation) {
+    Ctl* c = d.ctl;
+    if (c->halt) return;
+    const FireCtl* F = d.fire;
+    if (F->done) return;
+    long long e = c->epoch;
+    double dt = F->dt, alpha = F->alpha;
+    int uphill = F->uphill;
+    int tile;
+    int64_t i = particle_of_thread(tile);
+    double nd = 0.0, cd = 0.0;
+    if (i < d.N) {
+        double4 q = d.P0[i];
+        double2 v = d.vel[i], f = d.frc[i];
+        double a = d.angle[i], w = d.omega[i], t = d.torque[i];
+        double2 old = make_double2(q.x, q.y);
+        double dth0 = 0.0;
+        if (uphill) {  // updatePositions(-dt/2) with the decreased dt; setVelocitiesToZero
+            double bdt = -dt / 2.0;
+            double ddx = v.x * bdt, ddy = v.y * bdt;
+            dth0 = w * bdt;
+            q.x = q.x + ddx;
+            q.y = q.y + ddy;
+            a = a + dth0;
+            v = make_double2(0.0, 0.0);
+            w = 0.0;
+        }
+        d.lpos[i] = old;
+        d.mpos[i] = make_double2(q.x, q.y);
+        d.dth0[i] = dth0;
+        // updateVelocities(dt/2) (kernelUpdateRigidVelocities)
+        double hdt = dt / 2.0;
+        if (rotation) w += t * hdt / d.moi[i];
+        else w = 0.0;
+        v.x += f.x * hdt / q.w;
+        v.y += f.y * hdt / q.w;
+        rigid_mix_one(v, f, w, t, alpha);
+        // updatePositions(dt) (kernelUpdateRigidPositions)
+        double ddx = v.x * dt, ddy = v.y * dt, dda = w * dt;
+        d.dth[i] = dda;
+        double nx = q.x + ddx, ny = q.y + ddy;
+        d.P0[i] = make_double4(nx, ny, q.z, q.w);
+        d.angle[i] = a + dda;
+        d.vel[i] = v;
+        d.omega[i] = w;
+        disp_of(d, i, nx, ny, nd, cd);
+    }
+    disp_epilogue(d, nb, tile, nd, cd, pos & 1, stamp(e, pos));
+}
+
+// kernelTranslateAndRotateVertices1 (rigid_bumpy/kernels.cu:152-201) for the back-step
+// (when uphill) and then for the forward drift
+__global__ void k_bfire_vertices(Dev d) {
+    if (d.ctl->halt || d.fire->done) return;
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    int p = d.vpart[v];
+    double2 x = d.vpos[v], mid = d.mpos[p];
+    if (d.fire->uphill) {
+        double2 o = d.lpos[p];
+        double rx = x.x - o.x, ry = x.y - o.y;
+        double sn, c;
+        dpm_sincos(d.dth0[p], &sn, &c);
+        double rxr = rx * c - ry * sn;
+        double ryr = rx * sn + ry * c;
+        x = make_double2(mid.x + rxr, mid.y + ryr);
+    }
+    double4 nq = d.P0[p];
+    double rx = x.x - mid.x, ry = x.y - mid.y;
+    double sn, c;
+    dpm_sincos(d.dth[p], &sn, &c);
+    double rxr = rx * c - ry * sn;
+    double ryr = rx * sn + ry * c;
+    d.vpos[v] = make_double2(nq.x + rxr, nq.y + ryr);
+}
+
+// Particle::checkForNeighborUpdate of the step (particle.cu:1137-1161) from the
+// requests k_bfire_move recorded: 1 = list (rebuilt in the chunk), 2 = cells + list
+// (the loop halts; the chain at the chunk's end rebuilds and finishes the step).
+__global__ void k_bfire_decide(Dev d, int pos) {
+    Ctl* c = d.ctl;
+    if (c->halt || d.fire->done) {
+        c->kind = 0;
+        return;
+    }
+    int need = pending_rebuild(c, c->epoch, pos);
+    if (need >= 2) {
+        c->halt = 1;
+        c->halt_pos = pos;
+        c->halt_mode = H_FIRE;
+        c->halt_kind = 2;
+        c->kind = 0;
+        return;
+    }
+    c->kind = need;
+    if (need == 1) {
+        c->max_nnum = 0;
+        c->max_vnum = 0;
+        c->n_rebuilds += 1;
+    }
+}
+
+// An in-chunk list rebuild whose rows did not fit: halt with the step unfinished; the
+// host regrows the lists, rebuilds and finishes the step (fix_overflow).
+__global__ void k_bfire_check(Dev d, int pos) {
+    Ctl* c = d.ctl;
+    if (c->kind < 1) return;
+    if (c->max_nnum > d.cap || c->max_vnum > d.vcap) {
+        c->overflow = 1;
+        c->halt = 1;
+        c->halt_pos = pos;
+        c->halt_mode = H_FIRE;
+        c->halt_kind = 1;
+    }
+    c->kind = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bfire_vforce(Dev d, PairParams p, double rv,
+                                                        int gate) {
+    if (bfire_skip(d, gate)) return;
+    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= d.Nv) return;
+    vertex_force_one(d, p, rv, v, 0);
+}
+
+// kick: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
+__global__ __launch_bounds__(BLOCK) void k_bfire_reduce(Dev d, int kick, int gate,
+                                                        int rotation) {
+    if (bfire_skip(d, gate)) return;
+    const FireCtl* F = d.fire;
+    double hdt = F->dt / 2.0;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
+    if (i < d.N) {
+        double ax = 0.0, ay = 0.0, at = 0.0, e = 0.0;
+        int st = d.vstart[i], nv = d.nverts[i];
+        for (int k = 0; k < nv; ++k) {
+            double2 f = d.vfrc[st + k];
+            ax += f.x;
+            ay += f.y;
+            at += d.vtq[st + k];
+            e += d.vpe[st + k];
+        }
+        double fx = 0.0 + ax, fy = 0.0 + ay, t = 0.0 + at;
+        pe = 0.0 + e;
+        d.frc[i] = make_double2(fx, fy);
+        d.torque[i] = t;
+        d.pe[i] = pe;
+        double2 v = d.vel[i];
+        double w = d.omega[i];
+        if (kick) {
+            if (rotation) w += t * hdt / d.moi[i];
+            else w = 0.0;
+            double m = d.P0[i].w;
+            v.x += fx * hdt / m;
+            v.y += fy * hdt / m;
+            d.vel[i] = v;
+            d.omega[i] = w;
+        }
+        pxv = fx * v.x;
+        pyv = fy * v.y;
+        ptw = t * w;
+        f2 = fmax(fx * fx + fy * fy, t * t);
+    }
+    int64_t nc = (d.N + CHUNK - 1) / CHUNK;
+    int64_t nblk = (d.N + BLOCK - 1) / BLOCK;
+    double* P = d.part;
+    double s0 = wave_tree_sum(pxv);
+    double s1 = wave_tree_sum(pyv);
+    double s2 = wave_tree_sum(pe);
+    double s3 = wave_tree_sum(ptw);
+    int64_t ch = i / CHUNK;
+    if ((threadIdx.x % WAVE) == 0 && ch < nc) {
+        P[ch] = s0;
+        P[nc + ch] = s1;
+        P[2 * nc + ch] = s2;
+        P[3 * nc + nblk + ch] = s3;
+    }
+    double b = block_max(f2);
+    if (threadIdx.x == 0) P[3 * nc + blockIdx.x] = b;
+}
+
 // ------------------------------------------------------------------ host side
 int bumpy_alloc(dpm_system* s) {
     Dev& d = s->d;
@@ -389,7 +608,7 @@ int bumpy_alloc(dpm_system* s) {
         dalloc(s, &d.vpart, v) || dalloc(s, &d.vstart, n) || dalloc(s, &d.nverts, n) ||
         dalloc(s, &d.svidx, v) || dalloc(s, &d.vnum, v) || dalloc(s, &d.vfrc, v) ||
         dalloc(s, &d.vtq, v) || dalloc(s, &d.vpe, v) || dalloc(s, &d.lpos, n) ||
-        dalloc(s, &d.dth, n))
+        dalloc(s, &d.dth, n) || dalloc(s, &d.mpos, n) || dalloc(s, &d.dth0, n))
         return -1;
     Spare& sp = s->sp;
     if (dalloc(s, &sp.angle, n) || dalloc(s, &sp.omega, n) || dalloc(s, &sp.torque, n) ||
@@ -487,51 +706,36 @@ int bumpy_nve_run(dpm_system* s, double dt, int64_t n) {
 }
 
 
-// minimizeFire (src/routines/minimization.cu:75-112) driven from the host.
-int bumpy_minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
-    dpm_fire_state st;
-    dpm_fire_state_init(&st, prm->alpha_init, prm->dt);
-    int64_t step = 0;
-    double dof = (double)s->N;
-    if (dpm_zero_velocities(s) || dpm_zero_forces(s) || check_and_rebuild(s, nullptr) ||
-        dpm_compute_forces(s))
-        return -1;
-    double pe;
-    if (sum_to_host(s, S_PE, &pe)) return -1;
-    double last = pe / dof, avg = last;
-    int converged = 0;
-    while (step < prm->max_steps) {
-        if (dpm_fire_step(s, &st)) return -1;
-        if (sum_to_host(s, S_PE, &pe)) return -1;
-        avg = pe / dof;
-        double diff = std::abs(avg / last - 1);
-        last = avg;
-        bool fok = false;
-        if (prm->fmax_target > 0) {
-            double f2;
-            if (dpm_reduce(s, DPM_RED_FMAX_SQ, &f2)) return -1;
-            fok = f2 < prm->fmax_target * prm->fmax_target;
-        }
-        if (diff < prm->avg_pe_diff_target || avg < prm->avg_pe_target || st.stopped || fok) {
-            converged = st.stopped ? 0 : 1;
-            break;
-        }
-        step++;
-    }
-    if (dpm_zero_velocities(s)) return -1;
-    if (out) {
-        out->steps = std::min<int64_t>(step + 1, prm->max_steps);
-        out->stopped = st.stopped;
-        out->converged = converged;
-        out->avg_pe = avg;
-        double f2;
-        if (dpm_reduce(s, DPM_RED_FMAX_SQ, &f2)) return -1;
-        out->fmax_sq = f2;
-        out->dt = st.dt;
-        out->alpha = st.alpha;
-    }
-    return 0;
+// ---- device-resident FIRE (the loop driver is minimize_fire in dpm_core.hip) ----
+void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate) {
+    PairParams p = params_of(s);
+    hipLaunchKernelGGL(k_bfire_vforce, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
+                       gate);
+    hipLaunchKernelGGL(k_bfire_reduce, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d,
+                       kick, gate, (int)s->rotation);
 }
+
+void bumpy_enqueue_fire_step(dpm_system* s, int pos) {
+    const Dev& d = s->d;
+    NbrParams nb = nbr_params(s);
+    hipStream_t st = s->stream;
+    {
+        Timed t(s, DPM_K_DRIFT);
+        hipLaunchKernelGGL(k_bfire_move, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, pos,
+                           (int)s->rotation);
+        hipLaunchKernelGGL(k_bfire_vertices, pgrid(s->Nv), dim3(BLOCK), 0, st, d);
+    }
+    if (s->method == DPM_NBR_CELL || s->method == DPM_NBR_VERLET) {
+        Timed t(s, DPM_K_REBUILD);
+        hipLaunchKernelGGL(k_bfire_decide, dim3(1), dim3(1), 0, st, d, pos);
+        enqueue_list_build(s, 1);
+        bumpy_enqueue_vertex_build(s, 1);
+        hipLaunchKernelGGL(k_bfire_check, dim3(1), dim3(1), 0, st, d, pos);
+    }
+    Timed t(s, DPM_K_FORCE);
+    bumpy_enqueue_fire_forces(s, 1, G_NORMAL);
+}
+
 
 int bumpy_snapshot(dpm_system* s, bool save) {
     auto cp = [&](void* a, void* b, size_t n) {

@@ -29,69 +29,9 @@ namespace dpm {
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
 
-constexpr long long STAMP_STRIDE = 8192;  // > any chunk length
-__device__ __forceinline__ long long stamp(long long epoch, int pos) {
-    return epoch * STAMP_STRIDE + pos + 1;
-}
-
-__device__ __forceinline__ bool resumable(const Dev& d) {
-    const Ctl* c = d.ctl;
-    return c->max_nnum <= d.cap && (d.vnbr == nullptr || c->max_vnum <= d.vcap);
-}
-
 // =========================================================================
 //                                  kernels
 // =========================================================================
-
-__device__ __forceinline__ int64_t particle_of_thread(int& tile) {
-    tile = xcd_tile(blockIdx.x, gridDim.x);
-    return (int64_t)tile * BLOCK + threadIdx.x;
-}
-
-// Per-block maxima of the two displacement arrays (the reference keeps the full
-// arrays, particle.cu:1094-1100; only their maxima are ever read) and the rebuild
-// request of Particle::checkFor{Cell,Verlet}ListUpdate (particle.cu:1141-1161):
-// 3*max > threshold.  wr < 0: no request.
-__device__ __forceinline__ void disp_epilogue(const Dev& d, const NbrParams& nb, int tile,
-                                              double nd, double cd, int wr, long long st) {
-    double bn = block_max(nd);
-    double bc = block_max(cd);
-    if (threadIdx.x == 0) {
-        d.pmax_nd[tile] = bn;
-        d.pmax_cd[tile] = bc;
-        if (wr >= 0 && (nb.method == DPM_NBR_CELL || nb.method == DPM_NBR_VERLET)) {
-            if (nb.method == DPM_NBR_CELL && 3.0 * bc > nb.cthr) d.ctl->cflag[wr] = st;
-            if (3.0 * bn > nb.thr) d.ctl->flag[wr] = st;
-        }
-    }
-}
-
-__device__ __forceinline__ void disp_of(const Dev& d, int64_t i, double x, double y, double& nd,
-                                        double& cd) {
-    double2 ln = d.lref[i], lc = d.cref[i];
-    double dxn = x - ln.x, dyn = y - ln.y;
-    nd = dxn * dxn + dyn * dyn;
-    double dxc = x - lc.x, dyc = y - lc.y;
-    cd = dxc * dxc + dyc * dyc;
-}
-
-// The rebuild a step at (epoch, pos) must perform before its forces:
-// 0 none, 1 list, 2 cells + list (the cell test is the reference's first,
-// particle.cu:1141-1161), 3 the list of the last in-kernel rebuild again, from its
-// stored build positions, with a larger capacity.  ovf holds the stamp of the step
-// whose in-kernel rebuild overflowed; that step itself ignores it, so every block of
-// a launch reaches the same decision.
-__device__ __forceinline__ bool ovf_pending(const Ctl* c, long long st) {
-    return c->ovf != 0 && c->ovf != st;
-}
-
-__device__ __forceinline__ int pending_rebuild(const Ctl* c, long long epoch, int pos) {
-    long long st = stamp(epoch, pos);
-    if (c->cflag[pos & 1] == st) return 2;
-    bool list = c->flag[pos & 1] == st;
-    if (ovf_pending(c, st)) return list ? 1 : 3;
-    return list ? 1 : 0;
-}
 
 __device__ __forceinline__ int wrap_cell(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
@@ -1120,6 +1060,7 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
     double px = wave_final_sum(P, nc);
     double py = wave_final_sum(P + nc, nc);
     double pe = wave_final_sum(P + 2 * nc, nc);
+    double ptw = d.vpos ? wave_final_sum(P + 3 * nc + nblk, nc) : 0.0;
     double fm = 0.0;
     for (int64_t k = threadIdx.x; k < nblk; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
     fm = wave_max(fm);
@@ -1153,8 +1094,10 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
             return;
         }
     }
-    // Fire::step decision for the next iteration (src/integrator/fire.cu:14-33)
+    // Fire::step decision for the next iteration (src/integrator/fire.cu:14-33);
+    // RigidBumpy::getPowerFire adds sum(tau*w) (rigid_bumpy.cu:1181-1191)
     double power = px + py;
+    if (d.vpos) power = power + ptw;
     F->power = power;
     if (power > 0) {
         F->n_good++;
@@ -1383,6 +1326,16 @@ int ensure_ell(dpm_system* s, int64_t cap) {
 
 // The rebuild kernels (cell binning, scan, scatter, stable fix-up, reorder, list
 // build) in stream order; with `gated` each one consults the chain's decision.
+// The particle list build of a rebuild (after the cell chain for the cell method).
+void enqueue_list_build(dpm_system* s, int gated) {
+    NbrParams nb = nbr_params(s);
+    if (s->method == DPM_NBR_CELL)
+        hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nb, gated);
+    else
+        hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nb,
+                           gated);
+}
+
 static void enqueue_rebuild(dpm_system* s, int gated) {
     const Dev& d = s->d;
     NbrParams nb = nbr_params(s);
@@ -1402,14 +1355,12 @@ static void enqueue_rebuild(dpm_system* s, int gated) {
         hipLaunchKernelGGL(k_reorder_gather, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
         if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_reorder(s, gated);
         hipLaunchKernelGGL(k_reorder_copyback, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
-        hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
-    } else {
-        hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
     }
+    enqueue_list_build(s, gated);
     if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_build(s, gated);
 }
 
-// The halted step's remaining work. resume: 0 none, 1 NVE, 2 FIRE.
+// The halted step's remaining work. resume: 0 none, 1 NVE, 2 FIRE, 3 rigid bumpy FIRE.
 static void enqueue_resume(dpm_system* s, int resume, double dt) {
     const Dev& d = s->d;
     NbrParams nb = nbr_params(s);
@@ -1423,6 +1374,9 @@ static void enqueue_resume(dpm_system* s, int resume, double dt) {
     } else if (resume == 2) {
         LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 1,
                            (int)G_RESUME);
+        hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, 0, (int)G_RESUME);
+    } else if (resume == 3) {  // rigid bumpy FIRE
+        bumpy_enqueue_fire_forces(s, 1, G_RESUME);
         hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 0, 0, (int)G_RESUME);
     }
 }
@@ -1566,6 +1520,15 @@ static void fire_chunk(dpm_system* s, double, int64_t c) {
     enqueue_chain(s, 2, 0.0);
 }
 
+static void bumpy_fire_chunk(dpm_system* s, double, int64_t c) {
+    for (int64_t k = 0; k < c; ++k) {
+        bumpy_enqueue_fire_step(s, (int)k);
+        hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, s->stream, s->d, 0, (int)k,
+                           (int)G_NORMAL);
+    }
+    enqueue_chain(s, 3, 0.0);
+}
+
 static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 1, dt); }
 
 // Runs `total` steps in chunks with up to two launch units in flight; each unit ends
@@ -1576,7 +1539,10 @@ static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 
 // halted step the host enqueues the rebuild chain (captured once as a graph) as a
 // recovery unit behind the chunk already in flight, without waiting for the device.
 // A capacity overflow is fixed on the host.  With `stop_on_done` the loop ends when
-// the device reports completion (FIRE).
+// the device reports completion (FIRE).  The chunk length adapts: after a chunk that
+// halted at step p the next one is about 2(p + 1) steps long (a halted chunk's later
+// steps are no-ops), and each chunk that runs through doubles it again, up to the
+// configured length.  Chunk boundaries do not change results.
 static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, ChunkFn chunk_fn,
                       std::map<int64_t, Graph>& cache, bool stop_on_done, bool host_recovery) {
     struct Unit {
@@ -1589,6 +1555,7 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
     int64_t done = 0, planned = 0;
     int next_slot = 0;
     bool need_recovery = false;
+    int64_t cur = s->chunk;
     while (done < total || need_recovery) {
         while (q.size() < 2 && (need_recovery || planned < total)) {
             Unit u{0, 0, false, false};
@@ -1611,8 +1578,8 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
                 planned += 1;
                 need_recovery = false;
             } else {
-                int64_t c = std::min<int64_t>(s->chunk, total - planned);
-                if (s->use_graph && s->timing != 1 && c == s->chunk) {
+                int64_t c = std::min<int64_t>(cur, total - planned);
+                if (s->use_graph && s->timing != 1 && c == cur) {
                     if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
                     DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
                 } else {
@@ -1657,6 +1624,13 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
         }
         if (host_recovery && hc.halt && !f.recovery && !f.covered) need_recovery = true;
         if (stop_on_done && hc.done) break;
+        if (!f.recovery && f.c > 0) {
+            if (hc.last_progress < f.c)
+                cur = std::min<int64_t>(s->chunk, pow2_at_least(2 * (hc.last_progress + 1)));
+            else
+                cur = std::min<int64_t>(s->chunk, 2 * cur);
+            cur = std::max<int64_t>(cur, 2);
+        }
     }
     DPM_HIP(hipStreamSynchronize(s->stream));
     if (s->timing) collect_timing(s);
@@ -1697,7 +1671,7 @@ int nve_run(dpm_system* s, double dt, int64_t n) {
 }
 
 int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
-    if (s->kind == DPM_BUMPY) return bumpy_minimize_fire(s, prm, out);
+    const bool rigid = s->kind == DPM_BUMPY;
     if (!s->nbr_ready) {
         set_error("minimize_fire: neighbour list not initialised");
         return -1;
@@ -1721,12 +1695,17 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
     hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (check_and_rebuild(s, nullptr)) return -1;
     PairParams p = params_of(s);
-    LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 0,
-               (int)G_ALWAYS);
+    if (rigid) {
+        bumpy_enqueue_fire_forces(s, 0, G_ALWAYS);
+    } else {
+        LAUNCH_HOT(k_fire_force, dim3((unsigned)s->nblk), dim3(BLOCK), 0, st, d, p, 0, 0,
+                   (int)G_ALWAYS);
+    }
     hipLaunchKernelGGL(k_fire_ctl, dim3(1), dim3(WAVE), 0, st, d, 1, 0, (int)G_ALWAYS);
     if (check_launch()) return -1;
     if (prm->max_steps > 0 &&
-        run_chunks(s, prm->max_steps, 0.0, 2, fire_chunk, s->fire_graphs, true, false))
+        run_chunks(s, prm->max_steps, 0.0, rigid ? 3 : 2, rigid ? bumpy_fire_chunk : fire_chunk,
+                   s->fire_graphs, true, false))
         return -1;
     hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (scopy(s, &f, d.fire, sizeof(FireCtl), hipMemcpyDeviceToHost)) return -1;
@@ -1738,7 +1717,7 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
         out->dt = f.dt;
         out->alpha = f.alpha;
         double f2 = 0.0;
-        if (max_to_host(s, M_F2, &f2)) return -1;
+        if (dpm_reduce(s, DPM_RED_FMAX_SQ, &f2)) return -1;
         out->fmax_sq = f2;
     }
     return check_launch();

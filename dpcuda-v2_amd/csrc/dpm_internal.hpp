@@ -97,6 +97,8 @@ struct Dev {
     double *vtq, *vpe;
     double2* lpos;              // COM before the last drift (vertex rotation)
     double* dth;                // angle change of the last drift
+    double2* mpos;              // FIRE: COM after the uphill back-step (= lpos otherwise)
+    double* dth0;               // FIRE: angle change of the uphill back-step
 };
 
 // Reorder scratch: one spare copy of every permuted array.

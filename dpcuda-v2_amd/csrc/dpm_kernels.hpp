@@ -42,6 +42,68 @@ enum MaxWhat { M_ND = 0, M_CD, M_F2, M_T2 };
 // to finish a halted step), G_ALWAYS unconditionally.
 enum Gate { G_NORMAL = 0, G_RESUME = 1, G_ALWAYS = 2 };
 
+// ---- control-block helpers shared by the loop kernels ----
+constexpr long long STAMP_STRIDE = 8192;  // > any chunk length
+__device__ __forceinline__ long long stamp(long long epoch, int pos) {
+    return epoch * STAMP_STRIDE + pos + 1;
+}
+
+__device__ __forceinline__ bool resumable(const Dev& d) {
+    const Ctl* c = d.ctl;
+    return c->max_nnum <= d.cap && (d.vnbr == nullptr || c->max_vnum <= d.vcap);
+}
+
+
+__device__ __forceinline__ int64_t particle_of_thread(int& tile) {
+    tile = xcd_tile(blockIdx.x, gridDim.x);
+    return (int64_t)tile * BLOCK + threadIdx.x;
+}
+
+// Per-block maxima of the two displacement arrays (the reference keeps the full
+// arrays, particle.cu:1094-1100; only their maxima are ever read) and the rebuild
+// request of Particle::checkFor{Cell,Verlet}ListUpdate (particle.cu:1141-1161):
+// 3*max > threshold.  wr < 0: no request.
+__device__ __forceinline__ void disp_epilogue(const Dev& d, const NbrParams& nb, int tile,
+                                              double nd, double cd, int wr, long long st) {
+    double bn = block_max(nd);
+    double bc = block_max(cd);
+    if (threadIdx.x == 0) {
+        d.pmax_nd[tile] = bn;
+        d.pmax_cd[tile] = bc;
+        if (wr >= 0 && (nb.method == DPM_NBR_CELL || nb.method == DPM_NBR_VERLET)) {
+            if (nb.method == DPM_NBR_CELL && 3.0 * bc > nb.cthr) d.ctl->cflag[wr] = st;
+            if (3.0 * bn > nb.thr) d.ctl->flag[wr] = st;
+        }
+    }
+}
+
+__device__ __forceinline__ void disp_of(const Dev& d, int64_t i, double x, double y, double& nd,
+                                        double& cd) {
+    double2 ln = d.lref[i], lc = d.cref[i];
+    double dxn = x - ln.x, dyn = y - ln.y;
+    nd = dxn * dxn + dyn * dyn;
+    double dxc = x - lc.x, dyc = y - lc.y;
+    cd = dxc * dxc + dyc * dyc;
+}
+
+// The rebuild a step at (epoch, pos) must perform before its forces:
+// 0 none, 1 list, 2 cells + list (the cell test is the reference's first,
+// particle.cu:1141-1161), 3 the list of the last in-kernel rebuild again, from its
+// stored build positions, with a larger capacity.  ovf holds the stamp of the step
+// whose in-kernel rebuild overflowed; that step itself ignores it, so every block of
+// a launch reaches the same decision.
+__device__ __forceinline__ bool ovf_pending(const Ctl* c, long long st) {
+    return c->ovf != 0 && c->ovf != st;
+}
+
+__device__ __forceinline__ int pending_rebuild(const Ctl* c, long long epoch, int pos) {
+    long long st = stamp(epoch, pos);
+    if (c->cflag[pos & 1] == st) return 2;
+    bool list = c->flag[pos & 1] == st;
+    if (ovf_pending(c, st)) return list ? 1 : 3;
+    return list ? 1 : 0;
+}
+
 // ---- host helpers (dpm_core.hip) ----
 int dalloc_bytes(dpm_system* s, void** p, size_t bytes);
 template <class T>
@@ -64,6 +126,7 @@ int ensure_ell(dpm_system* s, int64_t cap);
 int device_rebuild(dpm_system* s, int mode, int* kind_out);
 int check_and_rebuild(dpm_system* s, int* kind_out);
 int refresh_rmax(dpm_system* s);
+void enqueue_list_build(dpm_system* s, int gated);
 void reset_ctl(dpm_system* s, int config);
 void collect_timing(dpm_system* s);
 
@@ -88,7 +151,8 @@ int bumpy_mix(dpm_system* s, double alpha);
 int bumpy_stress(dpm_system* s);
 int bumpy_contacts(dpm_system* s);
 int bumpy_nve_run(dpm_system* s, double dt, int64_t n);
-int bumpy_minimize_fire(dpm_system* s, const dpm_fire_params* p, dpm_fire_result* out);
+void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate);
+void bumpy_enqueue_fire_step(dpm_system* s, int pos);
 int bumpy_snapshot(dpm_system* s, bool save);
 int bumpy_download(dpm_system* s, int field, void* data, int64_t count,
                    const std::vector<int>& sidx);

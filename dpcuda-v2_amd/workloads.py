@@ -135,6 +135,12 @@ def bumpy_config(n: int, phi: float, seed: int = 12345, n_small: int = 20,
                       vertex_radius(1.0, n_small, 1.0))
 
 
+# RigidBumpy::initializeFromConfig scales the configured contact energy by the square
+# of the geometry scale 2 r_v / d_min (rigid_bumpy.cu:237-241, :1265-1269)
+def bumpy_contact_scale(st: BumpyState, e_c: float = 1.0) -> float:
+    return e_c * (2.0 * st.rv / (2.0 * float(st.rad.min()))) ** 2
+
+
 def make_bumpy_system(sys_cls, st: BumpyState, backend=None, e_c: float = 1.0,
                       rotation: bool = True, method: int = 3):
     s = sys_cls(1, st.n, st.nv, backend=backend)
