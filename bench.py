@@ -103,6 +103,7 @@ def rigid_fire(args, P, W):
     n = args.n if args.n != 262144 else RIGID_N
     print(f"[bench] rigid bumpy FIRE N={n}", file=sys.stderr, flush=True)
     st, s = prepare_rigid(P, W, n, args.seed)
+    s.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
     s.set_option(P.OPT_CHUNK, args.chunk)
     prm = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=1e-12,
                         max_steps=args.fire_max_steps)
@@ -131,12 +132,13 @@ def rigid_fire(args, P, W):
     kv = len(s.vertex_pairs()[0]) / nv
     force_us = f_ms / max(f_n, 1) * 1e3
     move_us = m_ms / max(m_n, 1) * 1e3
-    # algorithmic bytes of the force phase per step: per vertex 16 (position) + 4 (owner)
-    # + 4 (row length) + 32 (force, torque, energy written) + 4 k_v (row), and again 32
-    # read back by the per-particle sums; per particle 8 (vertex range) + 32 (mass) +
-    # 32 (force, torque, energy written) + 24 + 24 (velocity, spin read and written) + 8
-    # (moment of inertia)
-    force_bytes = nv * (56.0 + 4.0 * kv + 32.0) + n * (8 + 32 + 32 + 48 + 8)
+    # algorithmic bytes of the force phase of one step (forces, torques, energies, their
+    # per-particle sums, the closing kick and the reduction partials): per vertex 16
+    # (position) + 4 (owner) + 4 (row length) + 4 k_v (row; the neighbours' positions
+    # are the same array, not counted again); per particle 32 (position record: lever
+    # arm and mass) + 8 (vertex range) + 32 (force, torque, energy written) + 48
+    # (velocity and spin read and written) + 8 (moment of inertia)
+    force_bytes = nv * (24.0 + 4.0 * kv) + n * (32 + 8 + 32 + 48 + 8)
     force_gbs = force_bytes / (force_us * 1e-6) / 1e9 if force_us > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
@@ -185,7 +187,7 @@ def rigid_fire(args, P, W):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_bfire_vforce + k_bfire_reduce (force phase of one FIRE step)",
+            "kernel": "k_bfire_forces + k_bfire_partials (force phase of one FIRE step)",
             "achieved": force_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -205,31 +205,38 @@ __device__ __forceinline__ bool within_v(double ax, double ay, double bx, double
 
 // kernelUpdateVertexNeighborList, rigid_bumpy/kernels.cu:870-922
 // gated: run only when the rebuild chain chose a list rebuild
-__global__ void k_bumpy_vertex_build(Dev d, NbrParams nb, int gated) {
+__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams nb, int gated) {
     if (gated && d.ctl->kind < 1) return;
-    int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (v >= d.Nv) return;
-    double2 xv = d.vpos[v];
-    int p = d.vpart[v];
-    int nn = min(d.nnum[p], d.cap);
-    int cnt = 0;
-    for (int k = 0; k < nn; ++k) {
-        int q = d.nbr[(int64_t)k * d.N + p];
-        double4 cq = d.P0[q];
-        if (within_v(xv.x, xv.y, cq.x, cq.y, nb, nb.vp2)) {
-            int f0 = d.vstart[q], nv = d.nverts[q];
-            for (int u = f0; u < f0 + nv; ++u) {
-                double2 xu = d.vpos[u];
-                if (within_v(xv.x, xv.y, xu.x, xu.y, nb, nb.vc2)) {
-                    if (cnt < d.vcap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
-                    cnt++;
+    int mx = 0;
+    for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < d.Nv;
+         v += (int64_t)gridDim.x * BLOCK) {
+        double2 xv = d.vpos[v];
+        int p = d.vpart[v];
+        int nn = min(d.nnum[p], d.cap);
+        int cnt = 0;
+        for (int k = 0; k < nn; ++k) {
+            int q = d.nbr[(int64_t)k * d.N + p];
+            double4 cq = d.P0[q];
+            if (within_v(xv.x, xv.y, cq.x, cq.y, nb, nb.vp2)) {
+                int f0 = d.vstart[q], nv = d.nverts[q];
+                for (int u = f0; u < f0 + nv; ++u) {
+                    double2 xu = d.vpos[u];
+                    if (within_v(xv.x, xv.y, xu.x, xu.y, nb, nb.vc2)) {
+                        if (cnt < d.vcap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
+                        cnt++;
+                    }
                 }
             }
         }
+        d.vnum[v] = cnt;
+        mx = max(mx, cnt);
     }
-    d.vnum[v] = cnt;
-    atomicMax(&d.ctl->max_vnum, cnt);
-    atomicMax(&d.ctl->max_vnum_all, cnt);
+    // row-length statistics: one atomic per block
+    int m = block_max_int(mx);
+    if (threadIdx.x == 0) {
+        atomicMax(&d.ctl->max_vnum, m);
+        atomicMax(&d.ctl->max_vnum_all, m);
+    }
 }
 
 // exclusive scan of nverts (new order) into vstart for the vertex reorder
@@ -405,9 +412,10 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
 //                     chunk, a cell rebuild halts the loop until the chunk's end
 //   k_build_cell / k_build_verlet, k_bumpy_vertex_build   (gated on that decision)
 //   k_bfire_check     an in-chunk rebuild that overflowed the capacity halts the loop
-//   k_bfire_vforce    vertices: forces, torques, energies (kernelCalcRigidBumpyForces1)
-//   k_bfire_reduce    particles: vertex sums (kernelCalcRigidBumpyParticleForces1),
-//                     kick(dt/2), chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2)
+//   k_bfire_forces    vertex forces, torques, energies (kernelCalcRigidBumpyForces1) and
+//                     their per-particle sums (kernelCalcRigidBumpyParticleForces1) +
+//                     kick(dt/2), one pass
+//   k_bfire_partials  chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2)
 //   k_fire_ctl        stop test of the step and Fire::step's decision for the next
 __device__ __forceinline__ bool bfire_skip(const Dev& d, int gate) {
     const Ctl* c = d.ctl;
@@ -542,6 +550,130 @@ __global__ __launch_bounds__(BLOCK) void k_bfire_vforce(Dev d, PairParams p, dou
     vertex_force_one(d, p, rv, v, 0);
 }
 
+// The force phase of a FIRE step in one pass over the vertices.  Block b owns the
+// particles whose first vertex lies in [256 b, 256 b + 256) and all of their vertices
+// (at most 256 + nv_max - 1): lanes compute vertex forces, torques and energies into
+// LDS, then one lane per particle sums its vertices in vertex order (the order of
+// kernelCalcRigidBumpyParticleForces1), applies the closing kick and stores the
+// particle's force, torque and energy.  The per-vertex arrays are not written.
+constexpr int BF_TILE = 256;
+constexpr int BF_LDS_SMALL = 320;  // vertices per block (BF_TILE + nv_max - 1 must fit):
+constexpr int BF_LDS = 1024;       // the small form keeps more blocks resident per CU
+
+__device__ __forceinline__ int first_particle_from(const Dev& d, int64_t v) {
+    if (v >= d.Nv) return (int)d.N;
+    int p = d.vpart[v];
+    return d.vstart[p] == v ? p : p + 1;
+}
+
+template <int LDSV, bool HOT>
+__global__ __launch_bounds__(BF_TILE) void k_bfire_forces(Dev d, PairParams p, double rv,
+                                                          int kick, int gate, int rotation) {
+    if (bfire_skip(d, gate)) return;
+    __shared__ double sv[4][LDSV];       // per vertex: fx, fy, torque, energy
+    __shared__ double sres[4][BF_TILE];  // per particle: the sums of those
+    const int64_t b0 = (int64_t)blockIdx.x * BF_TILE;
+    const int p0 = first_particle_from(d, b0);
+    const int p1 = first_particle_from(d, b0 + BF_TILE);
+    if (p0 >= p1) return;
+    const int np = p1 - p0;
+    const int64_t lo = d.vstart[p0];
+    const int64_t hi = p1 < d.N ? (int64_t)d.vstart[p1] : d.Nv;
+    for (int64_t v = lo + threadIdx.x; v < hi; v += BF_TILE) {
+        const double4 cq = d.P0[d.vpart[v]];
+        const int nn = min(d.vnum[v], d.vcap);
+        const double2 xv = d.vpos[v];
+        double ax = 0.0, ay = 0.0, e = 0.0;
+        for (int k = 0; k < nn; ++k) {
+            int u = d.vnbr[(int64_t)k * d.Nv + v];
+            double2 xu = d.vpos[u];
+            double fx, fy, ie;
+            if constexpr (HOT)
+                ie = pair_hot<true>(make_double4(xv.x, xv.y, rv, 0.0),
+                                    make_double4(xu.x, xu.y, rv, 0.0), true, p, fx, fy);
+            else
+                ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+            ax += fx;
+            ay += fy;
+            e += ie;
+        }
+        // calcTorque, include/particles/base/kernels.cuh:237-241
+        double dx = xv.x - cq.x, dy = xv.y - cq.y;
+        double t = ay * dx - ax * dy;
+        int64_t l = v - lo;
+        sv[0][l] = 0.0 + ax;
+        sv[1][l] = 0.0 + ay;
+        sv[2][l] = 0.0 + t;
+        sv[3][l] = 0.0 + e;
+    }
+    __syncthreads();
+    // four lanes per particle, one per quantity, each summing in vertex order
+    for (int j = threadIdx.x; j < 4 * np; j += BF_TILE) {
+        int q = j >> 2, comp = j & 3;
+        int i = p0 + q;
+        int64_t st = d.vstart[i] - lo;
+        int nv = d.nverts[i];
+        const double* src = sv[comp] + st;
+        double acc = 0.0;
+        for (int k = 0; k < nv; ++k) acc += src[k];
+        sres[comp][q] = 0.0 + acc;
+    }
+    __syncthreads();
+    const double hdt = d.fire->dt / 2.0;
+    for (int q = threadIdx.x; q < np; q += BF_TILE) {
+        int i = p0 + q;
+        double fx = sres[0][q], fy = sres[1][q], t = sres[2][q];
+        d.frc[i] = make_double2(fx, fy);
+        d.torque[i] = t;
+        d.pe[i] = sres[3][q];
+        if (kick) {
+            double2 v = d.vel[i];
+            double w = d.omega[i];
+            if (rotation) w += t * hdt / d.moi[i];
+            else w = 0.0;
+            double m = d.P0[i].w;
+            v.x += fx * hdt / m;
+            v.y += fy * hdt / m;
+            d.vel[i] = v;
+            d.omega[i] = w;
+        }
+    }
+}
+
+// Chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2) over the particles, in the
+// canonical order (64-particle chunks) that k_fire_ctl finishes.
+__global__ __launch_bounds__(BLOCK) void k_bfire_partials(Dev d, int gate) {
+    if (bfire_skip(d, gate)) return;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
+    if (i < d.N) {
+        double2 f = d.frc[i], v = d.vel[i];
+        double t = d.torque[i], w = d.omega[i];
+        pe = d.pe[i];
+        pxv = f.x * v.x;
+        pyv = f.y * v.y;
+        ptw = t * w;
+        f2 = fmax(f.x * f.x + f.y * f.y, t * t);
+    }
+    int64_t nc = (d.N + CHUNK - 1) / CHUNK;
+    int64_t nblk = (d.N + BLOCK - 1) / BLOCK;
+    double* P = d.part;
+    double s0 = wave_tree_sum(pxv);
+    double s1 = wave_tree_sum(pyv);
+    double s2 = wave_tree_sum(pe);
+    double s3 = wave_tree_sum(ptw);
+    int64_t ch = i / CHUNK;
+    if ((threadIdx.x % WAVE) == 0 && ch < nc) {
+        P[ch] = s0;
+        P[nc + ch] = s1;
+        P[2 * nc + ch] = s2;
+        P[3 * nc + nblk + ch] = s3;
+    }
+    double b = block_max(f2);
+    if (threadIdx.x == 0) P[3 * nc + blockIdx.x] = b;
+}
+
+// Two-pass form for particles with more vertices than the fused kernel's LDS holds.
 // kick: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
 __global__ __launch_bounds__(BLOCK) void k_bfire_reduce(Dev d, int kick, int gate,
                                                         int rotation) {
@@ -636,8 +768,10 @@ int bumpy_ensure_vcap(dpm_system* s, int64_t cap) {
 // RigidBumpy::updateVertexVerletList (rigid_bumpy.cu:1198-1212), enqueued after the
 // particle list build; overflow is detected by the chain's end kernel.
 void bumpy_enqueue_vertex_build(dpm_system* s, int gated) {
-    hipLaunchKernelGGL(k_bumpy_vertex_build, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d,
-                       nbr_params(s), gated);
+    // grid-stride: a gated launch that skips stays cheap
+    dim3 g(std::min<unsigned>(pgrid(s->Nv).x, 2048u));
+    hipLaunchKernelGGL(k_bumpy_vertex_build, g, dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
+                       gated);
 }
 
 // Vertex half of the cell reorder: new vertex starts from the permuted counts, then
@@ -709,6 +843,18 @@ int bumpy_nve_run(dpm_system* s, double dt, int64_t n) {
 // ---- device-resident FIRE (the loop driver is minimize_fire in dpm_core.hip) ----
 void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate) {
     PairParams p = params_of(s);
+    if (BF_TILE + s->nv_max - 1 <= BF_LDS) {
+        unsigned nb = (unsigned)((s->Nv + BF_TILE - 1) / BF_TILE);
+        bool small = BF_TILE + s->nv_max - 1 <= BF_LDS_SMALL, hot = p.fast && p.fastdiv;
+        auto k = small ? (hot ? k_bfire_forces<BF_LDS_SMALL, true>
+                              : k_bfire_forces<BF_LDS_SMALL, false>)
+                       : (hot ? k_bfire_forces<BF_LDS, true> : k_bfire_forces<BF_LDS, false>);
+        hipLaunchKernelGGL(k, dim3(nb), dim3(BF_TILE), 0, s->stream, s->d, p, s->rv, kick, gate,
+                           (int)s->rotation);
+        hipLaunchKernelGGL(k_bfire_partials, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream,
+                           s->d, gate);
+        return;
+    }
     hipLaunchKernelGGL(k_bfire_vforce, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
                        gate);
     hipLaunchKernelGGL(k_bfire_reduce, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d,
@@ -910,6 +1056,7 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) 
         set_error("dpm_bumpy_setup: vertex counts do not sum to n_vertices");
         return -1;
     }
+    s->nv_max = *std::max_element(nv.begin(), nv.end());
     if (scopy(s, s->d.nverts, nv.data(), sizeof(int) * s->N, hipMemcpyHostToDevice)) return -1;
     if (scopy(s, s->d.vstart, st.data(), sizeof(int) * s->N, hipMemcpyHostToDevice)) return -1;
     std::vector<int> id(s->Nv);

@@ -1057,12 +1057,22 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
     int64_t nc = (d.N + CHUNK - 1) / CHUNK;
     int64_t nblk = (d.N + BLOCK - 1) / BLOCK;
     const double* P = d.part;
-    double px = wave_final_sum(P, nc);
-    double py = wave_final_sum(P + nc, nc);
-    double pe = wave_final_sum(P + 2 * nc, nc);
-    double ptw = d.vpos ? wave_final_sum(P + 3 * nc + nblk, nc) : 0.0;
+    double px, py, pe, ptw;
+    // (the fourth array is the rigid spin power; a disk system sums px's array again)
+    wave_final_sum4(P, P + nc, P + 2 * nc, d.vpos ? P + 3 * nc + nblk : P, nc, px, py, pe, ptw);
+    if (!d.vpos) ptw = 0.0;
     double fm = 0.0;
-    for (int64_t k = threadIdx.x; k < nblk; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
+    {
+        int64_t k = threadIdx.x;
+        for (; k + (FS_ROWS - 1) * WAVE < nblk; k += FS_ROWS * WAVE) {
+            double v[FS_ROWS];
+#pragma unroll
+            for (int u = 0; u < FS_ROWS; ++u) v[u] = P[3 * nc + k + u * WAVE];
+#pragma unroll
+            for (int u = 0; u < FS_ROWS; ++u) fm = fmax(fm, v[u]);
+        }
+        for (; k < nblk; k += WAVE) fm = fmax(fm, P[3 * nc + k]);
+    }
     fm = wave_max(fm);
     if (threadIdx.x != 0) return;
     if (!init) {

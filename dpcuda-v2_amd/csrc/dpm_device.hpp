@@ -218,11 +218,62 @@ __device__ __forceinline__ int block_max_int(int v) {
 // Final stage of the canonical sum over nc chunk partials, done by one wave:
 // lane t sums P[t], P[t+64], ... in order starting from 0.0, then the lanes are
 // tree-reduced.  Result valid in lane 0.
+// The loads of eight rows are issued before their additions (the partials were just
+// written by other XCDs, so each load is a long-latency trip); the additions keep
+// the order.
+constexpr int FS_ROWS = 8;
+
 __device__ __forceinline__ double wave_final_sum(const double* P, int64_t nc) {
     int t = threadIdx.x % WAVE;
     double s = 0.0;
-    for (int64_t k = t; k < nc; k += WAVE) s = s + P[k];
+    int64_t k = t;
+    for (; k + (FS_ROWS - 1) * WAVE < nc; k += FS_ROWS * WAVE) {
+        double v[FS_ROWS];
+#pragma unroll
+        for (int u = 0; u < FS_ROWS; ++u) v[u] = P[k + u * WAVE];
+#pragma unroll
+        for (int u = 0; u < FS_ROWS; ++u) s = s + v[u];
+    }
+    for (; k < nc; k += WAVE) s = s + P[k];
     return wave_tree_sum(s);
+}
+
+// wave_final_sum of four arrays at once (all loads of a row batch in flight together).
+__device__ __forceinline__ void wave_final_sum4(const double* A, const double* B,
+                                                const double* C, const double* D, int64_t nc,
+                                                double& ra, double& rb, double& rc,
+                                                double& rd) {
+    int t = threadIdx.x % WAVE;
+    double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+    int64_t k = t;
+    constexpr int R = FS_ROWS / 2;
+    for (; k + (R - 1) * WAVE < nc; k += R * WAVE) {
+        double va[R], vb[R], vc[R], vd[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            va[u] = A[k + u * WAVE];
+            vb[u] = B[k + u * WAVE];
+            vc[u] = C[k + u * WAVE];
+            vd[u] = D[k + u * WAVE];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            sa = sa + va[u];
+            sb = sb + vb[u];
+            sc = sc + vc[u];
+            sd = sd + vd[u];
+        }
+    }
+    for (; k < nc; k += WAVE) {
+        sa = sa + A[k];
+        sb = sb + B[k];
+        sc = sc + C[k];
+        sd = sd + D[k];
+    }
+    ra = wave_tree_sum(sa);
+    rb = wave_tree_sum(sb);
+    rc = wave_tree_sum(sc);
+    rd = wave_tree_sum(sd);
 }
 
 // XCD-aware tile index.  Workgroups are dispatched round-robin over the 8

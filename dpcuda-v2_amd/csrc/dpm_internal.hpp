@@ -148,6 +148,7 @@ struct dpm_system {
     int64_t ncd = -1, ncells = -1;
     int64_t phys_cap = 0;
     int64_t vphys_cap = 0;
+    int64_t nv_max = 0;         // most vertices of any particle
     int64_t ncells_alloc = 0;
     bool nbr_ready = false;
     // rigid bumpy
