@@ -128,6 +128,50 @@ __device__ __forceinline__ void row_forces(const Dev& d, const double4* P, const
     }
 }
 
+// row_forces for the fused NVE step, with the first ROW_PF index columns loaded by
+// the caller ahead of everything that waits on the control block, batches of ROW_PF
+// records gathered together, and the wave's slot bound taken from ballots instead of
+// a shuffle reduction.  Same slots, same order, same arithmetic as row_forces.
+constexpr int ROW_PF = 4;
+
+__device__ __forceinline__ bool wave_any(bool x) {
+    return __builtin_amdgcn_ballot_w64(x) != 0;
+}
+
+template <bool HOT>
+__device__ __forceinline__ void row_forces_pf(const Dev& d, const double4* P, const PairParams& p,
+                                              int64_t il, double4 qi, int nn, int (&jn)[ROW_PF],
+                                              double& ax, double& ay) {
+    const int cap = d.cap;
+    const int* col = d.nbr + il;
+    const int64_t N = d.N;
+    ax = 0.0;
+    ay = 0.0;
+    for (int k = 0; wave_any(k < nn); k += ROW_PF) {
+        double4 q[ROW_PF];
+#pragma unroll
+        for (int t = 0; t < ROW_PF; ++t) q[t] = P[jn[t]];  // every slot holds a valid index
+        if (wave_any(k + ROW_PF < nn)) {  // next batch's indices
+#pragma unroll
+            for (int t = 0; t < ROW_PF; ++t) jn[t] = col[(int64_t)min(k + ROW_PF + t, cap - 1) * N];
+        }
+#pragma unroll
+        for (int t = 0; t < ROW_PF; ++t) {
+            if (wave_any(k + t < nn)) {
+                const bool valid = k + t < nn;
+                double fx = 0.0, fy = 0.0;
+                if (HOT) {
+                    pair_hot<false>(qi, q[t], valid, p, fx, fy);
+                } else if (valid) {
+                    pair_law(qi.x, qi.y, qi.z, q[t].x, q[t].y, q[t].z, p, fx, fy);
+                }
+                ax += fx;
+                ay += fy;
+            }
+        }
+    }
+}
+
 // ---- NVE (src/integrator/nve.cu:15-25) ---------------------------------------
 // One launch per step: zero + forces + closing kick(dt/2) of step s, then the
 // opening kick(dt/2) + drift(dt) of step s+1 (kernelCalcDiskForces,
@@ -140,12 +184,37 @@ template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrParams nb,
                                                      double hdt, double dt, int cur, int pos,
                                                      int last) {
+    // the control-block words this step reads, all at once
     Ctl* c = d.ctl;
-    if (c->halt) return;
-    long long e = c->epoch;
-    int need = pending_rebuild(c, e, pos);
+    const int halted = c->halt;
+    const long long e = c->epoch;
+    const long long cfl = c->cflag[pos & 1], lfl = c->flag[pos & 1], ovf = c->ovf;
+    const double4* Pc = cur ? d.P1 : d.P0;
+    double4* Pn = cur ? d.P0 : d.P1;
+    int tile;
+    const int64_t i = particle_of_thread(tile);
+    const bool live = i < d.N;
+    const int64_t il = live ? i : 0;
+    // loads that do not depend on the control block go first: their latency overlaps
+    // the control-block read and each other
+    const double4 qi = Pc[il];
+    int nraw = d.nnum[il];
+    const double2 v0 = d.vel[il];
+    const double2 lc = d.cref[il];
+    double2 ln = d.lref[il];
+    int jn[ROW_PF];
+#pragma unroll
+    for (int t = 0; t < ROW_PF; ++t) jn[t] = d.nbr[(int64_t)min(t, d.cap - 1) * d.N + il];
+    // compiler-only fence: every load above is issued before the branch below (none is
+    // waited for here)
+    __asm__ __volatile__("" ::: "memory");
+    if (halted) return;
+    // pending_rebuild and ovf_pending on the words read above
+    const long long st0 = stamp(e, pos);
+    const bool ovf_other = ovf != 0 && ovf != st0;
+    int need = cfl == st0 ? 2 : (lfl == st0 ? 1 : (ovf_other ? 3 : 0));
     // list rebuilds of a cell list are done here, row by row; the rest halts the chunk
-    bool in_kernel = need == 1 && nb.method == DPM_NBR_CELL && !ovf_pending(c, stamp(e, pos));
+    bool in_kernel = need == 1 && nb.method == DPM_NBR_CELL && !ovf_other;
     if (need != 0 && !in_kernel) {
         if (threadIdx.x == 0) {
             c->halt = 1;
@@ -155,35 +224,29 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
         }
         return;
     }
-    const double4* Pc = cur ? d.P1 : d.P0;
-    double4* Pn = cur ? d.P0 : d.P1;
-    int tile;
-    const int64_t i = particle_of_thread(tile);
-    const bool live = i < d.N;
-    const int64_t il = live ? i : 0;
-    const double4 qi = Pc[il];
-    double2 ln = make_double2(qi.x, qi.y);
     int cnt = 0;
     if (in_kernel) {
         if (live) {  // rebuild this particle's row, then sum it like any other row
+            ln = make_double2(qi.x, qi.y);
             cnt = stencil_build(d, nb, i, ln, [&](int j) {
                 double4 q = Pc[j];
                 return make_double2(q.x, q.y);
             });
             d.lref[i] = ln;
             if (cnt > d.cap) c->ovf = stamp(e, pos);
+            nraw = cnt;
+#pragma unroll
+            for (int t = 0; t < ROW_PF; ++t) jn[t] = d.nbr[(int64_t)min(t, d.cap - 1) * d.N + il];
         }
-    } else {
-        ln = d.lref[il];
     }
-    const int nn = live ? min(d.nnum[il], d.cap) : 0;
-    double ax, ay, en;
-    row_forces<false, HOT>(d, Pc, p, il, qi, nn, ax, ay, en);
+    const int nn = live ? min(nraw, d.cap) : 0;
+    double ax, ay;
+    row_forces_pf<HOT>(d, Pc, p, il, qi, nn, jn, ax, ay);
     double nd = 0.0, cd = 0.0;
     if (live) {
         double fx = 0.0 + ax, fy = 0.0 + ay;
         double dtm = hdt / qi.w;
-        double2 v = d.vel[i];
+        double2 v = v0;
         v.x += fx * dtm;  // closing half-kick of step s
         v.y += fy * dtm;
         v.x += fx * dtm;  // opening half-kick of step s+1
@@ -193,7 +256,6 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
         Pn[i] = make_double4(x, y, qi.z, qi.w);
         double dxn = x - ln.x, dyn = y - ln.y;
         nd = dxn * dxn + dyn * dyn;
-        double2 lc = d.cref[i];
         double dxc = x - lc.x, dyc = y - lc.y;
         cd = dxc * dxc + dyc * dyc;
     }

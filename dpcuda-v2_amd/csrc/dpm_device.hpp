@@ -183,6 +183,31 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
+// Block-wide maxima of two values at once (the two shuffle chains interleave);
+// results valid in thread 0.
+__device__ __forceinline__ void block_max2(double& a, double& b) {
+    __shared__ double sm2[2][BLOCK / WAVE];
+#pragma unroll
+    for (int s = WAVE / 2; s > 0; s >>= 1) {
+        double ta = __shfl_down(a, s, WAVE), tb = __shfl_down(b, s, WAVE);
+        a = fmax(a, ta);
+        b = fmax(b, tb);
+    }
+    int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    if (l == 0) {
+        sm2[0][w] = a;
+        sm2[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x / WAVE); ++k) {
+            a = fmax(a, sm2[0][k]);
+            b = fmax(b, sm2[1][k]);
+        }
+    }
+    __syncthreads();
+}
+
 // Block-wide max over a 256-thread block; result valid in thread 0.
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double sm[BLOCK / WAVE];

@@ -65,8 +65,8 @@ __device__ __forceinline__ int64_t particle_of_thread(int& tile) {
 // 3*max > threshold.  wr < 0: no request.
 __device__ __forceinline__ void disp_epilogue(const Dev& d, const NbrParams& nb, int tile,
                                               double nd, double cd, int wr, long long st) {
-    double bn = block_max(nd);
-    double bc = block_max(cd);
+    double bn = nd, bc = cd;
+    block_max2(bn, bc);
     if (threadIdx.x == 0) {
         d.pmax_nd[tile] = bn;
         d.pmax_cd[tile] = bc;
