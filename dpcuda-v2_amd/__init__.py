@@ -142,6 +142,13 @@ _GPU_ONLY = {
     "kernel_time": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_L)]),
     "reset_kernel_time": (_I, [_P]),
     "synchronize": (_I, [_P]),
+    "batch_create": (_P, [_L, _P]),
+    "batch_destroy": (None, [_P]),
+    "batch_set_system": (_I, [_P, _L, _P, _P, _P, _P, _D]),
+    "batch_get_system": (_I, [_P, _L, _P, _P, C.POINTER(_D)]),
+    "batch_config": (_I, [_P, _D, _D, _D, _D]),
+    "batch_set_option": (_I, [_P, _I, _L]),
+    "batch_jam": (_I, [_P, C.POINTER(JamParams), _L, _P, C.POINTER(_L)]),
 }
 
 
@@ -404,3 +411,60 @@ class System:
 
     def synchronize(self):
         self._c("synchronize")
+
+
+class Batch:
+    """Many independent disk systems jammed together on one GPU (dpm_batch_*)."""
+
+    def __init__(self, sizes):
+        self.be = load()
+        self.sizes = [int(n) for n in sizes]
+        a = np.ascontiguousarray(self.sizes, dtype=np.int64)
+        self.h = self.be.fn["batch_create"](len(self.sizes), _ptr(a))
+        if not self.h:
+            raise DpmError(f"dpm_batch_create: {self.be.err()}")
+
+    def close(self):
+        if self.h:
+            self.be.fn["batch_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, name, *args):
+        r = self.be.fn[name](self.h, *args)
+        if r < 0:
+            raise DpmError(f"dpm_{name}: {self.be.err()}")
+        return r
+
+    def set_system(self, k, x, y, rad, mass, side):
+        if not 0 <= k < len(self.sizes):
+            raise DpmError(f"set_system: no system {k}")
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, rad, mass)]
+        if any(len(v) != self.sizes[k] for v in arrs):
+            raise DpmError("set_system: array lengths differ from the system size")
+        self._c("batch_set_system", int(k), *[_ptr(v) for v in arrs], float(side))
+
+    def get_system(self, k):
+        x = np.empty(self.sizes[k])
+        y = np.empty(self.sizes[k])
+        side = C.c_double()
+        self._c("batch_get_system", int(k), _ptr(x), _ptr(y), C.byref(side))
+        return x, y, side.value
+
+    def config(self, e_c=1.0, n_c=2.0, cutoff=1.7, disp_thresh_sq=3 * 0.15 ** 2):
+        self._c("batch_config", float(e_c), float(n_c), float(cutoff), float(disp_thresh_sq))
+
+    def set_option(self, opt, value):
+        self._c("batch_set_option", int(opt), int(value))
+
+    def jam(self, params: JamParams, max_batched_steps=10**9):
+        out = (JamResult * len(self.sizes))()
+        steps = C.c_int64()
+        self._c("batch_jam", C.byref(params), int(max_batched_steps), C.cast(out, C.c_void_p),
+                C.byref(steps))
+        return list(out), steps.value

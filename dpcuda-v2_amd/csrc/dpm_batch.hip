@@ -1,0 +1,932 @@
+// dpm_batch.hip -- batched jamming engine: many independent disk systems of uneven N
+// in one device arena, all advanced by the same per-phase kernels, which a hipGraph
+// replays; each system carries its own FIRE state, jamming controller and mode, so a
+// system that has finished drops out (its waves exit at once) while the others go on.
+//
+// Per system the arithmetic is that of one dpm_system with the Verlet method running
+// dpm_jam (the pressure bisection of scripts/runArthurJamWithFire.cpp:36-106 around
+// minimizeFire, src/routines/minimization.cu:75-112): the same pair law and row order
+// (ascending index), the same FIRE update, the same canonical 64-element reduction
+// tree and the same controller steps, so every system's result is bit-identical to
+// that system jammed alone (alone or in any mix).
+//
+// Arena layout: system s owns slots [off_s, off_s + n_s) with off_s a multiple of 64,
+// so a wavefront never spans two systems and a wave's 64 lanes are one canonical
+// chunk.  Padding slots belong to their system but never hold a particle.
+//   P, snap   : double4 {x, y, radius, mass}      vel, frc, lref : double2
+//   pe        : double                            nnum, sys_of  : int32
+//   nbr       : int32 ELL, column-major nbr[k*S + slot] (S = arena slots)
+//   part      : per chunk, 9 partial sums / maxima (below)
+//
+// One batched step is a fixed sequence of launches:
+//   k_b_prep    snapshot save/restore and box scaling requested by the controller,
+//               zero velocities (minimizeFire prologue), displacement maxima
+//   k_b_move    FIRE move of systems inside a minimisation (Fire::step first half)
+//   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
+//   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
+//   k_b_force   forces (+ closing kick) and the chunk partials of P, PE, max F^2
+//   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision
+//   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials
+//   k_b_jam     per system: the jamming controller (next minimisation, or done)
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "dpm_kernels.hpp"
+
+namespace dpm {
+
+enum BMode { BM_PROLOGUE = 0, BM_FIRE = 1, BM_EVAL = 2, BM_DONE = 3 };
+enum BPhase { BJ_DECOMP = 0, BJ_COMP = 1 };
+enum BPart { BP_PX = 0, BP_PY, BP_PE, BP_F2, BP_SXX, BP_SYY, BP_R2, BP_DMAX, BP_Z, BP_COUNT };
+
+struct BSys {
+    long long off;  // first arena slot (multiple of 64)
+    int n, nc;      // particles, 64-slot chunks
+    double Lx, Ly;
+    double rmax;
+    // minimizeFire / Fire state (the fields of FireCtl)
+    double alpha, dt, dt_max, dt_min, last_avg_pe, avg_pe, power, fmax_sq;
+    long long n_good, n_bad, step;
+    int stopped, done, converged, uphill;
+    int mode;      // BMode of the next batched step
+    int rebuild;   // Verlet rebuild before the next force evaluation
+    int snap_op;   // pending for k_b_prep: 1 save, 2 restore
+    int pad0;
+    double scale_f;  // pending position scale (0: none)
+    // jamming controller (dpm_jam)
+    int jphase, dec_it, status, pad1;
+    double L_low, L_high, L_last, snapLx, snapLy, pr;
+    long long cstep, fire_total, contacts;
+    double pe_final, phi_final;
+};
+
+struct BDev {
+    long long nsys, nslot, ntiles;
+    int cap;
+    BSys* sys;
+    int* sys_of;
+    double4 *P, *snap;
+    double2 *vel, *frc, *lref;
+    double* pe;
+    int *nnum, *nbr;
+    double* part;      // BP_COUNT arrays of nslot/64 chunk values
+    int2* tiles;       // (system, first slot) of each build tile
+    int* flags;        // [0] capacity overflow, [1] largest row, [2] systems done
+    // common parameters
+    double e_c, n_c, ncm1, two_nc, inv2nc;
+    int fastdiv, pad;
+    double rc, rc2, thr2;
+    // FIRE and jamming parameters (dpm_jam_params)
+    double alpha0, dt0, pe_t, pe_diff_t, fmax2;
+    long long max_steps;
+    double P_low, P_high, rate, dfac, ltol, zmin;
+    long long max_csteps;
+};
+
+__device__ __forceinline__ double* bpart(const BDev& b, int which) {
+    return b.part + (long long)which * (b.nslot / WAVE);
+}
+
+__device__ __forceinline__ PairParams bparams(const BDev& b, const BSys& S) {
+    PairParams p{};
+    p.Lx = S.Lx;
+    p.Ly = S.Ly;
+    p.invLx = 1.0 / S.Lx;
+    p.invLy = 1.0 / S.Ly;
+    p.halfLx = 0.5 * S.Lx;
+    p.halfLy = 0.5 * S.Ly;
+    p.e_c = b.e_c;
+    p.n_c = b.n_c;
+    p.ncm1 = b.ncm1;
+    p.two_nc = b.two_nc;
+    p.inv2nc = b.inv2nc;
+    double lim = fmax(2.0 * S.rmax, b.rc);
+    p.fast = 0.5 * fmin(S.Lx, S.Ly) > lim * (1.0 + 1e-9) ? 1 : 0;
+    p.fastdiv = b.fastdiv;
+    return p;
+}
+
+__device__ __forceinline__ void fire_reset(const BDev& b, BSys& S) {
+    S.alpha = b.alpha0;
+    S.dt = b.dt0;
+    S.dt_max = 10.0 * b.dt0;
+    S.dt_min = 0.001 * b.dt0;
+    S.n_good = 0;
+    S.n_bad = 0;
+    S.step = 0;
+    S.stopped = 0;
+    S.done = 0;
+    S.converged = 0;
+    S.uphill = 0;
+    S.power = 0.0;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ __launch_bounds__(BLOCK) void k_b_prep(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    const BSys& S = b.sys[s];
+    const bool start = S.mode == BM_PROLOGUE;
+    if (!start && S.snap_op == 0) return;  // wave-uniform: one system per wave
+    const bool live = i - S.off < S.n;
+    double d2 = 0.0;
+    if (live) {
+        // the controller's order: revertToLastState or setLastState, then scaleBox;
+        // op 3 (leaving a full decompression loop) saves after the scaling
+        double4 q = b.P[i];
+        if (S.snap_op == 2) q = b.snap[i];
+        if (S.snap_op == 1) b.snap[i] = q;
+        if (S.scale_f != 0.0) {
+            q.x *= S.scale_f;
+            q.y *= S.scale_f;
+        }
+        if (S.snap_op == 3) b.snap[i] = q;
+        b.P[i] = q;
+        if (start) {
+            b.vel[i] = make_double2(0.0, 0.0);  // minimizeFire: setVelocitiesToZero
+            double2 l = b.lref[i];
+            double dx = q.x - l.x, dy = q.y - l.y;
+            d2 = dx * dx + dy * dy;
+        }
+    }
+    if (!start) return;
+    double m = wave_max(d2);
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = m;
+}
+
+// Fire::step first half (src/integrator/fire.cu:14-40): [uphill back-step with the
+// decreased dt, zero v] + kick(dt/2) + mix + drift(dt)
+__global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_FIRE) return;
+    const bool live = i - S.off < S.n;
+    double d2 = 0.0;
+    if (live) {
+        const double dt = S.dt, alpha = S.alpha;
+        double4 q = b.P[i];
+        double2 v = b.vel[i], f = b.frc[i];
+        if (S.uphill) {
+            double bdt = -dt / 2.0;
+            q.x += v.x * bdt;
+            q.y += v.y * bdt;
+            v.x = 0.0;
+            v.y = 0.0;
+        }
+        double dtm = (dt / 2.0) / q.w;
+        v.x += f.x * dtm;
+        v.y += f.y * dtm;
+        mix_one(v, f, alpha);
+        q.x += v.x * dt;
+        q.y += v.y * dt;
+        b.P[i] = q;
+        b.vel[i] = v;
+        double2 l = b.lref[i];
+        double dx = q.x - l.x, dy = q.y - l.y;
+        d2 = dx * dx + dy * dy;
+    }
+    double m = wave_max(d2);
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = m;
+}
+
+// Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
+// the controller's requests that k_b_prep has applied.
+__global__ void k_b_decide(BDev b) {
+    const long long s = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= b.nsys) return;
+    BSys& S = b.sys[s];
+    S.snap_op = 0;
+    S.scale_f = 0.0;
+    if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
+    const double* D = bpart(b, BP_DMAX) + S.off / WAVE;
+    double m = 0.0;
+    for (int c = 0; c < S.nc; ++c) m = fmax(m, D[c]);
+    if (3.0 * m > b.thr2) S.rebuild = 1;
+}
+
+// kernelUpdateNeighborList (base/kernels.cu:256-293) for the systems that need it:
+// every other particle of the system with d^2 < rc^2, ascending index.  One block per
+// 256-slot tile of a system; the system's positions pass through LDS.
+__global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
+    const int2 t = b.tiles[blockIdx.x];
+    const BSys& S = b.sys[t.x];
+    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL) return;
+    __shared__ double2 tp[BLOCK];
+    const long long i = (long long)t.y + threadIdx.x;
+    const long long end = S.off + S.n;
+    const bool live = i < end;
+    double4 qi = live ? b.P[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double Lx = S.Lx, Ly = S.Ly;
+    int cnt = 0;
+    for (long long base = S.off; base < end; base += BLOCK) {
+        __syncthreads();
+        if (base + threadIdx.x < end) {
+            double4 q = b.P[base + threadIdx.x];
+            tp[threadIdx.x] = make_double2(q.x, q.y);
+        }
+        __syncthreads();
+        int lim = (int)min((long long)BLOCK, end - base);
+        if (live) {
+            for (int u = 0; u < lim; ++u) {
+                long long j = base + u;
+                if (j == i) continue;
+                double dx = pbc(qi.x, tp[u].x, Lx, 1.0 / Lx, 0.5 * Lx, 0);
+                double dy = pbc(qi.y, tp[u].y, Ly, 1.0 / Ly, 0.5 * Ly, 0);
+                if (dx * dx + dy * dy < b.rc2) {
+                    if (cnt < b.cap) b.nbr[(long long)cnt * b.nslot + i] = (int)j;
+                    cnt++;
+                }
+            }
+        }
+    }
+    if (live) {
+        b.nnum[i] = cnt;
+        b.lref[i] = make_double2(qi.x, qi.y);
+    }
+    int m = block_max_int(live ? cnt : 0);
+    if (threadIdx.x == 0) {
+        atomicMax(&b.flags[1], m);
+        if (m > b.cap) atomicMax(&b.flags[0], 1);
+    }
+}
+
+// zero + calculateForces + (closing) updateVelocities(dt/2), chunk partials of
+// F.v, PE and max |F|^2 (k_fire_force's arithmetic)
+__global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
+    const int kick = S.mode == BM_FIRE;
+    const PairParams p = bparams(b, S);
+    const bool live = i - S.off < S.n;
+    const long long il = live ? i : S.off;
+    const double4 qi = b.P[il];
+    const int nn = live ? min(b.nnum[il], b.cap) : 0;
+    const int* col = b.nbr + il;
+    double ax = 0.0, ay = 0.0, e = 0.0;
+    const bool hot = p.fast && p.fastdiv;
+    for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; ++k) {
+        const bool valid = k < nn;
+        const int j = col[(long long)min(k, b.cap - 1) * b.nslot];
+        const double4 q = b.P[valid ? j : il];
+        double fx = 0.0, fy = 0.0, ie = 0.0;
+        if (hot) {
+            ie = pair_hot<true>(qi, q, valid, p, fx, fy);
+        } else if (valid) {
+            ie = pair_law(qi.x, qi.y, qi.z, q.x, q.y, q.z, p, fx, fy);
+        }
+        ax += fx;
+        ay += fy;
+        e += ie;
+    }
+    double pxv = 0.0, pyv = 0.0, pe = 0.0, f2 = 0.0;
+    if (live) {
+        double fx = 0.0 + ax, fy = 0.0 + ay;
+        b.frc[i] = make_double2(fx, fy);
+        pe = 0.0 + e;
+        b.pe[i] = pe;
+        double2 v = b.vel[i];
+        if (kick) {
+            double dtm = (S.dt / 2.0) / qi.w;
+            v.x += fx * dtm;
+            v.y += fy * dtm;
+            b.vel[i] = v;
+        }
+        pxv = fx * v.x;
+        pyv = fy * v.y;
+        f2 = fx * fx + fy * fy;
+    }
+    double s0 = wave_tree_sum(pxv);
+    double s1 = wave_tree_sum(pyv);
+    double s2 = wave_tree_sum(pe);
+    double m = wave_max(f2);
+    if ((threadIdx.x % WAVE) == 0) {
+        long long ch = i / WAVE;
+        bpart(b, BP_PX)[ch] = s0;
+        bpart(b, BP_PY)[ch] = s1;
+        bpart(b, BP_PE)[ch] = s2;
+        bpart(b, BP_F2)[ch] = m;
+    }
+}
+
+// one wave per system: minimizeFire's stop test (minimization.cu:96-105, with the
+// optional |F|max criterion) and Fire::step's decision for the next iteration
+// (fire.cu:14-33) -- k_fire_ctl's arithmetic on the system's chunk partials
+__global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
+    BSys& S = b.sys[blockIdx.x];
+    const int mode = S.mode;
+    if (mode != BM_PROLOGUE && mode != BM_FIRE) return;
+    const long long c0 = S.off / WAVE;
+    double px, py, pe, unused;
+    wave_final_sum4(bpart(b, BP_PX) + c0, bpart(b, BP_PY) + c0, bpart(b, BP_PE) + c0,
+                    bpart(b, BP_PE) + c0, S.nc, px, py, pe, unused);
+    double fm = 0.0;
+    for (int k = threadIdx.x; k < S.nc; k += WAVE) fm = fmax(fm, bpart(b, BP_F2)[c0 + k]);
+    fm = wave_max(fm);
+    if (threadIdx.x != 0) return;
+    S.rebuild = 0;  // the list this step used is current
+    const double dof = (double)S.n;
+    double avg = pe / dof;
+    S.fmax_sq = fm;
+    S.avg_pe = avg;
+    if (mode == BM_PROLOGUE) {
+        S.last_avg_pe = avg;
+        S.mode = BM_FIRE;
+        if (b.max_steps <= 0) {
+            S.done = 1;
+            S.mode = BM_EVAL;
+            return;
+        }
+    } else {
+        double diff = fabs(avg / S.last_avg_pe - 1);
+        S.last_avg_pe = avg;
+        bool fok = b.fmax2 > 0.0 && fm < b.fmax2;
+        S.step += 1;
+        if (diff < b.pe_diff_t || avg < b.pe_t || fok) {
+            S.converged = 1;
+            S.done = 1;
+            S.mode = BM_EVAL;
+            return;
+        }
+        if (S.step >= b.max_steps) {
+            S.done = 1;
+            S.mode = BM_EVAL;
+            return;
+        }
+    }
+    double power = px + py;
+    S.power = power;
+    if (power > 0) {
+        S.n_good++;
+        S.n_bad = 0;
+        if (S.n_good > 5) {
+            S.dt = fmin(S.dt * 1.1, S.dt_max);
+            S.alpha *= 0.99;
+        }
+        S.uphill = 0;
+    } else {
+        S.n_good = 0;
+        S.n_bad++;
+        if (S.n_bad > 10) {
+            S.stopped = 1;
+            S.step += 1;
+            S.done = 1;
+            S.mode = BM_EVAL;
+            return;
+        }
+        S.dt = fmax(S.dt * 0.5, S.dt_min);
+        S.alpha = b.alpha0;
+        S.uphill = 1;
+    }
+}
+
+// after a minimisation: setVelocitiesToZero (minimizeFire's end), then
+// calculateStressTensor (kernelCalcDiskStressTensor, disk/kernels.cu:203-251) and
+// countContacts (kernelCountDiskContacts, disk/kernels.cu:177-201) per particle, with
+// chunk partials of sigma_xx, sigma_yy, r^2 (packing fraction) and the contact count
+__global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_EVAL) return;
+    const PairParams p = bparams(b, S);
+    const bool live = i - S.off < S.n;
+    double sxx = 0.0, syy = 0.0, r2 = 0.0, z = 0.0;
+    if (live) {
+        b.vel[i] = make_double2(0.0, 0.0);
+        const double4 qi = b.P[i];
+        const double2 v = make_double2(0.0, 0.0);
+        const double m = qi.w;
+        double axx = m * v.x * v.x, ayy = m * v.y * v.y;
+        int c = 0;
+        const int nn = min(b.nnum[i], b.cap);
+        for (int k = 0; k < nn; ++k) {
+            int j = b.nbr[(long long)k * b.nslot + i];
+            double4 qj = b.P[j];
+            double xd = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, p.fast);
+            double yd = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, p.fast);
+            double fx, fy;
+            pair_law(qi.x, qi.y, qi.z, qj.x, qj.y, qj.z, p, fx, fy);
+            axx += xd * fx / 2;
+            ayy += yd * fy / 2;
+            double dist = sqrt(xd * xd + yd * yd);
+            if (dist < qi.z + qj.z) c++;
+        }
+        double area = S.Lx * S.Ly;
+        sxx = axx / area;
+        syy = ayy / area;
+        r2 = qi.z * qi.z;
+        z = (double)c;
+    }
+    double s0 = wave_tree_sum(sxx);
+    double s1 = wave_tree_sum(syy);
+    double s2 = wave_tree_sum(r2);
+    double s3 = wave_tree_sum(z);
+    if ((threadIdx.x % WAVE) == 0) {
+        long long ch = i / WAVE;
+        bpart(b, BP_SXX)[ch] = s0;
+        bpart(b, BP_SYY)[ch] = s1;
+        bpart(b, BP_R2)[ch] = s2;
+        bpart(b, BP_Z)[ch] = s3;
+    }
+}
+
+// Particle::scaleBox's box update (particle.cu:1057-1064): side = sqrt(area) * f
+__device__ __forceinline__ void scale_box_of(BSys& S, double f) {
+    double side = sqrt(S.Lx * S.Ly);
+    double ns = side * f;
+    S.Lx = ns;
+    S.Ly = ns;
+    S.scale_f = f;
+}
+
+__device__ __forceinline__ void start_minimization(const BDev& b, BSys& S) {
+    fire_reset(b, S);
+    S.mode = BM_PROLOGUE;
+}
+
+// one wave per system: the jamming controller of dpm_jam after a minimisation
+__global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
+    BSys& S = b.sys[blockIdx.x];
+    if (S.mode != BM_EVAL) return;
+    const long long c0 = S.off / WAVE;
+    double sxx, syy, pe, r2;
+    wave_final_sum4(bpart(b, BP_SXX) + c0, bpart(b, BP_SYY) + c0, bpart(b, BP_PE) + c0,
+                    bpart(b, BP_R2) + c0, S.nc, sxx, syy, pe, r2);
+    double z = 0.0;
+    for (int k = threadIdx.x; k < S.nc; k += WAVE) z += bpart(b, BP_Z)[c0 + k];
+    z = wave_tree_sum(z);  // integer counts: exact in any order
+    if (threadIdx.x != 0) return;
+    S.fire_total += min(S.step, b.max_steps);
+    const double pr = (sxx + syy) / 2;
+    const double P_low = b.P_low, P_high = b.P_high;
+    bool finished = false;
+    if (S.jphase == BJ_DECOMP) {
+        // for (it < 1000) { minimize; if (P < P_low) break; scaleBox(f); }
+        bool leave = pr < P_low;
+        int op = 1;
+        if (!leave) {
+            scale_box_of(S, b.dfac);
+            S.dec_it += 1;
+            leave = S.dec_it >= 1000;
+            op = 3;  // the snapshot below is taken after this scaling
+        }
+        if (leave) {  // L_* = -1, setLastState, then the compression loop
+            S.jphase = BJ_COMP;
+            S.L_low = S.L_high = S.L_last = -1;
+            S.cstep = 0;
+            S.status = 1;
+            S.pr = 0.0;
+            S.contacts = 0;
+            S.snap_op = op;
+            S.snapLx = S.Lx;
+            S.snapLy = S.Ly;
+            if (S.cstep >= b.max_csteps) finished = true;
+            else start_minimization(b, S);
+        } else {
+            start_minimization(b, S);
+        }
+    } else {
+        S.pr = pr;
+        S.contacts = (long long)z;
+        double L = sqrt(S.Lx * S.Ly);
+        double L_prior = L;
+        bool stop = false;
+        int restore = 0;
+        if (pr < P_low) {
+            S.L_low = L;
+            S.snap_op = 1;
+            S.snapLx = S.Lx;
+            S.snapLy = S.Ly;
+            S.L_last = L;
+            if (S.L_high != -1) {
+                L = (S.L_high + S.L_low) / 2;
+                S.L_high = -1;
+            } else {
+                L = L * b.rate;
+            }
+        } else {
+            if (pr > P_high) {
+                S.L_high = L;
+                restore = 1;
+                L_prior = S.L_last;
+                L = (S.L_high + S.L_low) / 2;
+            } else if (z > b.zmin * (double)S.n) {
+                S.status = 0;
+                stop = true;
+            }
+        }
+        if (!stop && S.L_low > 0 && S.L_high > 0 && fabs(S.L_low / S.L_high - 1) < b.ltol) {
+            S.status = 2;
+            stop = true;
+        }
+        if (restore) {  // revertToLastState: positions (k_b_prep) and box
+            S.snap_op = 2;
+            S.Lx = S.snapLx;
+            S.Ly = S.snapLy;
+        }
+        if (stop) {
+            finished = true;
+            if (restore) S.snap_op = 2;
+        } else {
+            scale_box_of(S, L / L_prior);
+            S.cstep += 1;
+            if (S.cstep >= b.max_csteps) finished = true;
+            else start_minimization(b, S);
+        }
+    }
+    if (finished) {
+        S.pe_final = pe;
+        const double PI = 3.141592653589793238462643383279502884197;
+        S.phi_final = r2 * PI / (S.Lx * S.Ly);
+        S.mode = BM_DONE;
+        atomicAdd(&b.flags[2], 1);
+    }
+}
+
+}  // namespace dpm
+
+// =========================================================================
+//                               host side
+// =========================================================================
+struct dpm_batch {
+    int64_t nsys = 0, nslot = 0, ntiles = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dpm::BDev d{};
+    std::vector<void*> allocs;
+    std::vector<long long> off;
+    std::vector<int> n;
+    std::vector<dpm::BSys> hsys;
+    std::vector<double> rmax, rmin;
+    double rc = 1.7, thr_sq = 3.0 * 0.15 * 0.15, e_c = 1.0, n_c = 2.0;
+    int64_t steps_per_graph = 32;
+    bool use_graph = true;
+    int* hflags = nullptr;  // pinned: [0] overflow, [1] largest row, [2] systems done
+    hipEvent_t poll[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t graph_steps = 0;
+};
+
+namespace dpm {
+
+static int balloc(dpm_batch* b, void** p, size_t bytes) {
+    void* q = nullptr;
+    DPM_HIP(hipMalloc(&q, std::max<size_t>(bytes, 64)));
+    DPM_HIP(hipMemset(q, 0, std::max<size_t>(bytes, 64)));
+    b->allocs.push_back(q);
+    *p = q;
+    return 0;
+}
+
+template <class T>
+static int balloc_n(dpm_batch* b, T** p, int64_t n) {
+    void* q = nullptr;
+    if (balloc(b, &q, sizeof(T) * (size_t)std::max<int64_t>(n, 1))) return -1;
+    *p = (T*)q;
+    return 0;
+}
+
+static void enqueue_batch_step(dpm_batch* b) {
+    const BDev& d = b->d;
+    hipStream_t st = b->stream;
+    dim3 slots((unsigned)((b->nslot + BLOCK - 1) / BLOCK));
+    dim3 sysb((unsigned)((b->nsys + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL(k_b_prep, slots, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_move, slots, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_decide, sysb, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_force, slots, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
+    hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_jam, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
+}
+
+static int ensure_batch_graph(dpm_batch* b, int64_t k) {
+    if (b->exec && b->graph_steps == k) return 0;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+    }
+    DPM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int64_t s = 0; s < k; ++s) enqueue_batch_step(b);
+    DPM_HIP(hipStreamEndCapture(b->stream, &b->graph));
+    DPM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_steps = k;
+    return 0;
+}
+
+}  // namespace dpm
+
+using namespace dpm;
+
+extern "C" {
+
+dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
+    if (n_systems <= 0 || n_systems >= (int64_t)1 << 24 || !n_particles) {
+        set_error("dpm_batch_create: bad arguments");
+        return nullptr;
+    }
+    if (dpm_device_count() <= 0) {
+        set_error("dpm_batch_create: no HIP device");
+        return nullptr;
+    }
+    auto* b = new dpm_batch();
+    b->nsys = n_systems;
+    b->off.resize(n_systems);
+    b->n.resize(n_systems);
+    b->rmax.assign(n_systems, 0.0);
+    b->rmin.assign(n_systems, 0.0);
+    long long acc = 0;
+    std::vector<int2> tiles;
+    for (int64_t s = 0; s < n_systems; ++s) {
+        int64_t ns = n_particles[s];
+        if (ns <= 0 || ns >= (int64_t)1 << 24) {
+            set_error("dpm_batch_create: every system needs 1 .. 2^24 particles");
+            delete b;
+            return nullptr;
+        }
+        b->off[s] = acc;
+        b->n[s] = (int)ns;
+        for (int64_t t = 0; t < ns; t += BLOCK) tiles.push_back(make_int2((int)s, (int)(acc + t)));
+        acc += (ns + WAVE - 1) / WAVE * WAVE;
+    }
+    if (acc >= (int64_t)1 << 31) {
+        set_error("dpm_batch_create: arena exceeds 2^31 slots");
+        delete b;
+        return nullptr;
+    }
+    b->nslot = acc;
+    b->ntiles = (int64_t)tiles.size();
+    (void)hipGetDevice(&b->device);
+    BDev& d = b->d;
+    d.nsys = n_systems;
+    d.nslot = acc;
+    d.ntiles = b->ntiles;
+    d.cap = 32;
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && !balloc_n(b, &d.sys, n_systems) && !balloc_n(b, &d.sys_of, acc) &&
+         !balloc_n(b, &d.P, acc) && !balloc_n(b, &d.snap, acc) && !balloc_n(b, &d.vel, acc) &&
+         !balloc_n(b, &d.frc, acc) && !balloc_n(b, &d.lref, acc) && !balloc_n(b, &d.pe, acc) &&
+         !balloc_n(b, &d.nnum, acc) && !balloc_n(b, &d.nbr, acc * d.cap) &&
+         !balloc_n(b, &d.part, (acc / WAVE) * BP_COUNT) && !balloc_n(b, &d.tiles, b->ntiles) &&
+         !balloc_n(b, &d.flags, 4);
+    if (ok) {
+        std::vector<int> so(acc);
+        for (int64_t s = 0; s < n_systems; ++s) {
+            long long end = s + 1 < n_systems ? b->off[s + 1] : acc;
+            for (long long k = b->off[s]; k < end; ++k) so[k] = (int)s;
+        }
+        ok = hipMemcpy(d.sys_of, so.data(), sizeof(int) * acc, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemcpy(d.tiles, tiles.data(), sizeof(int2) * tiles.size(),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    }
+    ok = ok && hipHostMalloc((void**)&b->hflags, 2 * 4 * sizeof(int), hipHostMallocDefault) ==
+                   hipSuccess;
+    for (int k = 0; ok && k < 2; ++k)
+        ok = hipEventCreateWithFlags(&b->poll[k], hipEventDisableTiming) == hipSuccess;
+    b->hsys.assign(n_systems, BSys{});
+    for (int64_t s = 0; s < n_systems; ++s) {
+        b->hsys[s].off = b->off[s];
+        b->hsys[s].n = b->n[s];
+        b->hsys[s].nc = (b->n[s] + WAVE - 1) / WAVE;
+        b->hsys[s].Lx = b->hsys[s].Ly = 1.0;
+    }
+    if (!ok) {
+        set_error("dpm_batch_create: device allocation failed");
+        dpm_batch_destroy(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void dpm_batch_destroy(dpm_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->hflags) (void)hipHostFree(b->hflags);
+    for (auto e : b->poll)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int dpm_batch_set_system(dpm_batch* b, int64_t s, const double* x, const double* y,
+                         const double* rad, const double* mass, double box_side) {
+    if (s < 0 || s >= b->nsys || !(box_side > 0)) {
+        set_error("dpm_batch_set_system: bad system index or box side");
+        return -1;
+    }
+    int ns = b->n[s];
+    std::vector<double4> h(ns);
+    double rm = 0.0, rn = 1e300;
+    for (int k = 0; k < ns; ++k) {
+        h[k] = make_double4(x[k], y[k], rad[k], mass[k]);
+        rm = std::max(rm, rad[k]);
+        rn = std::min(rn, rad[k]);
+        if (!(rad[k] > 0) || !(mass[k] > 0)) {
+            set_error("dpm_batch_set_system: radii and masses must be positive");
+            return -1;
+        }
+    }
+    DPM_HIP(hipMemcpy(b->d.P + b->off[s], h.data(), sizeof(double4) * ns, hipMemcpyHostToDevice));
+    b->rmax[s] = rm;
+    b->rmin[s] = rn;
+    b->hsys[s].Lx = b->hsys[s].Ly = box_side;
+    b->hsys[s].rmax = rm;
+    return 0;
+}
+
+int dpm_batch_get_system(dpm_batch* b, int64_t s, double* x, double* y, double* box_side) {
+    if (s < 0 || s >= b->nsys) {
+        set_error("dpm_batch_get_system: bad system index");
+        return -1;
+    }
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    int ns = b->n[s];
+    std::vector<double4> h(ns);
+    DPM_HIP(hipMemcpy(h.data(), b->d.P + b->off[s], sizeof(double4) * ns, hipMemcpyDeviceToHost));
+    for (int k = 0; k < ns; ++k) {
+        if (x) x[k] = h[k].x;
+        if (y) y[k] = h[k].y;
+    }
+    if (box_side) {
+        BSys S;
+        DPM_HIP(hipMemcpy(&S, b->d.sys + s, sizeof(BSys), hipMemcpyDeviceToHost));
+        *box_side = S.Lx;
+    }
+    return 0;
+}
+
+int dpm_batch_config(dpm_batch* b, double e_c, double n_c, double cutoff, double disp_thresh_sq) {
+    if (!(e_c > 0) || !(n_c > 0) || !(cutoff > 0) || !(disp_thresh_sq > 0)) {
+        set_error("dpm_batch_config: parameters must be positive");
+        return -1;
+    }
+    b->e_c = e_c;
+    b->n_c = n_c;
+    b->rc = cutoff;
+    b->thr_sq = disp_thresh_sq;
+    return 0;
+}
+
+int dpm_batch_set_option(dpm_batch* b, int option, int64_t value) {
+    if (option == DPM_OPT_GRAPH) {
+        b->use_graph = value != 0;
+        return 0;
+    }
+    if (option == DPM_OPT_CHUNK) {
+        if (value < 1 || value > 1024) {
+            set_error("dpm_batch_set_option: steps per launch must be in [1, 1024]");
+            return -1;
+        }
+        b->steps_per_graph = value;
+        return 0;
+    }
+    set_error("dpm_batch_set_option: unknown option");
+    return -1;
+}
+
+int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_steps,
+                  dpm_jam_result* out, int64_t* steps_done) {
+    if (!P || !out || max_batched_steps < 0) {
+        set_error("dpm_batch_jam: bad arguments");
+        return -1;
+    }
+    BDev& d = b->d;
+    // common parameters (pair_params's derived numbers, dpm_jam's loop constants)
+    d.e_c = b->e_c;
+    d.n_c = b->n_c;
+    d.ncm1 = b->n_c - 1.0;
+    d.two_nc = 2.0 * b->n_c;
+    {
+        int ex;
+        double m = frexp(d.two_nc, &ex);
+        d.inv2nc = (m == 0.5) ? 1.0 / d.two_nc : 0.0;
+    }
+    double rmin_all = 1e300, rmax_all = 0.0;
+    for (int64_t s = 0; s < b->nsys; ++s) {
+        if (b->rmin[s] <= 0) {
+            set_error("dpm_batch_jam: system " + std::to_string(s) + " has no particles set");
+            return -1;
+        }
+        rmin_all = std::min(rmin_all, b->rmin[s]);
+        rmax_all = std::max(rmax_all, b->rmax[s]);
+    }
+    const double lo = 0x1p-100, hi = 0x1p100;
+    d.fastdiv = (b->n_c == 2.0 && b->e_c >= lo && b->e_c <= hi && rmin_all >= lo && rmax_all <= hi)
+                    ? 1 : 0;
+    d.rc = b->rc;
+    d.rc2 = b->rc * b->rc;
+    d.thr2 = b->thr_sq;
+    d.alpha0 = P->fire.alpha_init;
+    d.dt0 = P->fire.dt;
+    d.pe_t = P->fire.avg_pe_target;
+    d.pe_diff_t = P->fire.avg_pe_diff_target;
+    d.fmax2 = P->fire.fmax_target > 0 ? P->fire.fmax_target * P->fire.fmax_target : 0.0;
+    d.max_steps = P->fire.max_steps;
+    d.P_low = P->pressure_target;
+    d.P_high = P->pressure_target * (1.0 + P->window_rel);
+    d.rate = 1.0 - P->strain_step;
+    d.dfac = P->decompress_factor;
+    d.ltol = P->length_tol;
+    d.zmin = P->min_contacts_per_particle;
+    d.max_csteps = P->max_compression_steps;
+    // every system starts minimizeFire (decompression phase) with a fresh list
+    for (int64_t s = 0; s < b->nsys; ++s) {
+        BSys& S = b->hsys[s];
+        BSys f{};
+        f.off = S.off;
+        f.n = S.n;
+        f.nc = S.nc;
+        f.Lx = S.Lx;
+        f.Ly = S.Ly;
+        f.rmax = b->rmax[s];
+        f.alpha = d.alpha0;
+        f.dt = d.dt0;
+        f.dt_max = 10.0 * d.dt0;
+        f.dt_min = 0.001 * d.dt0;
+        f.mode = BM_PROLOGUE;
+        f.rebuild = 1;
+        f.jphase = BJ_DECOMP;
+        f.status = 1;
+        f.L_low = f.L_high = f.L_last = -1;
+        S = f;
+    }
+    DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
+                           b->stream));
+    DPM_HIP(hipMemsetAsync(d.flags, 0, 4 * sizeof(int), b->stream));
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    if (b->exec) {  // the parameters are baked into the graph's kernel arguments
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+    }
+    // up to two launch units in flight; each ends with a copy of the flags
+    int64_t launched = 0, finished_steps = 0;
+    int inflight = 0, slot = 0;
+    int64_t unit_steps[2] = {0, 0};
+    bool all_done = false;
+    while (!all_done) {
+        while (inflight < 2 && launched < max_batched_steps) {
+            int64_t k = std::min<int64_t>(b->steps_per_graph, max_batched_steps - launched);
+            if (b->use_graph && k == b->steps_per_graph) {
+                if (ensure_batch_graph(b, k)) return -1;
+                DPM_HIP(hipGraphLaunch(b->exec, b->stream));
+            } else {
+                for (int64_t s = 0; s < k; ++s) enqueue_batch_step(b);
+                if (check_launch()) return -1;
+            }
+            int ws = (slot + inflight) % 2;
+            DPM_HIP(hipMemcpyAsync(b->hflags + 4 * ws, d.flags, 4 * sizeof(int),
+                                   hipMemcpyDeviceToHost, b->stream));
+            DPM_HIP(hipEventRecord(b->poll[ws], b->stream));
+            unit_steps[ws] = k;
+            launched += k;
+            inflight++;
+        }
+        if (inflight == 0) break;
+        DPM_HIP(hipEventSynchronize(b->poll[slot]));
+        finished_steps += unit_steps[slot];
+        const int* f = b->hflags + 4 * slot;
+        inflight--;
+        slot = (slot + 1) % 2;
+        if (f[0]) {
+            DPM_HIP(hipStreamSynchronize(b->stream));
+            set_error("dpm_batch_jam: a neighbour row exceeded the row capacity");
+            return -1;
+        }
+        if (f[2] >= b->nsys) all_done = true;
+    }
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    if (steps_done) *steps_done = finished_steps;
+    DPM_HIP(hipMemcpy(b->hsys.data(), d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < b->nsys; ++s) {
+        const BSys& S = b->hsys[s];
+        dpm_jam_result& r = out[s];
+        r.compression_steps = S.cstep;
+        r.fire_steps_total = S.fire_total;
+        r.status = S.mode == BM_DONE ? S.status : -1;  // -1: still running
+        r.pressure = S.pr;
+        r.packing_fraction = S.phi_final;
+        r.pe = S.pe_final;
+        r.box_length = S.Lx;
+        r.contacts = S.contacts;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -408,21 +408,6 @@ __global__ __launch_bounds__(BLOCK) void k_drift(Dev d, NbrParams nb, double dt)
 }
 
 // kernelMixVelocitiesAndForces, src/particles/base/kernels.cu:91-112
-__device__ __forceinline__ void mix_one(double2& v, double2 f, double alpha) {
-    double fn = sqrt(f.x * f.x + f.y * f.y);
-    double vn = sqrt(v.x * v.x + v.y * v.y);
-    double mr = 0.0;
-    if (fn > 1e-16) {
-        mr = vn / fn * alpha;
-    } else {
-        v.x = 0.0;
-        v.y = 0.0;
-        mr = 0.0;
-    }
-    v.x = v.x * (1 - alpha) + f.x * mr;
-    v.y = v.y * (1 - alpha) + f.y * mr;
-}
-
 __global__ void k_mix(Dev d, double alpha) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
@@ -2458,10 +2443,12 @@ int dpm_scale_positions(dpm_system* s, double f) {
     return check_launch();
 }
 
-// Particle::scaleBox (particle.cu:1057-1064)
+// Particle::scaleBox (particle.cu:1057-1064).  The reference's pow(area, 1/2) is taken
+// as the correctly rounded square root (the batched engine computes the same on the
+// device, where only sqrt is correctly rounded).
 int dpm_scale_box(dpm_system* s, double f) {
     if (dpm_scale_positions(s, f)) return -1;
-    double side = std::pow(s->box[0] * s->box[1], 1.0 / 2);
+    double side = std::sqrt(s->box[0] * s->box[1]);
     double ns = side * f;
     return dpm_set_box(s, ns, ns);
 }

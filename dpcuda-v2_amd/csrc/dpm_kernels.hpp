@@ -104,6 +104,23 @@ __device__ __forceinline__ int pending_rebuild(const Ctl* c, long long epoch, in
     return list ? 1 : 0;
 }
 
+// kernelMixVelocitiesAndForces (src/particles/base/kernels.cu:91-112): per-particle FIRE
+// mixing with the 1e-16 force-norm guard
+__device__ __forceinline__ void mix_one(double2& v, double2 f, double alpha) {
+    double fn = sqrt(f.x * f.x + f.y * f.y);
+    double vn = sqrt(v.x * v.x + v.y * v.y);
+    double mr = 0.0;
+    if (fn > 1e-16) {
+        mr = vn / fn * alpha;
+    } else {
+        v.x = 0.0;
+        v.y = 0.0;
+        mr = 0.0;
+    }
+    v.x = v.x * (1 - alpha) + f.x * mr;
+    v.y = v.y * (1 - alpha) + f.y * mr;
+}
+
 // ---- host helpers (dpm_core.hip) ----
 int dalloc_bytes(dpm_system* s, void** p, size_t bytes);
 template <class T>

@@ -190,6 +190,29 @@ int dpm_jam(dpm_system* s, const dpm_jam_params* p, dpm_jam_result* out);
 int dpm_bumpy_setup(dpm_system* s, const int64_t* n_vertices_per_particle, double vertex_radius,
                     int rotation);
 
+/* ---- batched jamming engine (no reference counterpart: the reference jams one system per
+ * process; batching is a TODO there, todo.txt:43) ---------------------------------------------
+ * Many independent disk systems of uneven N in one device arena.  dpm_batch_jam runs dpm_jam
+ * (the Verlet-list form) on every system at once: one launch sequence per step advances each
+ * system's own minimizeFire / controller state, and finished systems drop out.  Each
+ * system's result is bit-identical to dpm_jam on a dpm_system holding the same particles
+ * with the Verlet method, cutoff and threshold given here.
+ * dpm_batch_config: pair law (e_c, n_c), Verlet cutoff and rebuild threshold (3 max d^2 >
+ * disp_thresh_sq), common to all systems.  dpm_batch_jam: out[] has n_systems entries;
+ * status -1 marks a system still running after max_batched_steps; steps_done returns the
+ * batched steps executed.  dpm_batch_set_option takes DPM_OPT_GRAPH and DPM_OPT_CHUNK
+ * (batched steps per graph launch). */
+typedef struct dpm_batch dpm_batch;
+dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles);
+void dpm_batch_destroy(dpm_batch* b);
+int dpm_batch_set_system(dpm_batch* b, int64_t system, const double* x, const double* y,
+                         const double* radius, const double* mass, double box_side);
+int dpm_batch_get_system(dpm_batch* b, int64_t system, double* x, double* y, double* box_side);
+int dpm_batch_config(dpm_batch* b, double e_c, double n_c, double cutoff, double disp_thresh_sq);
+int dpm_batch_set_option(dpm_batch* b, int option, int64_t value);
+int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* p, int64_t max_batched_steps,
+                  dpm_jam_result* out, int64_t* steps_done);
+
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
  * DPM_OPT_CHUNK: steps per host poll of the device status (default 64).

@@ -807,10 +807,11 @@ void scale_positions(orc_system* s, double f) {
     }
 }
 
-// Particle::scaleBox particle.cu:1057-1064
+// Particle::scaleBox particle.cu:1057-1064 (pow(area, 1/2) taken as the correctly
+// rounded square root)
 void scale_box(orc_system* s, double f) {
     scale_positions(s, f);
-    double side = std::pow(s->box[0] * s->box[1], 1.0 / 2);
+    double side = std::sqrt(s->box[0] * s->box[1]);
     double ns = side * f;
     s->box[0] = ns;
     s->box[1] = ns;
