@@ -203,6 +203,91 @@ def rigid_fire(args, P, W):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- batched jamming
+def jam_batch(args, P, W):
+    """Jamming of many independent disk systems of uneven N: the batched engine against
+    the plain one-system-at-a-time GPU path (dpm_jam, same Verlet lists, same result bits),
+    both on the same systems.  value = jammed packings per second of the batched engine."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    ns = np.exp(rng.uniform(np.log(64), np.log(2048), args.systems)).astype(np.int64)
+    sts = [W.disk_config(int(n), args.jam_phi0, seed=args.seed + 1 + k) for k, n in enumerate(ns)]
+    prm = P.jam_params(pressure_target=args.jam_pressure, window_rel=args.jam_window,
+                       strain_step=args.jam_strain, max_compression_steps=args.jam_csteps,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=args.jam_fmax,
+                                          max_steps=args.jam_fire_steps))
+    rc, thr_sq = 1.7, 3.0 * 0.15 * 0.15
+    print(f"[bench] batched jamming: {len(sts)} systems, {int(ns.sum())} particles",
+          file=sys.stderr, flush=True)
+    b = P.Batch([st.n for st in sts])
+    b.config(1.0, 2.0, rc, thr_sq)
+    b.set_option(P.OPT_CHUNK, args.batch_chunk)
+    for k, st in enumerate(sts):
+        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    t0 = time.perf_counter()
+    res, steps = b.jam(prm, args.batch_max_steps)
+    elapsed = time.perf_counter() - t0
+    fin = b.finish_steps()
+    done = [k for k in range(len(sts)) if res[k].status >= 0]
+    step_s = elapsed / max(steps, 1)
+    lat = np.sort(fin[fin > 0] * step_s)
+    p50 = float(np.percentile(lat, 50)) if len(lat) else None
+    p99 = float(np.percentile(lat, 99)) if len(lat) else None
+    # plain path on a sample of the same systems, one at a time
+    k_sample = min(args.plain_sample, len(sts))
+    pick = rng.choice(len(sts), size=k_sample, replace=False)
+    print(f"[bench] plain path on {k_sample} of the systems", file=sys.stderr, flush=True)
+    t_plain, same = [], True
+    for k in pick:
+        st = sts[int(k)]
+        s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
+        s.synchronize()
+        c0 = time.perf_counter()
+        r = s.jam(prm)
+        s.synchronize()
+        t_plain.append(time.perf_counter() - c0)
+        for f in ("status", "fire_steps_total", "pressure", "pe", "packing_fraction"):
+            same = same and getattr(r, f) == getattr(res[int(k)], f)
+        s.close()
+    plain_jams_s = 1.0 / (sum(t_plain) / len(t_plain))
+    value = len(done) / elapsed
+    out = {
+        "metric": "batched jammed packings/s vs the plain one-system-at-a-time GPU path",
+        "value": value,
+        "unit": "packings/s",
+        "n_gpus": 1,
+        "steps": int(steps),
+        "warmup": 0,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform positions; N log-uniform in [64, 2048])",
+        "config": {
+            "workload": f"{len(sts)} bidisperse disk systems (1.0/1.4, 50/50), N log-uniform "
+                        f"in [64, 2048], phi0={args.jam_phi0}, compress by strain "
+                        f"{args.jam_strain} with FIRE to |F|max<{args.jam_fmax} (at most "
+                        f"{args.jam_fire_steps} steps each), pressure bisection to "
+                        f"{args.jam_pressure} (window {args.jam_window}), Verlet rc=1.7",
+            "systems": len(sts),
+            "particles": int(ns.sum()),
+            "jammed": len(done),
+            "elapsed_s": elapsed,
+            "latency_p50_s": p50,
+            "latency_p99_s": p99,
+            "plain_jams_per_s": plain_jams_s,
+            "plain_sample": k_sample,
+            "plain_mean_s": sum(t_plain) / len(t_plain),
+            "speedup_vs_plain": value / plain_jams_s,
+            "sample_results_bit_identical": same,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def traffic_from_profiles(n):
     path = os.path.join(ROOT, "profiles", "k_nve_fused_n262144_traffic.json")
     if not os.path.exists(path):
@@ -227,7 +312,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--chunk", type=int, default=64)
-    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid"), default="nve-disk",
+    ap.add_argument("--systems", type=int, default=4096)
+    ap.add_argument("--jam-phi0", type=float, default=0.80)
+    ap.add_argument("--jam-pressure", type=float, default=1e-7)
+    ap.add_argument("--jam-window", type=float, default=0.05)
+    ap.add_argument("--jam-strain", type=float, default=1e-4)
+    ap.add_argument("--jam-fmax", type=float, default=1e-12)
+    ap.add_argument("--jam-fire-steps", type=int, default=100000)
+    ap.add_argument("--jam-csteps", type=int, default=100000)
+    ap.add_argument("--batch-chunk", type=int, default=32)
+    ap.add_argument("--batch-max-steps", type=int, default=10**9)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch"),
+                    default="nve-disk",
                     help="nve-disk: the headline NVE benchmark (default); fire-rigid: FIRE "
                          "of N=65,536 rigid bumpy particles to |F|max < 1e-12")
     ap.add_argument("--fire-max-steps", type=int, default=200000)
@@ -258,6 +355,9 @@ def main():
     P, W = load_pkg()
     if args.workload == "fire-rigid":
         rigid_fire(args, P, W)
+        return
+    if args.workload == "jam-batch":
+        jam_batch(args, P, W)
         return
     n = args.n
     print(f"[bench] preparing N={n} (FIRE 200 steps, T=1e-5)", file=sys.stderr, flush=True)

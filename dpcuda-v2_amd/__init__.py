@@ -149,6 +149,7 @@ _GPU_ONLY = {
     "batch_config": (_I, [_P, _D, _D, _D, _D]),
     "batch_set_option": (_I, [_P, _I, _L]),
     "batch_jam": (_I, [_P, C.POINTER(JamParams), _L, _P, C.POINTER(_L)]),
+    "batch_finish_steps": (_I, [_P, _P]),
 }
 
 
@@ -461,6 +462,11 @@ class Batch:
 
     def set_option(self, opt, value):
         self._c("batch_set_option", int(opt), int(value))
+
+    def finish_steps(self):
+        out = np.empty(len(self.sizes), dtype=np.int64)
+        self._c("batch_finish_steps", _ptr(out))
+        return out
 
     def jam(self, params: JamParams, max_batched_steps=10**9):
         out = (JamResult * len(self.sizes))()

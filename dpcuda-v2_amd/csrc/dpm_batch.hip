@@ -60,6 +60,7 @@ struct BSys {
     double L_low, L_high, L_last, snapLx, snapLy, pr;
     long long cstep, fire_total, contacts;
     double pe_final, phi_final;
+    long long done_step;  // batched step in which the system finished
 };
 
 struct BDev {
@@ -73,7 +74,8 @@ struct BDev {
     int *nnum, *nbr;
     double* part;      // BP_COUNT arrays of nslot/64 chunk values
     int2* tiles;       // (system, first slot) of each build tile
-    int* flags;        // [0] capacity overflow, [1] largest row, [2] systems done
+    int* flags;        // [0] capacity overflow, [1] largest row, [2] systems done,
+                       // [3] batched steps executed
     // common parameters
     double e_c, n_c, ncm1, two_nc, inv2nc;
     int fastdiv, pad;
@@ -198,6 +200,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
 // the controller's requests that k_b_prep has applied.
 __global__ void k_b_decide(BDev b) {
     const long long s = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (s == 0) b.flags[3] += 1;  // one thread of the step counts it
     if (s >= b.nsys) return;
     BSys& S = b.sys[s];
     S.snap_op = 0;
@@ -544,6 +547,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
         }
     }
     if (finished) {
+        S.done_step = b.flags[3];
         S.pe_final = pe;
         const double PI = 3.141592653589793238462643383279502884197;
         S.phi_final = r2 * PI / (S.Lx * S.Ly);
@@ -926,6 +930,18 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
         r.box_length = S.Lx;
         r.contacts = S.contacts;
     }
+    return 0;
+}
+
+int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out) {
+    if (!steps_out) {
+        set_error("dpm_batch_finish_steps: null output");
+        return -1;
+    }
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    DPM_HIP(hipMemcpy(b->hsys.data(), b->d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < b->nsys; ++s)
+        steps_out[s] = b->hsys[s].mode == BM_DONE ? b->hsys[s].done_step : -1;
     return 0;
 }
 

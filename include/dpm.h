@@ -212,6 +212,9 @@ int dpm_batch_config(dpm_batch* b, double e_c, double n_c, double cutoff, double
 int dpm_batch_set_option(dpm_batch* b, int option, int64_t value);
 int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* p, int64_t max_batched_steps,
                   dpm_jam_result* out, int64_t* steps_done);
+/* the batched step (1-based) in which each system finished its last dpm_batch_jam, -1 if
+ * it has not */
+int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out);
 
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
