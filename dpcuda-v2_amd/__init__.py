@@ -81,6 +81,14 @@ def fire_params(alpha_init=0.1, dt=1e-2, avg_pe_target=1e-16, avg_pe_diff_target
                       int(max_steps))
 
 
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("n_particles", C.c_int64),
+                ("n_vertices", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("radius", C.c_void_p), ("mass", C.c_void_p), ("angle", C.c_void_p),
+                ("vertices_per_particle", C.c_void_p), ("vertex_radius", C.c_double),
+                ("box_side", C.c_double), ("e_c", C.c_double), ("jam", JamParams)]
+
+
 def jam_params(pressure_target=1e-7, window_rel=1e-2, strain_step=1e-4,
                decompress_factor=1.01, length_tol=1e-10, min_contacts_per_particle=0.0,
                max_compression_steps=100000, fire=None):
@@ -150,6 +158,14 @@ _GPU_ONLY = {
     "batch_set_option": (_I, [_P, _I, _L]),
     "batch_jam": (_I, [_P, C.POINTER(JamParams), _L, _P, C.POINTER(_L)]),
     "batch_finish_steps": (_I, [_P, _P]),
+    "server_create": (_P, [_L, _L, _L, C.c_int32]),
+    "server_destroy": (None, [_P]),
+    "server_submit": (_L, [_P, C.POINTER(Request)]),
+    "server_progress": (_I, [_P, _L, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
+    "server_result": (_I, [_P, _L, C.POINTER(C.c_int32), C.POINTER(JamResult), C.POINTER(_D),
+                           _P, _P, C.POINTER(_D)]),
+    "batch_autotune": (_I, [_P, C.POINTER(JamParams), _L, _P, _I, _P, _I, C.POINTER(_D),
+                           C.POINTER(_L), C.POINTER(_D)]),
 }
 
 
@@ -463,6 +479,15 @@ class Batch:
     def set_option(self, opt, value):
         self._c("batch_set_option", int(opt), int(value))
 
+    def autotune(self, params: JamParams, trial_steps=200, cutoffs=(1.6, 1.7, 1.9),
+                 chunks=(16, 32, 64)):
+        c = np.ascontiguousarray(cutoffs, dtype=np.float64)
+        k = np.ascontiguousarray(chunks, dtype=np.int64)
+        bc, bk, ms = C.c_double(), C.c_int64(), C.c_double()
+        self._c("batch_autotune", C.byref(params), int(trial_steps), _ptr(c), len(c), _ptr(k),
+                len(k), C.byref(bc), C.byref(bk), C.byref(ms))
+        return bc.value, bk.value, ms.value
+
     def finish_steps(self):
         out = np.empty(len(self.sizes), dtype=np.int64)
         self._c("batch_finish_steps", _ptr(out))
@@ -474,3 +499,74 @@ class Batch:
         self._c("batch_jam", C.byref(params), int(max_batched_steps), C.cast(out, C.c_void_p),
                 C.byref(steps))
         return list(out), steps.value
+
+
+class Server:
+    """Continuous batching of jamming requests on one GPU (dpm_server_*)."""
+
+    def __init__(self, max_systems=4096, arena_slots=1 << 22, budget_bytes=8 << 30,
+                 bumpy_workers=2):
+        self.be = load()
+        self.h = self.be.fn["server_create"](int(max_systems), int(arena_slots),
+                                             int(budget_bytes), int(bumpy_workers))
+        if not self.h:
+            raise DpmError(f"dpm_server_create: {self.be.err()}")
+        self.n = {}
+
+    def close(self):
+        if self.h:
+            self.be.fn["server_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, x, y, rad, mass, side, jam: JamParams, e_c=1.0, angle=None,
+               vertices_per_particle=None, vertex_radius=0.0):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, rad, mass)]
+        r = Request()
+        bumpy = angle is not None
+        r.kind = BUMPY if bumpy else DISK
+        r.n_particles = len(arrs[0])
+        r.x, r.y, r.radius, r.mass = [_ptr(a) for a in arrs]
+        keep = list(arrs)
+        if bumpy:
+            a = np.ascontiguousarray(angle, dtype=np.float64)
+            v = np.ascontiguousarray(vertices_per_particle, dtype=np.int64)
+            keep += [a, v]
+            r.angle, r.vertices_per_particle = _ptr(a), _ptr(v)
+            r.n_vertices = int(v.sum())
+            r.vertex_radius = float(vertex_radius)
+        r.box_side = float(side)
+        r.e_c = float(e_c)
+        r.jam = jam
+        rid = self.be.fn["server_submit"](self.h, C.byref(r))
+        if rid < 0:
+            raise DpmError(f"dpm_server_submit: {self.be.err()}")
+        self.n[rid] = r.n_particles
+        return rid
+
+    def progress(self, batched_steps=32):
+        f, i, q = C.c_int64(), C.c_int64(), C.c_int64()
+        if self.be.fn["server_progress"](self.h, int(batched_steps), C.byref(f), C.byref(i),
+                                         C.byref(q)) < 0:
+            raise DpmError(f"dpm_server_progress: {self.be.err()}")
+        return f.value, i.value, q.value
+
+    def result(self, rid):
+        st = C.c_int32()
+        out = JamResult()
+        lat = C.c_double()
+        n = self.n[rid]
+        x, y = np.empty(n), np.empty(n)
+        side = C.c_double()
+        if self.be.fn["server_result"](self.h, int(rid), C.byref(st), C.byref(out), C.byref(lat),
+                                       _ptr(x), _ptr(y), C.byref(side)) < 0:
+            raise DpmError(f"dpm_server_result: {self.be.err()}")
+        if st.value != 2:
+            return {"state": st.value}
+        return {"state": 2, "result": out, "latency_s": lat.value, "x": x, "y": y,
+                "box_side": side.value}

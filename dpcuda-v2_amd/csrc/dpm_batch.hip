@@ -29,6 +29,7 @@
 //   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials
 //   k_b_jam     per system: the jamming controller (next minimisation, or done)
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -37,7 +38,15 @@
 
 namespace dpm {
 
-enum BMode { BM_PROLOGUE = 0, BM_FIRE = 1, BM_EVAL = 2, BM_DONE = 3 };
+enum BMode { BM_PROLOGUE = 0, BM_FIRE = 1, BM_EVAL = 2, BM_DONE = 3, BM_EMPTY = 4 };
+
+// dpm_jam_params of one system (minimizeFire's and the controller's constants)
+struct BJam {
+    double alpha0, dt0, pe_t, pe_diff_t, fmax2;
+    long long max_steps;
+    double P_low, P_high, rate, dfac, ltol, zmin;
+    long long max_csteps;
+};
 enum BPhase { BJ_DECOMP = 0, BJ_COMP = 1 };
 enum BPart { BP_PX = 0, BP_PY, BP_PE, BP_F2, BP_SXX, BP_SYY, BP_R2, BP_DMAX, BP_Z, BP_COUNT };
 
@@ -61,6 +70,7 @@ struct BSys {
     long long cstep, fire_total, contacts;
     double pe_final, phi_final;
     long long done_step;  // batched step in which the system finished
+    BJam J;
 };
 
 struct BDev {
@@ -74,17 +84,13 @@ struct BDev {
     int *nnum, *nbr;
     double* part;      // BP_COUNT arrays of nslot/64 chunk values
     int2* tiles;       // (system, first slot) of each build tile
-    int* flags;        // [0] capacity overflow, [1] largest row, [2] systems done,
-                       // [3] batched steps executed
+    int* flags;        // [0] capacity overflow, [1] largest row, [2] systems finished,
+                       // [3] batched steps executed, [4] build tiles in use
+    int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double e_c, n_c, ncm1, two_nc, inv2nc;
     int fastdiv, pad;
     double rc, rc2, thr2;
-    // FIRE and jamming parameters (dpm_jam_params)
-    double alpha0, dt0, pe_t, pe_diff_t, fmax2;
-    long long max_steps;
-    double P_low, P_high, rate, dfac, ltol, zmin;
-    long long max_csteps;
 };
 
 __device__ __forceinline__ double* bpart(const BDev& b, int which) {
@@ -111,10 +117,10 @@ __device__ __forceinline__ PairParams bparams(const BDev& b, const BSys& S) {
 }
 
 __device__ __forceinline__ void fire_reset(const BDev& b, BSys& S) {
-    S.alpha = b.alpha0;
-    S.dt = b.dt0;
-    S.dt_max = 10.0 * b.dt0;
-    S.dt_min = 0.001 * b.dt0;
+    S.alpha = S.J.alpha0;
+    S.dt = S.J.dt0;
+    S.dt_max = 10.0 * S.J.dt0;
+    S.dt_min = 0.001 * S.J.dt0;
     S.n_good = 0;
     S.n_bad = 0;
     S.step = 0;
@@ -130,6 +136,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_prep(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     const bool start = S.mode == BM_PROLOGUE;
     if (!start && S.snap_op == 0) return;  // wave-uniform: one system per wave
@@ -165,6 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     if (S.mode != BM_FIRE) return;
     const bool live = i - S.off < S.n;
@@ -216,6 +224,7 @@ __global__ void k_b_decide(BDev b) {
 // every other particle of the system with d^2 < rc^2, ascending index.  One block per
 // 256-slot tile of a system; the system's positions pass through LDS.
 __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
+    if ((int)blockIdx.x >= b.flags[4]) return;
     const int2 t = b.tiles[blockIdx.x];
     const BSys& S = b.sys[t.x];
     if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL) return;
@@ -264,6 +273,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
     const int kick = S.mode == BM_FIRE;
@@ -342,7 +352,7 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
     if (mode == BM_PROLOGUE) {
         S.last_avg_pe = avg;
         S.mode = BM_FIRE;
-        if (b.max_steps <= 0) {
+        if (S.J.max_steps <= 0) {
             S.done = 1;
             S.mode = BM_EVAL;
             return;
@@ -350,15 +360,15 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
     } else {
         double diff = fabs(avg / S.last_avg_pe - 1);
         S.last_avg_pe = avg;
-        bool fok = b.fmax2 > 0.0 && fm < b.fmax2;
+        bool fok = S.J.fmax2 > 0.0 && fm < S.J.fmax2;
         S.step += 1;
-        if (diff < b.pe_diff_t || avg < b.pe_t || fok) {
+        if (diff < S.J.pe_diff_t || avg < S.J.pe_t || fok) {
             S.converged = 1;
             S.done = 1;
             S.mode = BM_EVAL;
             return;
         }
-        if (S.step >= b.max_steps) {
+        if (S.step >= S.J.max_steps) {
             S.done = 1;
             S.mode = BM_EVAL;
             return;
@@ -385,7 +395,7 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
             return;
         }
         S.dt = fmax(S.dt * 0.5, S.dt_min);
-        S.alpha = b.alpha0;
+        S.alpha = S.J.alpha0;
         S.uphill = 1;
     }
 }
@@ -398,6 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     if (S.mode != BM_EVAL) return;
     const PairParams p = bparams(b, S);
@@ -468,16 +479,16 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
     for (int k = threadIdx.x; k < S.nc; k += WAVE) z += bpart(b, BP_Z)[c0 + k];
     z = wave_tree_sum(z);  // integer counts: exact in any order
     if (threadIdx.x != 0) return;
-    S.fire_total += min(S.step, b.max_steps);
+    S.fire_total += min(S.step, S.J.max_steps);
     const double pr = (sxx + syy) / 2;
-    const double P_low = b.P_low, P_high = b.P_high;
+    const double P_low = S.J.P_low, P_high = S.J.P_high;
     bool finished = false;
     if (S.jphase == BJ_DECOMP) {
         // for (it < 1000) { minimize; if (P < P_low) break; scaleBox(f); }
         bool leave = pr < P_low;
         int op = 1;
         if (!leave) {
-            scale_box_of(S, b.dfac);
+            scale_box_of(S, S.J.dfac);
             S.dec_it += 1;
             leave = S.dec_it >= 1000;
             op = 3;  // the snapshot below is taken after this scaling
@@ -492,7 +503,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
             S.snap_op = op;
             S.snapLx = S.Lx;
             S.snapLy = S.Ly;
-            if (S.cstep >= b.max_csteps) finished = true;
+            if (S.cstep >= S.J.max_csteps) finished = true;
             else start_minimization(b, S);
         } else {
             start_minimization(b, S);
@@ -514,7 +525,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
                 L = (S.L_high + S.L_low) / 2;
                 S.L_high = -1;
             } else {
-                L = L * b.rate;
+                L = L * S.J.rate;
             }
         } else {
             if (pr > P_high) {
@@ -522,12 +533,12 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
                 restore = 1;
                 L_prior = S.L_last;
                 L = (S.L_high + S.L_low) / 2;
-            } else if (z > b.zmin * (double)S.n) {
+            } else if (z > S.J.zmin * (double)S.n) {
                 S.status = 0;
                 stop = true;
             }
         }
-        if (!stop && S.L_low > 0 && S.L_high > 0 && fabs(S.L_low / S.L_high - 1) < b.ltol) {
+        if (!stop && S.L_low > 0 && S.L_high > 0 && fabs(S.L_low / S.L_high - 1) < S.J.ltol) {
             S.status = 2;
             stop = true;
         }
@@ -542,7 +553,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
         } else {
             scale_box_of(S, L / L_prior);
             S.cstep += 1;
-            if (S.cstep >= b.max_csteps) finished = true;
+            if (S.cstep >= S.J.max_csteps) finished = true;
             else start_minimization(b, S);
         }
     }
@@ -552,7 +563,8 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
         const double PI = 3.141592653589793238462643383279502884197;
         S.phi_final = r2 * PI / (S.Lx * S.Ly);
         S.mode = BM_DONE;
-        atomicAdd(&b.flags[2], 1);
+        int k = atomicAdd(&b.flags[2], 1);
+        b.done_list[k % b.nsys] = (int)blockIdx.x;
     }
 }
 
@@ -570,7 +582,8 @@ struct dpm_batch {
     std::vector<long long> off;
     std::vector<int> n;
     std::vector<dpm::BSys> hsys;
-    std::vector<double> rmax, rmin;
+    std::vector<double> rmax, rmin, box0;  // per system: radii range, box side as set
+    double4* P_init = nullptr;              // positions as set (autotuning restores them)
     double rc = 1.7, thr_sq = 3.0 * 0.15 * 0.15, e_c = 1.0, n_c = 2.0;
     int64_t steps_per_graph = 32;
     bool use_graph = true;
@@ -583,10 +596,22 @@ struct dpm_batch {
 
 namespace dpm {
 
+// Copies and fills are ordered on the batch's own stream (never the legacy stream, which
+// would tie them to graph captures of other threads' systems).
+static hipError_t bsync(dpm_batch* b, hipError_t e) {
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(b->stream);
+}
+#define BCOPY(dst, src, bytes, kind) \
+    DPM_HIP(bsync(b, hipMemcpyAsync((dst), (src), (bytes), (kind), b->stream)))
+
 static int balloc(dpm_batch* b, void** p, size_t bytes) {
     void* q = nullptr;
-    DPM_HIP(hipMalloc(&q, std::max<size_t>(bytes, 64)));
-    DPM_HIP(hipMemset(q, 0, std::max<size_t>(bytes, 64)));
+    {
+        std::lock_guard<std::mutex> g(capture_mutex());
+        DPM_HIP(hipMalloc(&q, std::max<size_t>(bytes, 64)));
+    }
+    DPM_HIP(bsync(b, hipMemsetAsync(q, 0, std::max<size_t>(bytes, 64), b->stream)));
     b->allocs.push_back(q);
     *p = q;
     return 0;
@@ -598,6 +623,63 @@ static int balloc_n(dpm_batch* b, T** p, int64_t n) {
     if (balloc(b, &q, sizeof(T) * (size_t)std::max<int64_t>(n, 1))) return -1;
     *p = (T*)q;
     return 0;
+}
+
+// Stream, arena and tables for up to nsys systems in nslot slots (sys_of = -1: free).
+static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_cap) {
+    (void)hipGetDevice(&b->device);
+    BDev& d = b->d;
+    d.nsys = nsys;
+    d.nslot = nslot;
+    d.ntiles = tiles_cap;
+    d.cap = 32;
+    DPM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    if (balloc_n(b, &d.sys, nsys) || balloc_n(b, &d.sys_of, nslot) || balloc_n(b, &d.P, nslot) ||
+        balloc_n(b, &d.snap, nslot) || balloc_n(b, &d.vel, nslot) || balloc_n(b, &d.frc, nslot) ||
+        balloc_n(b, &d.lref, nslot) || balloc_n(b, &d.pe, nslot) || balloc_n(b, &d.nnum, nslot) ||
+        balloc_n(b, &d.nbr, nslot * d.cap) || balloc_n(b, &d.part, (nslot / WAVE) * BP_COUNT) ||
+        balloc_n(b, &d.tiles, tiles_cap) || balloc_n(b, &d.flags, 8) ||
+        balloc_n(b, &d.done_list, nsys))
+        return -1;
+    DPM_HIP(bsync(b, hipMemsetAsync(d.sys_of, 0xff, sizeof(int) * nslot, b->stream)));  // -1: free
+    DPM_HIP(hipHostMalloc((void**)&b->hflags, 2 * 4 * sizeof(int), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k)
+        DPM_HIP(hipEventCreateWithFlags(&b->poll[k], hipEventDisableTiming));
+    return 0;
+}
+
+// The numbers every system shares, baked into the kernels' arguments.
+static void batch_common(dpm_batch* b, bool fastdiv) {
+    BDev& d = b->d;
+    d.e_c = b->e_c;
+    d.n_c = b->n_c;
+    d.ncm1 = b->n_c - 1.0;
+    d.two_nc = 2.0 * b->n_c;
+    int ex;
+    double m = frexp(d.two_nc, &ex);
+    d.inv2nc = (m == 0.5) ? 1.0 / d.two_nc : 0.0;
+    d.fastdiv = fastdiv ? 1 : 0;
+    d.rc = b->rc;
+    d.rc2 = b->rc * b->rc;
+    d.thr2 = b->thr_sq;
+}
+
+static BJam jam_constants(const dpm_jam_params* P) {
+    BJam J{};
+    J.alpha0 = P->fire.alpha_init;
+    J.dt0 = P->fire.dt;
+    J.pe_t = P->fire.avg_pe_target;
+    J.pe_diff_t = P->fire.avg_pe_diff_target;
+    J.fmax2 = P->fire.fmax_target > 0 ? P->fire.fmax_target * P->fire.fmax_target : 0.0;
+    J.max_steps = P->fire.max_steps;
+    J.P_low = P->pressure_target;
+    J.P_high = P->pressure_target * (1.0 + P->window_rel);
+    J.rate = 1.0 - P->strain_step;
+    J.dfac = P->decompress_factor;
+    J.ltol = P->length_tol;
+    J.zmin = P->min_contacts_per_particle;
+    J.max_csteps = P->max_compression_steps;
+    return J;
 }
 
 static void enqueue_batch_step(dpm_batch* b) {
@@ -622,6 +704,7 @@ static int ensure_batch_graph(dpm_batch* b, int64_t k) {
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
     }
+    std::lock_guard<std::mutex> lk(capture_mutex());
     DPM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int64_t s = 0; s < k; ++s) enqueue_batch_step(b);
     DPM_HIP(hipStreamEndCapture(b->stream, &b->graph));
@@ -651,6 +734,7 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
     b->n.resize(n_systems);
     b->rmax.assign(n_systems, 0.0);
     b->rmin.assign(n_systems, 0.0);
+    b->box0.assign(n_systems, 0.0);
     long long acc = 0;
     std::vector<int2> tiles;
     for (int64_t s = 0; s < n_systems; ++s) {
@@ -672,34 +756,19 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
     }
     b->nslot = acc;
     b->ntiles = (int64_t)tiles.size();
-    (void)hipGetDevice(&b->device);
+    bool ok = batch_alloc(b, n_systems, acc, b->ntiles) == 0;
     BDev& d = b->d;
-    d.nsys = n_systems;
-    d.nslot = acc;
-    d.ntiles = b->ntiles;
-    d.cap = 32;
-    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && !balloc_n(b, &d.sys, n_systems) && !balloc_n(b, &d.sys_of, acc) &&
-         !balloc_n(b, &d.P, acc) && !balloc_n(b, &d.snap, acc) && !balloc_n(b, &d.vel, acc) &&
-         !balloc_n(b, &d.frc, acc) && !balloc_n(b, &d.lref, acc) && !balloc_n(b, &d.pe, acc) &&
-         !balloc_n(b, &d.nnum, acc) && !balloc_n(b, &d.nbr, acc * d.cap) &&
-         !balloc_n(b, &d.part, (acc / WAVE) * BP_COUNT) && !balloc_n(b, &d.tiles, b->ntiles) &&
-         !balloc_n(b, &d.flags, 4);
     if (ok) {
         std::vector<int> so(acc);
         for (int64_t s = 0; s < n_systems; ++s) {
             long long end = s + 1 < n_systems ? b->off[s + 1] : acc;
             for (long long k = b->off[s]; k < end; ++k) so[k] = (int)s;
         }
-        ok = hipMemcpy(d.sys_of, so.data(), sizeof(int) * acc, hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMemcpy(d.tiles, tiles.data(), sizeof(int2) * tiles.size(),
-                       hipMemcpyHostToDevice) == hipSuccess;
+        ok = bsync(b, hipMemcpyAsync(d.sys_of, so.data(), sizeof(int) * acc,
+                                     hipMemcpyHostToDevice, b->stream)) == hipSuccess &&
+             bsync(b, hipMemcpyAsync(d.tiles, tiles.data(), sizeof(int2) * tiles.size(),
+                                     hipMemcpyHostToDevice, b->stream)) == hipSuccess;
     }
-    ok = ok && hipHostMalloc((void**)&b->hflags, 2 * 4 * sizeof(int), hipHostMallocDefault) ==
-                   hipSuccess;
-    for (int k = 0; ok && k < 2; ++k)
-        ok = hipEventCreateWithFlags(&b->poll[k], hipEventDisableTiming) == hipSuccess;
     b->hsys.assign(n_systems, BSys{});
     for (int64_t s = 0; s < n_systems; ++s) {
         b->hsys[s].off = b->off[s];
@@ -720,7 +789,10 @@ void dpm_batch_destroy(dpm_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void* p : b->allocs) (void)hipFree(p);
+    {
+        std::lock_guard<std::mutex> g(capture_mutex());
+        for (void* p : b->allocs) (void)hipFree(p);
+    }
     if (b->hflags) (void)hipHostFree(b->hflags);
     for (auto e : b->poll)
         if (e) (void)hipEventDestroy(e);
@@ -746,9 +818,10 @@ int dpm_batch_set_system(dpm_batch* b, int64_t s, const double* x, const double*
             return -1;
         }
     }
-    DPM_HIP(hipMemcpy(b->d.P + b->off[s], h.data(), sizeof(double4) * ns, hipMemcpyHostToDevice));
+    BCOPY(b->d.P + b->off[s], h.data(), sizeof(double4) * ns, hipMemcpyHostToDevice);
     b->rmax[s] = rm;
     b->rmin[s] = rn;
+    b->box0[s] = box_side;
     b->hsys[s].Lx = b->hsys[s].Ly = box_side;
     b->hsys[s].rmax = rm;
     return 0;
@@ -762,14 +835,14 @@ int dpm_batch_get_system(dpm_batch* b, int64_t s, double* x, double* y, double* 
     DPM_HIP(hipStreamSynchronize(b->stream));
     int ns = b->n[s];
     std::vector<double4> h(ns);
-    DPM_HIP(hipMemcpy(h.data(), b->d.P + b->off[s], sizeof(double4) * ns, hipMemcpyDeviceToHost));
+    BCOPY(h.data(), b->d.P + b->off[s], sizeof(double4) * ns, hipMemcpyDeviceToHost);
     for (int k = 0; k < ns; ++k) {
         if (x) x[k] = h[k].x;
         if (y) y[k] = h[k].y;
     }
     if (box_side) {
         BSys S;
-        DPM_HIP(hipMemcpy(&S, b->d.sys + s, sizeof(BSys), hipMemcpyDeviceToHost));
+        BCOPY(&S, b->d.sys + s, sizeof(BSys), hipMemcpyDeviceToHost);
         *box_side = S.Lx;
     }
     return 0;
@@ -812,15 +885,6 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     }
     BDev& d = b->d;
     // common parameters (pair_params's derived numbers, dpm_jam's loop constants)
-    d.e_c = b->e_c;
-    d.n_c = b->n_c;
-    d.ncm1 = b->n_c - 1.0;
-    d.two_nc = 2.0 * b->n_c;
-    {
-        int ex;
-        double m = frexp(d.two_nc, &ex);
-        d.inv2nc = (m == 0.5) ? 1.0 / d.two_nc : 0.0;
-    }
     double rmin_all = 1e300, rmax_all = 0.0;
     for (int64_t s = 0; s < b->nsys; ++s) {
         if (b->rmin[s] <= 0) {
@@ -831,24 +895,9 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
         rmax_all = std::max(rmax_all, b->rmax[s]);
     }
     const double lo = 0x1p-100, hi = 0x1p100;
-    d.fastdiv = (b->n_c == 2.0 && b->e_c >= lo && b->e_c <= hi && rmin_all >= lo && rmax_all <= hi)
-                    ? 1 : 0;
-    d.rc = b->rc;
-    d.rc2 = b->rc * b->rc;
-    d.thr2 = b->thr_sq;
-    d.alpha0 = P->fire.alpha_init;
-    d.dt0 = P->fire.dt;
-    d.pe_t = P->fire.avg_pe_target;
-    d.pe_diff_t = P->fire.avg_pe_diff_target;
-    d.fmax2 = P->fire.fmax_target > 0 ? P->fire.fmax_target * P->fire.fmax_target : 0.0;
-    d.max_steps = P->fire.max_steps;
-    d.P_low = P->pressure_target;
-    d.P_high = P->pressure_target * (1.0 + P->window_rel);
-    d.rate = 1.0 - P->strain_step;
-    d.dfac = P->decompress_factor;
-    d.ltol = P->length_tol;
-    d.zmin = P->min_contacts_per_particle;
-    d.max_csteps = P->max_compression_steps;
+    batch_common(b, b->n_c == 2.0 && b->e_c >= lo && b->e_c <= hi && rmin_all >= lo &&
+                        rmax_all <= hi);
+    const BJam J = jam_constants(P);
     // every system starts minimizeFire (decompression phase) with a fresh list
     for (int64_t s = 0; s < b->nsys; ++s) {
         BSys& S = b->hsys[s];
@@ -856,13 +905,14 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
         f.off = S.off;
         f.n = S.n;
         f.nc = S.nc;
-        f.Lx = S.Lx;
-        f.Ly = S.Ly;
+        f.Lx = b->box0[s];
+        f.Ly = b->box0[s];
         f.rmax = b->rmax[s];
-        f.alpha = d.alpha0;
-        f.dt = d.dt0;
-        f.dt_max = 10.0 * d.dt0;
-        f.dt_min = 0.001 * d.dt0;
+        f.J = J;
+        f.alpha = J.alpha0;
+        f.dt = J.dt0;
+        f.dt_max = 10.0 * J.dt0;
+        f.dt_min = 0.001 * J.dt0;
         f.mode = BM_PROLOGUE;
         f.rebuild = 1;
         f.jphase = BJ_DECOMP;
@@ -872,7 +922,11 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     }
     DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
                            b->stream));
-    DPM_HIP(hipMemsetAsync(d.flags, 0, 4 * sizeof(int), b->stream));
+    DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
+    {
+        int nt = (int)b->ntiles;
+        DPM_HIP(hipMemcpyAsync(d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    }
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (b->exec) {  // the parameters are baked into the graph's kernel arguments
         (void)hipGraphExecDestroy(b->exec);
@@ -917,7 +971,7 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     }
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (steps_done) *steps_done = finished_steps;
-    DPM_HIP(hipMemcpy(b->hsys.data(), d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost));
+    BCOPY(b->hsys.data(), d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost);
     for (int64_t s = 0; s < b->nsys; ++s) {
         const BSys& S = b->hsys[s];
         dpm_jam_result& r = out[s];
@@ -933,15 +987,569 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     return 0;
 }
 
+// Autotuner: runs `trial_steps` batched steps of the jam from the configured state for
+// every (cutoff, steps per launch) pair, restores the state after each trial, and keeps
+// the fastest pair.  Neither choice changes any result bit (rows only add listed
+// non-contacts, which contribute +0.0; launch grouping is not visible to the systems).
+int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_steps,
+                       const double* cutoffs, int n_cutoffs, const int64_t* chunks,
+                       int n_chunks, double* best_cutoff, int64_t* best_chunk,
+                       double* best_ms_per_step) {
+    if (!P || trial_steps <= 0 || n_cutoffs <= 0 || n_chunks <= 0 || !cutoffs || !chunks) {
+        set_error("dpm_batch_autotune: bad arguments");
+        return -1;
+    }
+    if (!b->P_init && balloc_n(b, &b->P_init, b->nslot)) return -1;
+    DPM_HIP(hipMemcpyAsync(b->P_init, b->d.P, sizeof(double4) * b->nslot,
+                           hipMemcpyDeviceToDevice, b->stream));
+    const double rc0 = b->rc, thr0 = b->thr_sq;
+    const int64_t ch0 = b->steps_per_graph;
+    double best = 1e300, bc = rc0;
+    int64_t bk = ch0;
+    std::vector<dpm_jam_result> tmp(b->nsys);
+    for (int i = 0; i < n_cutoffs; ++i) {
+        for (int j = 0; j < n_chunks; ++j) {
+            if (!(cutoffs[i] > 0) || chunks[j] < 1 || chunks[j] > 1024) {
+                set_error("dpm_batch_autotune: bad candidate");
+                return -1;
+            }
+            // rebuild when a particle moved half the skin (the threshold keeps its form)
+            double skin = cutoffs[i] - 2.0 * *std::max_element(b->rmax.begin(), b->rmax.end());
+            if (!(skin > 0)) continue;
+            b->rc = cutoffs[i];
+            b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
+            b->steps_per_graph = chunks[j];
+            if (b->exec) {
+                (void)hipGraphExecDestroy(b->exec);
+                (void)hipGraphDestroy(b->graph);
+                b->exec = nullptr;
+            }
+            DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
+                                   hipMemcpyDeviceToDevice, b->stream));
+            // one untimed launch unit first (graph capture and instantiation)
+            int64_t done = 0;
+            if (dpm_batch_jam(b, P, chunks[j], tmp.data(), &done)) return -1;
+            DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
+                                   hipMemcpyDeviceToDevice, b->stream));
+            DPM_HIP(hipStreamSynchronize(b->stream));
+            auto t0 = std::chrono::steady_clock::now();
+            if (dpm_batch_jam(b, P, trial_steps, tmp.data(), &done)) return -1;
+            double ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count() /
+                        (double)std::max<int64_t>(done, 1);
+            if (ms < best) {
+                best = ms;
+                bc = cutoffs[i];
+                bk = chunks[j];
+            }
+        }
+    }
+    if (best == 1e300) {
+        b->rc = rc0;
+        b->thr_sq = thr0;
+        b->steps_per_graph = ch0;
+        set_error("dpm_batch_autotune: no cutoff exceeds the largest contact distance");
+        return -1;
+    }
+    double skin = bc - 2.0 * *std::max_element(b->rmax.begin(), b->rmax.end());
+    b->rc = bc;
+    b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
+    b->steps_per_graph = bk;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+    }
+    DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
+                           hipMemcpyDeviceToDevice, b->stream));
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    if (best_cutoff) *best_cutoff = bc;
+    if (best_chunk) *best_chunk = bk;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return 0;
+}
+
 int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out) {
     if (!steps_out) {
         set_error("dpm_batch_finish_steps: null output");
         return -1;
     }
     DPM_HIP(hipStreamSynchronize(b->stream));
-    DPM_HIP(hipMemcpy(b->hsys.data(), b->d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost));
+    BCOPY(b->hsys.data(), b->d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost);
     for (int64_t s = 0; s < b->nsys; ++s)
         steps_out[s] = b->hsys[s].mode == BM_DONE ? b->hsys[s].done_step : -1;
+    return 0;
+}
+
+}  // extern "C"
+
+// =========================================================================
+//                 serving loop (continuous batching of jam requests)
+// =========================================================================
+// Requests (disk or rigid bumpy systems, each with its own jamming parameters) are
+// queued on submission and admitted while the device budget allows: disk systems take
+// a free region of the batched engine's arena and a system entry and join the running
+// batch at the next launch; rigid bumpy systems get a dpm_system of their own, jammed
+// by a worker thread on its own stream, overlapping the batch on the device.  A
+// finished disk system leaves the arena at once, so its space and entry serve the next
+// request.  Every result is the one the request's system gives when jammed alone.
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+struct dpm_server_req {
+    int64_t id = 0;
+    int kind = DPM_DISK;
+    int64_t n = 0, nv = 0;
+    std::vector<double> x, y, r, m, ang;
+    std::vector<int64_t> vcounts;
+    double rv = 0.0, side = 0.0, e_c = 1.0;
+    dpm_jam_params jam{};
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    std::chrono::steady_clock::time_point t_submit, t_done;
+    dpm_jam_result res{};
+    std::vector<double> rx, ry;
+    double rside = 0.0;
+    std::string err;
+    int sys = -1;
+    long long off = -1, slots = 0;
+    int64_t bytes = 0;
+    std::vector<double4> staged;
+};
+
+struct dpm_server {
+    dpm_batch* b = nullptr;
+    int64_t budget = 0, used = 0;
+    int workers = 1, running_bumpy = 0;
+    std::mutex mu;
+    std::map<int64_t, dpm_server_req> reqs;
+    std::deque<int64_t> queue;
+    std::vector<int> free_sys;
+    std::map<long long, long long> free_ranges;  // first slot -> length
+    std::vector<int64_t> sys_req;                // system entry -> request id (0: free)
+    int64_t next_id = 1, seen_done = 0, finished = 0, disk_active = 0;
+    std::vector<std::thread> threads;
+    std::vector<int2> tiles_host;
+    bool fastdiv = true;
+};
+
+namespace dpm {
+
+constexpr int64_t SLOT_BYTES = 32 + 32 + 16 + 16 + 16 + 8 + 4 + 4 + 32 * 4 + 8;
+
+static int64_t bumpy_bytes(int64_t n, int64_t nv) { return 1024 * n + 640 * nv; }
+
+static void serve_upload_tiles(dpm_server* sv) {
+    dpm_batch* b = sv->b;
+    sv->tiles_host.clear();
+    for (size_t s = 0; s < sv->sys_req.size(); ++s) {
+        if (!sv->sys_req[s]) continue;
+        const dpm_server_req& q = sv->reqs.at(sv->sys_req[s]);
+        for (int64_t t = 0; t < q.n; t += BLOCK)
+            sv->tiles_host.push_back(make_int2((int)s, (int)(q.off + t)));
+    }
+    int nt = (int)sv->tiles_host.size();
+    if (nt)
+        (void)bsync(b, hipMemcpyAsync(b->d.tiles, sv->tiles_host.data(), sizeof(int2) * nt,
+                                      hipMemcpyHostToDevice, b->stream));
+    (void)bsync(b, hipMemcpyAsync(b->d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice,
+                                  b->stream));
+}
+
+static bool take_range(dpm_server* sv, long long slots, long long* off) {
+    for (auto it = sv->free_ranges.begin(); it != sv->free_ranges.end(); ++it) {
+        if (it->second < slots) continue;
+        *off = it->first;
+        long long rest = it->second - slots, start = it->first + slots;
+        sv->free_ranges.erase(it);
+        if (rest) sv->free_ranges[start] = rest;
+        return true;
+    }
+    return false;
+}
+
+static void give_range(dpm_server* sv, long long off, long long slots) {
+    auto it = sv->free_ranges.emplace(off, slots).first;
+    auto nx = std::next(it);
+    if (nx != sv->free_ranges.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        sv->free_ranges.erase(nx);
+    }
+    if (it != sv->free_ranges.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            sv->free_ranges.erase(it);
+        }
+    }
+}
+
+// A rigid bumpy request, jammed alone on its own dpm_system (same set-up as a caller
+// would make: cell list with cutoff 1.5 d_max, vertex cutoff 3 r_v, threshold (0.2 r_v)^2,
+// the rules of RigidBumpy::setNeighborSize, rigid_bumpy.cu:1278-1300).
+static void bumpy_worker(dpm_server* sv, int64_t id) {
+    dpm_server_req* q;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        q = &sv->reqs.at(id);
+    }
+    auto fail = [&](const char* where) {
+        std::lock_guard<std::mutex> g(sv->mu);
+        q->err = std::string(where) + ": " + dpm_last_error();
+        q->state = 3;
+        q->t_done = std::chrono::steady_clock::now();
+        sv->used -= q->bytes;
+        sv->running_bumpy--;
+        sv->finished++;
+    };
+    (void)hipSetDevice(sv->b->device);
+    dpm_system* s = dpm_create(DPM_BUMPY, q->n, q->nv);
+    if (!s) return fail("dpm_create");
+    double dmax = 0.0;
+    for (double r : q->r) dmax = std::max(dmax, 2.0 * r);
+    double rc = 1.5 * dmax;
+    int64_t ncd = std::max<int64_t>(3, (int64_t)std::floor(q->side / rc));
+    double cs = q->side / ncd;
+    bool ok = !dpm_set_box(s, q->side, q->side) && !dpm_set_energy(s, q->e_c, 2.0) &&
+              !dpm_upload(s, DPM_F_X, q->x.data(), q->n) &&
+              !dpm_upload(s, DPM_F_Y, q->y.data(), q->n) &&
+              !dpm_upload(s, DPM_F_RAD, q->r.data(), q->n) &&
+              !dpm_upload(s, DPM_F_MASS, q->m.data(), q->n) &&
+              !dpm_upload(s, DPM_F_ANGLE, q->ang.data(), q->n) &&
+              !dpm_bumpy_setup(s, q->vcounts.data(), q->rv, 1) &&
+              !dpm_neighbor_config(s, DPM_NBR_CELL, rc, (0.2 * q->rv) * (0.2 * q->rv), ncd,
+                                   (0.5 * cs) * (0.5 * cs)) &&
+              !dpm_bumpy_neighbor_config(s, 1.5 * 2.0 * q->rv, dmax) && !dpm_neighbor_init(s);
+    dpm_jam_result r{};
+    ok = ok && !dpm_jam(s, &q->jam, &r);
+    std::vector<double> rx(q->n), ry(q->n);
+    double lx = 0, ly = 0;
+    ok = ok && !dpm_download(s, DPM_F_X, rx.data(), q->n) &&
+         !dpm_download(s, DPM_F_Y, ry.data(), q->n) && !dpm_get_box(s, &lx, &ly);
+    if (!ok) {
+        dpm_destroy(s);
+        return fail("rigid bumpy jam");
+    }
+    dpm_destroy(s);
+    std::lock_guard<std::mutex> g(sv->mu);
+    q->res = r;
+    q->rx = std::move(rx);
+    q->ry = std::move(ry);
+    q->rside = lx;
+    q->state = 2;
+    q->t_done = std::chrono::steady_clock::now();
+    sv->used -= q->bytes;
+    sv->running_bumpy--;
+    sv->finished++;
+}
+
+// Disk systems that finished since the last look leave the arena.  The stream is idle.
+static int serve_collect(dpm_server* sv) {
+    dpm_batch* b = sv->b;
+    int fl[8];
+    BCOPY(fl, b->d.flags, sizeof(fl), hipMemcpyDeviceToHost);
+    if (fl[0]) {
+        set_error("dpm_server: a neighbour row exceeded the row capacity");
+        return -1;
+    }
+    int64_t total = fl[2];
+    if (total == sv->seen_done) return 0;
+    std::vector<int> ids(b->nsys);
+    BCOPY(ids.data(), b->d.done_list, sizeof(int) * b->nsys, hipMemcpyDeviceToHost);
+    bool changed = false;
+    for (int64_t k = sv->seen_done; k < total; ++k) {
+        int s = ids[k % b->nsys];
+        int64_t id = sv->sys_req[s];
+        if (!id) continue;
+        dpm_server_req& q = sv->reqs.at(id);
+        BSys S;
+        BCOPY(&S, b->d.sys + s, sizeof(BSys), hipMemcpyDeviceToHost);
+        std::vector<double4> h(q.n);
+        BCOPY(h.data(), b->d.P + q.off, sizeof(double4) * q.n, hipMemcpyDeviceToHost);
+        q.rx.resize(q.n);
+        q.ry.resize(q.n);
+        for (int64_t i = 0; i < q.n; ++i) {
+            q.rx[i] = h[i].x;
+            q.ry[i] = h[i].y;
+        }
+        q.res.compression_steps = S.cstep;
+        q.res.fire_steps_total = S.fire_total;
+        q.res.status = S.status;
+        q.res.pressure = S.pr;
+        q.res.packing_fraction = S.phi_final;
+        q.res.pe = S.pe_final;
+        q.res.box_length = S.Lx;
+        q.res.contacts = S.contacts;
+        q.rside = S.Lx;
+        q.state = 2;
+        q.t_done = std::chrono::steady_clock::now();
+        // the system entry and its slots become free
+        S.mode = BM_EMPTY;
+        BCOPY(b->d.sys + s, &S, sizeof(BSys), hipMemcpyHostToDevice);
+        std::vector<int> neg(q.slots, -1);
+        BCOPY(b->d.sys_of + q.off, neg.data(), sizeof(int) * q.slots, hipMemcpyHostToDevice);
+        give_range(sv, q.off, q.slots);
+        sv->free_sys.push_back(s);
+        sv->sys_req[s] = 0;
+        sv->used -= q.bytes;
+        sv->finished++;
+        sv->disk_active--;
+        changed = true;
+    }
+    sv->seen_done = total;
+    if (changed) serve_upload_tiles(sv);
+    return 0;
+}
+
+// Queued requests that fit join: disk systems into the arena, bumpy ones to a worker.
+static int serve_admit(dpm_server* sv) {
+    dpm_batch* b = sv->b;
+    bool tiles_changed = false;
+    for (auto it = sv->queue.begin(); it != sv->queue.end();) {
+        dpm_server_req& q = sv->reqs.at(*it);
+        if (sv->used + q.bytes > sv->budget) {
+            ++it;
+            continue;
+        }
+        if (q.kind == DPM_BUMPY) {
+            if (sv->running_bumpy >= sv->workers) {
+                ++it;
+                continue;
+            }
+            q.state = 1;
+            sv->used += q.bytes;
+            sv->running_bumpy++;
+            sv->threads.emplace_back(bumpy_worker, sv, q.id);
+            it = sv->queue.erase(it);
+            continue;
+        }
+        long long off;
+        if (sv->free_sys.empty() || !take_range(sv, q.slots, &off)) {
+            ++it;
+            continue;
+        }
+        int s = sv->free_sys.back();
+        sv->free_sys.pop_back();
+        q.sys = s;
+        q.off = off;
+        q.state = 1;
+        sv->used += q.bytes;
+        sv->sys_req[s] = q.id;
+        sv->disk_active++;
+        // particles, slot ownership and the system entry (minimizeFire, decompression)
+        BCOPY(b->d.P + off, q.staged.data(), sizeof(double4) * q.n, hipMemcpyHostToDevice);
+        std::vector<int> own(q.slots, s);
+        BCOPY(b->d.sys_of + off, own.data(), sizeof(int) * q.slots, hipMemcpyHostToDevice);
+        BSys f{};
+        f.off = off;
+        f.n = (int)q.n;
+        f.nc = (int)((q.n + WAVE - 1) / WAVE);
+        f.Lx = f.Ly = q.side;
+        double rm = 0.0;
+        for (double r : q.r) rm = std::max(rm, r);
+        f.rmax = rm;
+        f.J = jam_constants(&q.jam);
+        f.alpha = f.J.alpha0;
+        f.dt = f.J.dt0;
+        f.dt_max = 10.0 * f.J.dt0;
+        f.dt_min = 0.001 * f.J.dt0;
+        f.mode = BM_PROLOGUE;
+        f.rebuild = 1;
+        f.jphase = BJ_DECOMP;
+        f.status = 1;
+        f.L_low = f.L_high = f.L_last = -1;
+        BCOPY(b->d.sys + s, &f, sizeof(BSys), hipMemcpyHostToDevice);
+        tiles_changed = true;
+        it = sv->queue.erase(it);
+    }
+    if (tiles_changed) serve_upload_tiles(sv);
+    return 0;
+}
+
+}  // namespace dpm
+
+extern "C" {
+
+dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t budget_bytes,
+                              int32_t bumpy_workers) {
+    if (max_systems <= 0 || max_systems >= (int64_t)1 << 24 || arena_slots < WAVE ||
+        arena_slots >= (int64_t)1 << 31 || budget_bytes <= 0 || bumpy_workers < 0 ||
+        bumpy_workers > 64) {
+        set_error("dpm_server_create: bad arguments");
+        return nullptr;
+    }
+    if (dpm_device_count() <= 0) {
+        set_error("dpm_server_create: no HIP device");
+        return nullptr;
+    }
+    auto* sv = new dpm_server();
+    sv->b = new dpm_batch();
+    int64_t slots = arena_slots / WAVE * WAVE;
+    dpm_batch* b = sv->b;
+    b->nsys = max_systems;
+    b->nslot = slots;
+    b->ntiles = slots / WAVE;
+    if (batch_alloc(b, max_systems, slots, b->ntiles)) {
+        std::string e = dpm_last_error();
+        dpm_batch_destroy(b);
+        delete sv;
+        set_error("dpm_server_create: " + e);
+        return nullptr;
+    }
+    std::vector<BSys> empty(max_systems);
+    for (auto& S : empty) S.mode = BM_EMPTY;
+    if (bsync(b, hipMemcpyAsync(b->d.sys, empty.data(), sizeof(BSys) * max_systems,
+                                hipMemcpyHostToDevice, b->stream)) != hipSuccess) {
+        dpm_batch_destroy(b);
+        delete sv;
+        set_error("dpm_server_create: device copy failed");
+        return nullptr;
+    }
+    sv->budget = budget_bytes;
+    sv->workers = bumpy_workers;
+    sv->sys_req.assign(max_systems, 0);
+    for (int64_t s = max_systems - 1; s >= 0; --s) sv->free_sys.push_back((int)s);
+    sv->free_ranges[0] = slots;
+    batch_common(b, true);
+    if (b->use_graph && ensure_batch_graph(b, b->steps_per_graph)) {
+        std::string e = dpm_last_error();
+        dpm_batch_destroy(b);
+        delete sv;
+        set_error("dpm_server_create: " + e);
+        return nullptr;
+    }
+    return sv;
+}
+
+void dpm_server_destroy(dpm_server* sv) {
+    if (!sv) return;
+    for (auto& t : sv->threads)
+        if (t.joinable()) t.join();
+    dpm_batch_destroy(sv->b);
+    delete sv;
+}
+
+int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) {
+    if (!r || r->n_particles <= 0 || r->n_particles >= (int64_t)1 << 24 || !r->x || !r->y ||
+        !r->radius || !r->mass || !(r->box_side > 0) ||
+        (r->kind != DPM_DISK && r->kind != DPM_BUMPY)) {
+        set_error("dpm_server_submit: bad request");
+        return -1;
+    }
+    dpm_server_req q;
+    q.kind = r->kind;
+    q.n = r->n_particles;
+    q.x.assign(r->x, r->x + q.n);
+    q.y.assign(r->y, r->y + q.n);
+    q.r.assign(r->radius, r->radius + q.n);
+    q.m.assign(r->mass, r->mass + q.n);
+    q.side = r->box_side;
+    q.e_c = r->e_c;
+    q.jam = r->jam;
+    for (int64_t i = 0; i < q.n; ++i)
+        if (!(q.r[i] >= 0x1p-100 && q.r[i] <= 0x1p100) || !(q.m[i] > 0)) {
+            set_error("dpm_server_submit: radii must lie in [2^-100, 2^100], masses > 0");
+            return -1;
+        }
+    if (q.kind == DPM_DISK) {
+        if (q.e_c != sv->b->e_c) {
+            set_error("dpm_server_submit: disk requests share the server's contact energy");
+            return -1;
+        }
+        q.slots = (q.n + WAVE - 1) / WAVE * WAVE;
+        q.bytes = q.slots * SLOT_BYTES;
+        if (q.slots > sv->b->nslot) {
+            set_error("dpm_server_submit: system larger than the arena");
+            return -1;
+        }
+        q.staged.resize(q.n);
+        for (int64_t i = 0; i < q.n; ++i)
+            q.staged[i] = make_double4(q.x[i], q.y[i], q.r[i], q.m[i]);
+    } else {
+        if (!r->angle || !r->vertices_per_particle || !(r->vertex_radius > 0) ||
+            r->n_vertices <= 0) {
+            set_error("dpm_server_submit: rigid bumpy requests need angles, vertex counts and "
+                      "a vertex radius");
+            return -1;
+        }
+        if (sv->workers == 0) {
+            set_error("dpm_server_submit: this server takes no rigid bumpy requests");
+            return -1;
+        }
+        q.nv = r->n_vertices;
+        q.ang.assign(r->angle, r->angle + q.n);
+        q.vcounts.assign(r->vertices_per_particle, r->vertices_per_particle + q.n);
+        q.rv = r->vertex_radius;
+        q.bytes = bumpy_bytes(q.n, q.nv);
+    }
+    if (q.bytes > sv->budget) {
+        set_error("dpm_server_submit: request exceeds the device budget");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(sv->mu);
+    q.id = sv->next_id++;
+    q.t_submit = std::chrono::steady_clock::now();
+    int64_t id = q.id;
+    sv->reqs.emplace(id, std::move(q));
+    sv->queue.push_back(id);
+    return id;
+}
+
+int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished_total,
+                        int64_t* in_flight, int64_t* queued) {
+    dpm_batch* b = sv->b;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        if (serve_collect(sv) || serve_admit(sv)) return -1;
+    }
+    if (sv->disk_active > 0 && batched_steps > 0) {
+        int64_t done = 0;
+        while (done < batched_steps) {
+            int64_t k = std::min<int64_t>(b->steps_per_graph, batched_steps - done);
+            if (b->use_graph && k == b->steps_per_graph) {
+                if (ensure_batch_graph(b, k)) return -1;
+                DPM_HIP(hipGraphLaunch(b->exec, b->stream));
+            } else {
+                for (int64_t s = 0; s < k; ++s) enqueue_batch_step(b);
+                if (check_launch()) return -1;
+            }
+            done += k;
+        }
+        DPM_HIP(hipStreamSynchronize(b->stream));
+        std::lock_guard<std::mutex> g(sv->mu);
+        if (serve_collect(sv) || serve_admit(sv)) return -1;
+    } else if (sv->running_bumpy > 0) {
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (finished_total) *finished_total = sv->finished;
+    if (in_flight) *in_flight = sv->disk_active + sv->running_bumpy;
+    if (queued) *queued = (int64_t)sv->queue.size();
+    return 0;
+}
+
+int dpm_server_result(dpm_server* sv, int64_t id, int32_t* state, dpm_jam_result* out,
+                      double* latency_s, double* x, double* y, double* box_side) {
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) {
+        set_error("dpm_server_result: unknown request");
+        return -1;
+    }
+    const dpm_server_req& q = it->second;
+    if (state) *state = q.state;
+    if (q.state == 3) {
+        set_error("dpm_server_result: request failed: " + q.err);
+        return -1;
+    }
+    if (q.state != 2) return 0;
+    if (out) *out = q.res;
+    if (latency_s)
+        *latency_s = std::chrono::duration<double>(q.t_done - q.t_submit).count();
+    if (x) std::copy(q.rx.begin(), q.rx.end(), x);
+    if (y) std::copy(q.ry.begin(), q.ry.end(), y);
+    if (box_side) *box_side = q.rside;
     return 0;
 }
 

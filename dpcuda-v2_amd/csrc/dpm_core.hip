@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <type_traits>
 
 #include "dpm_kernels.hpp"
@@ -1196,10 +1197,24 @@ __global__ void k_fire_ctl(Dev d, int init, int pos, int gate) {
 
 namespace dpm {
 
+// Graph captures, device allocations and frees of all systems are serialised: several
+// systems may be driven from several host threads at once, and an allocation or free
+// must not fall inside another thread's capture.
+std::mutex& capture_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+// Fills and copies go through the system's own stream (never the legacy stream, which
+// would tie them to graph captures running for other systems in other threads).
 int dalloc_bytes(dpm_system* s, void** p, size_t bytes) {
     void* q = nullptr;
-    DPM_HIP(hipMalloc(&q, bytes));
-    DPM_HIP(hipMemset(q, 0, bytes));
+    {
+        std::lock_guard<std::mutex> g(capture_mutex());
+        DPM_HIP(hipMalloc(&q, bytes));
+    }
+    DPM_HIP(hipMemsetAsync(q, 0, bytes, s->stream));
+    DPM_HIP(hipStreamSynchronize(s->stream));
     s->allocs.push_back(q);
     *p = q;
     return 0;
@@ -1209,6 +1224,7 @@ void dfree(dpm_system* s, void* p) {
     if (!p) return;
     auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
     if (it != s->allocs.end()) s->allocs.erase(it);
+    std::lock_guard<std::mutex> g(capture_mutex());
     (void)hipFree(p);
 }
 
@@ -1521,6 +1537,7 @@ static int ensure_graph(dpm_system* s, std::map<int64_t, Graph>& cache, int64_t 
         (void)hipGraphDestroy(g.graph);
         g.exec = nullptr;
     }
+    std::lock_guard<std::mutex> lk(capture_mutex());
     DPM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     fill(s, dt, steps);
     DPM_HIP(hipStreamEndCapture(s->stream, &g.graph));
@@ -1915,10 +1932,10 @@ dpm_system* dpm_create(int kind, int64_t n, int64_t nv) {
     if (ok) {
         std::vector<int> id(n);
         for (int64_t k = 0; k < n; ++k) id[k] = (int)k;
-        ok = hipMemcpy(d.sidx, id.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess;
+        ok = scopy(s, d.sidx, id.data(), sizeof(int) * n, hipMemcpyHostToDevice) == 0;
         Ctl c{};
         c.epoch = 1;
-        ok = ok && hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && scopy(s, d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) == 0;
     }
     ok = ok && hipHostMalloc((void**)&s->hctl, 4 * sizeof(Ctl), hipHostMallocDefault) == hipSuccess;
     for (int k = 0; ok && k < 4; ++k)
@@ -1942,8 +1959,11 @@ void dpm_destroy(dpm_system* s) {
                 (void)hipGraphExecDestroy(kv.second.exec);
                 (void)hipGraphDestroy(kv.second.graph);
             }
-    for (void* p : s->allocs) (void)hipFree(p);
-    for (void* p : s->snap_allocs) (void)hipFree(p);
+    {
+        std::lock_guard<std::mutex> g(capture_mutex());
+        for (void* p : s->allocs) (void)hipFree(p);
+        for (void* p : s->snap_allocs) (void)hipFree(p);
+    }
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     for (auto e : s->poll_ev)
         if (e) (void)hipEventDestroy(e);

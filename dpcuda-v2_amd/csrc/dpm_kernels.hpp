@@ -1,5 +1,6 @@
 // dpm_kernels.hpp -- launch helpers shared between the library's .hip sources.
 #pragma once
+#include <mutex>
 #include <vector>
 
 #include "dpm_internal.hpp"
@@ -122,6 +123,7 @@ __device__ __forceinline__ void mix_one(double2& v, double2 f, double alpha) {
 }
 
 // ---- host helpers (dpm_core.hip) ----
+std::mutex& capture_mutex();
 int dalloc_bytes(dpm_system* s, void** p, size_t bytes);
 template <class T>
 int dalloc(dpm_system* s, T** p, int64_t n) {

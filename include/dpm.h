@@ -215,6 +215,45 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* p, int64_t max_batched_ste
 /* the batched step (1-based) in which each system finished its last dpm_batch_jam, -1 if
  * it has not */
 int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out);
+/* Autotuner: times trial_steps batched steps of dpm_batch_jam for every pair of a Verlet
+ * cutoff (rebuild threshold: half its skin) and a number of steps per graph launch,
+ * restoring the configured state after each trial, and keeps the fastest pair.  Neither
+ * choice changes a result bit. */
+int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* p, int64_t trial_steps,
+                       const double* cutoffs, int n_cutoffs, const int64_t* chunks,
+                       int n_chunks, double* best_cutoff, int64_t* best_chunk,
+                       double* best_ms_per_step);
+
+/* ---- serving loop (no reference counterpart) ---------------------------------------------
+ * Continuous batching of jamming requests.  dpm_server_submit queues a request (arrays are
+ * copied); dpm_server_progress admits queued requests while the device budget allows (disk
+ * systems join the running batch of the batched engine, whose arena holds arena_slots
+ * particle slots and max_systems systems; rigid bumpy systems run on worker threads, each
+ * on its own stream), advances the batch by batched_steps and collects finished requests.
+ * Disk requests share the server's pair law (e_c = 1, n_c = 2, Verlet cutoff 1.7, rebuild at
+ * 3 max d^2 > 3 (0.15)^2); each result equals dpm_jam on the request's system alone.
+ * dpm_server_result: state 0 queued, 1 running, 2 done (out, latency from submission to
+ * completion, final positions and box side are filled), 3 failed. */
+typedef struct dpm_server dpm_server;
+typedef struct {
+    int32_t kind;             /* DPM_DISK or DPM_BUMPY */
+    int32_t pad;
+    int64_t n_particles, n_vertices;
+    const double *x, *y, *radius, *mass;
+    const double* angle;                  /* rigid bumpy: orientation per particle */
+    const int64_t* vertices_per_particle; /* rigid bumpy */
+    double vertex_radius;                 /* rigid bumpy */
+    double box_side, e_c;
+    dpm_jam_params jam;
+} dpm_request;
+dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t budget_bytes,
+                              int32_t bumpy_workers);
+void dpm_server_destroy(dpm_server* s);
+int64_t dpm_server_submit(dpm_server* s, const dpm_request* r);
+int dpm_server_progress(dpm_server* s, int64_t batched_steps, int64_t* finished_total,
+                        int64_t* in_flight, int64_t* queued);
+int dpm_server_result(dpm_server* s, int64_t id, int32_t* state, dpm_jam_result* out,
+                      double* latency_s, double* x, double* y, double* box_side);
 
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).

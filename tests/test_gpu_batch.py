@@ -90,3 +90,21 @@ def test_bad_arguments_fail_loudly():
         b.jam(params())  # no particles set
     with pytest.raises(P.DpmError):
         b.set_system(3, np.zeros(10), np.zeros(10), np.ones(10), np.ones(10), 5.0)
+
+
+def test_autotune_keeps_results():
+    sts = systems(SPEC[:3])
+    prm = params(max_steps=600)
+    ref, _, _ = batched(sts, prm)
+    b = P.Batch([st.n for st in sts])
+    b.config(1.0, 2.0, 1.7, 3.0 * 0.15 * 0.15)
+    for k, st in enumerate(sts):
+        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    cut, chunk, ms = b.autotune(prm, trial_steps=64, cutoffs=(1.6, 1.9), chunks=(8, 32))
+    say("autotune", cut, chunk, ms)
+    assert cut in (1.6, 1.9) and chunk in (8, 32) and ms > 0
+    res, _ = b.jam(prm)
+    b.close()
+    for k in range(len(sts)):
+        for f in FIELDS:
+            assert getattr(res[k], f) == getattr(ref[k], f), (k, f)
