@@ -225,6 +225,12 @@ def jam_batch(args, P, W):
     b.set_option(P.OPT_CHUNK, args.batch_chunk)
     for k, st in enumerate(sts):
         b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    tuned = None
+    if args.autotune:
+        cut, chunk, ms = b.autotune(prm, trial_steps=args.autotune_steps)
+        tuned = {"cutoff": cut, "steps_per_launch": chunk, "trial_ms_per_step": ms}
+        print(f"[bench] autotune: cutoff {cut}, {chunk} steps per launch", file=sys.stderr,
+              flush=True)
     t0 = time.perf_counter()
     res, steps = b.jam(prm, args.batch_max_steps)
     elapsed = time.perf_counter() - t0
@@ -282,6 +288,126 @@ def jam_batch(args, P, W):
             "plain_mean_s": sum(t_plain) / len(t_plain),
             "speedup_vs_plain": value / plain_jams_s,
             "sample_results_bit_identical": same,
+            "autotune": tuned,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+# ---------------------------------------------------------------- serving loop
+def serve(args, P, W):
+    """Poisson arrivals of jamming requests served by continuous batching (dpm_server)
+    against the plain sequential GPU path (one dpm_jam at a time, first come first served)
+    on the same arrival sequence.  value = packings/s of the server over the run."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    reqs = []
+    t = 0.0
+    for k in range(args.requests):
+        t += rng.exponential(1.0 / args.rate)
+        bumpy = rng.uniform() < args.bumpy_fraction
+        pt = float(np.exp(rng.uniform(np.log(args.p_lo), np.log(args.p_hi))))
+        if bumpy:
+            n = int(np.exp(rng.uniform(np.log(args.bumpy_n_lo), np.log(args.bumpy_n_hi))))
+            st = W.bumpy_config(n, args.jam_phi0 - 0.1, seed=args.seed + 1000 + k)
+        else:
+            n = int(np.exp(rng.uniform(np.log(args.disk_n_lo), np.log(args.disk_n_hi))))
+            st = W.disk_config(n, args.jam_phi0, seed=args.seed + 1000 + k)
+        prm = P.jam_params(pressure_target=pt, window_rel=args.jam_window,
+                           strain_step=args.jam_strain, max_compression_steps=args.jam_csteps,
+                           fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                              fmax_target=args.jam_fmax,
+                                              max_steps=args.jam_fire_steps))
+        reqs.append((t, bumpy, st, prm))
+    n_bumpy = sum(1 for r in reqs if r[1])
+    print(f"[bench] serving {len(reqs)} requests ({n_bumpy} rigid bumpy) at rate {args.rate}/s",
+          file=sys.stderr, flush=True)
+
+    def submit(sv, r):
+        _, bumpy, st, prm = r
+        if bumpy:
+            return sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm,
+                             e_c=W.bumpy_contact_scale(st), angle=st.angle,
+                             vertices_per_particle=st.nverts, vertex_radius=st.rv)
+        return sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm)
+
+    sv = P.Server(max_systems=args.serve_systems, arena_slots=args.serve_slots,
+                  budget_bytes=args.serve_budget, bumpy_workers=args.serve_workers)
+    ids = [None] * len(reqs)
+    t0 = time.perf_counter()
+    nxt = 0
+    done = 0
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            ids[nxt] = submit(sv, reqs[nxt])
+            nxt += 1
+        done, inflight, queued = sv.progress(args.batch_chunk)
+        if inflight == 0 and queued == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, min(0.01, reqs[nxt][0] - (time.perf_counter() - t0))))
+    elapsed = time.perf_counter() - t0
+    lat = np.array([sv.result(i)["latency_s"] for i in ids])
+    served = [sv.result(i)["result"] for i in ids]
+    sv.close()
+    # the plain path: the same arrivals, one request at a time
+    print("[bench] plain sequential path", file=sys.stderr, flush=True)
+    k_plain = min(args.plain_sample, len(reqs))
+    free_at, lat_plain, busy, same = 0.0, [], 0.0, True
+    for k in range(k_plain):
+        arr, bumpy, st, prm = reqs[k]
+        if bumpy:
+            s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st))
+        else:
+            s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
+        s.synchronize()
+        c0 = time.perf_counter()
+        r = s.jam(prm)
+        s.synchronize()
+        dt = time.perf_counter() - c0
+        s.close()
+        start = max(arr, free_at)
+        free_at = start + dt
+        busy += dt
+        lat_plain.append(free_at - arr)
+        for f in ("status", "fire_steps_total", "pressure", "pe", "packing_fraction"):
+            same = same and getattr(r, f) == getattr(served[k], f)
+    plain_rate = k_plain / busy
+    value = len(reqs) / elapsed
+    lp = np.array(lat_plain)
+    out = {
+        "metric": "served jammed packings/s and latency vs the plain sequential GPU path",
+        "value": value,
+        "unit": "packings/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 0,
+        "ms_per_step": elapsed / len(reqs) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Poisson arrivals and configurations)",
+        "config": {
+            "workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s: "
+                        f"{100 * (1 - args.bumpy_fraction):.0f}% disks N log-uniform in "
+                        f"[{args.disk_n_lo}, {args.disk_n_hi}], "
+                        f"{100 * args.bumpy_fraction:.0f}% rigid bumpy (20/28 vertices) N "
+                        f"log-uniform in [{args.bumpy_n_lo}, {args.bumpy_n_hi}]; target "
+                        f"pressure log-uniform in [{args.p_lo}, {args.p_hi}]; strain "
+                        f"{args.jam_strain}, FIRE to |F|max<{args.jam_fmax} (at most "
+                        f"{args.jam_fire_steps} steps each)",
+            "requests": len(reqs),
+            "rigid_bumpy_requests": n_bumpy,
+            "elapsed_s": elapsed,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "plain_packings_per_s": plain_rate,
+            "plain_latency_p50_s": float(np.percentile(lp, 50)),
+            "plain_latency_p99_s": float(np.percentile(lp, 99)),
+            "plain_sample": k_plain,
+            "speedup_vs_plain": value / plain_rate,
+            "sample_results_bit_identical": same,
         },
         "cpu_baseline": None,
     }
@@ -323,7 +449,22 @@ def main():
     ap.add_argument("--batch-chunk", type=int, default=32)
     ap.add_argument("--batch-max-steps", type=int, default=10**9)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch"),
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--autotune-steps", type=int, default=100)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--bumpy-fraction", type=float, default=0.3)
+    ap.add_argument("--disk-n-lo", type=int, default=128)
+    ap.add_argument("--disk-n-hi", type=int, default=16384)
+    ap.add_argument("--bumpy-n-lo", type=int, default=128)
+    ap.add_argument("--bumpy-n-hi", type=int, default=4096)
+    ap.add_argument("--p-lo", type=float, default=1e-7)
+    ap.add_argument("--p-hi", type=float, default=1e-3)
+    ap.add_argument("--serve-systems", type=int, default=1024)
+    ap.add_argument("--serve-slots", type=int, default=1 << 22)
+    ap.add_argument("--serve-budget", type=int, default=16 << 30)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch", "serve"),
                     default="nve-disk",
                     help="nve-disk: the headline NVE benchmark (default); fire-rigid: FIRE "
                          "of N=65,536 rigid bumpy particles to |F|max < 1e-12")
@@ -358,6 +499,9 @@ def main():
         return
     if args.workload == "jam-batch":
         jam_batch(args, P, W)
+        return
+    if args.workload == "serve":
+        serve(args, P, W)
         return
     n = args.n
     print(f"[bench] preparing N={n} (FIRE 200 steps, T=1e-5)", file=sys.stderr, flush=True)
