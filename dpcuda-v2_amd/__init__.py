@@ -29,6 +29,18 @@ F_STRESS_XX, F_STRESS_XY, F_STRESS_YX, F_STRESS_YY = 17, 18, 19, 20
 F_VERTEX_X, F_VERTEX_Y, F_VERTEX_MASS, F_NUM_VERTICES, F_VERTEX_CONTACTS = 21, 22, 23, 24, 25
 INT_FIELDS = {F_STATIC_INDEX, F_NUM_NEIGHBORS, F_CONTACTS, F_NUM_VERTICES, F_VERTEX_CONTACTS}
 VERTEX_FIELDS = {F_VERTEX_X, F_VERTEX_Y, F_VERTEX_MASS}
+# pair analytics fields (dpm.h DPM_P_*), in field order; the first 18 exist for disks
+PAIR_FIELDS = ("potential_pairs", "force_pairs_x", "force_pairs_y", "distance_pairs_x",
+               "distance_pairs_y", "pair_ids_i", "pair_ids_j", "overlap_pairs", "radsum_pairs",
+               "pair_separation_angle", "hessian_pairs_xx", "hessian_pairs_xy",
+               "hessian_pairs_yx", "hessian_pairs_yy", "hessian_ii_xx", "hessian_ii_xy",
+               "hessian_ii_yx", "hessian_ii_yy", "angle_pairs_i", "angle_pairs_j",
+               "this_vertex_contact_counts", "pair_friction_coefficient", "pair_vertex_overlaps",
+               "hessian_pairs_xt", "hessian_pairs_yt", "hessian_pairs_tx", "hessian_pairs_ty",
+               "hessian_pairs_tt", "hessian_ii_xt", "hessian_ii_yt", "hessian_ii_tx",
+               "hessian_ii_ty", "hessian_ii_tt")
+PAIR_DISK_FIELDS = 18
+PAIR_INT_FIELDS = {5, 6, 20}
 
 (RED_PE, RED_KE, RED_POWER, RED_FMAX_SQ, RED_FORCE_BALANCE, RED_PRESSURE, RED_CONTACTS,
  RED_TEMPERATURE, RED_PACKING_FRACTION, RED_MAX_NEIGH_DISP_SQ, RED_MAX_CELL_DISP_SQ,
@@ -142,6 +154,9 @@ _SIGS = {
     "scale_to_temperature": (_I, [_P, _D]),
     "jam": (_I, [_P, C.POINTER(JamParams), C.POINTER(JamResult)]),
     "bumpy_setup": (_I, [_P, _P, _D, _I]),
+    "pair_analytics": (_I, [_P, C.POINTER(_L)]),
+    "pair_download": (_I, [_P, _I, _P, _L]),
+    "bumpy_area": (_I, [_P, _P, _L]),
 }
 # entry points only the HIP library has
 _GPU_ONLY = {
@@ -318,6 +333,24 @@ class System:
 
     def vertex_pairs(self):
         return self._pairs("vertex_pairs", max(16, 16 * self.nv))
+
+    def pair_analytics(self) -> dict:
+        """Per-slot pair arrays in the reference's layout: {name: (N, row) array}; rows are
+        the library's internal particle order, unused slots hold the fill values."""
+        row = C.c_int64(0)
+        self._c("pair_analytics", C.byref(row))
+        out = {}
+        nf = len(PAIR_FIELDS) if self.kind == BUMPY else PAIR_DISK_FIELDS
+        for f, name in enumerate(PAIR_FIELDS[:nf]):
+            a = np.empty(self.n * row.value, np.int64 if f in PAIR_INT_FIELDS else np.float64)
+            self._c("pair_download", f, _ptr(a), a.size)
+            out[name] = a.reshape(self.n, row.value)
+        return out
+
+    def bumpy_area(self) -> np.ndarray:
+        a = np.empty(self.n)
+        self._c("bumpy_area", _ptr(a), self.n)
+        return a
 
     def neighbor_stats(self):
         v = [C.c_int64() for _ in range(4)]

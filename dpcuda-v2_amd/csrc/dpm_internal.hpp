@@ -154,6 +154,10 @@ struct dpm_system {
     // rigid bumpy
     bool rotation = true;
     double rv = 0.0, rc_v = -1, pcut_v = -1;
+    // pair analytics (dpm_pairs.hip): pair_fields blocks of N * pair_row values
+    double* pairs = nullptr;
+    int64_t pair_alloc = 0, pair_row = 0;
+    int pair_fields = 0;
     // snapshot (device resident)
     bool snap_valid = false;
     std::vector<void*> snap_allocs;

@@ -1,7 +1,11 @@
-// run_args.hpp -- key=value command-line arguments into a ConfigDict, standing in for
-// the run_config.json that the reference's drivers read through load_configs
-// (include/io/arg_parsing.h:7-64).  Values that parse as integers become long, other
-// numbers double, true/false bool, anything else a string.
+// run_args.hpp -- the two ways a driver is started:
+//   driver <run directory>      the reference's form: <run directory>/system/run_config.json
+//                               read by load_configs (include/io/arg_parsing.h:7-64), logs
+//                               written through IOManager as the reference's driver does;
+//   driver key=value ...        a particle from the standard configuration and the run
+//                               parameters on the command line, nothing written to disk.
+// key=value values that parse as integers become long, other numbers double, true/false
+// bool, anything else a string.
 #pragma once
 #include <cstdlib>
 #include <iostream>
@@ -10,6 +14,11 @@
 
 #include "../config_dict.hpp"
 #include "../factory.hpp"
+
+// one argument that is not key=value: the reference's run-directory form
+inline bool directory_mode(int argc, char** argv) {
+    return argc == 2 && std::string(argv[1]).find('=') == std::string::npos;
+}
 
 inline ConfigDict parse_run_args(int argc, char** argv, ConfigDict defaults) {
     for (int i = 1; i < argc; ++i) {

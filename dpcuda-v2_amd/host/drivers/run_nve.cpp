@@ -2,16 +2,66 @@
 // optional Gaussian velocities at `temperature`, NVE with dt = dt_dimless / time unit,
 // num_steps of nve.step() (or one device-resident nve.run with loop=device).
 //
+//   run_nve <run directory>      as the reference (load_configs, IOManager logs)
 //   run_nve n_particles=262144 packing_fraction=0.85 num_steps=10000 dt_dimless=0.01
 //           temperature=1e-5 seed=0 fire_steps=200 loop=device
 #include <chrono>
 #include <iostream>
 
 #include "../integrator.hpp"
+#include "../io.hpp"
 #include "../routines.hpp"
 #include "run_args.hpp"
 
+// scripts/runNVE.cpp:15-55.  The NVE steps between two logged steps run as one
+// device-resident loop (NVE::run gives the same states as that many NVE::step calls).
+static int run_directory(int argc, char** argv) {
+    auto [particle, step, console_config, energy_config, state_config, run_config] =
+        load_configs(argc, argv);
+    long num_steps = run_config["num_steps"].get<long>() + step;
+    double dt_dimless = run_config["dt_dimless"].get<double>();
+    double temperature = run_config["temperature"].get<double>();
+    std::filesystem::path output_dir = run_config["output_dir"].get<std::filesystem::path>();
+    bool overwrite = true;
+    if (temperature > 0) particle->setRandomVelocities(temperature);
+    ConfigDict nve_config_dict = get_nve_config_dict(dt_dimless / particle->getTimeUnit());
+    NVE nve(*particle, nve_config_dict);
+    std::vector<std::string> init_names = particle->getFundamentalValues();
+    std::vector<ConfigDict> log_group_configs = {
+        console_config, energy_config, state_config,
+        config_from_names_lin_everyN(init_names, 10000, "restart")};
+    IOManager io(log_group_configs, *particle, &nve, output_dir.string(), 20, overwrite);
+    io.write_params();
+    run_config.save(output_dir / "system" / "run_config.json");
+    auto start_time = std::chrono::high_resolution_clock::now();
+    long pending = 0;
+    while (step < num_steps) {
+        ++pending;  // nve.step() of this step
+        if (io.prepare(step)) {
+            nve.run(pending);
+            pending = 0;
+            io.write(step);
+        }
+        step++;
+    }
+    if (pending) nve.run(pending);
+    io.log(step, true);
+    auto end_time = std::chrono::high_resolution_clock::now();
+    std::cout << "Time taken: "
+              << std::chrono::duration_cast<std::chrono::seconds>(end_time - start_time).count()
+              << " seconds" << std::endl;
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (directory_mode(argc, argv)) {
+        try {
+            return run_directory(argc, argv);
+        } catch (const std::exception& e) {
+            std::cerr << "run_nve: " << e.what() << std::endl;
+            return 1;
+        }
+    }
     ConfigDict run_config;
     run_config["particle_type"] = "Disk";
     run_config["n_particles"] = 1024L;

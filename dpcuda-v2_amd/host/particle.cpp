@@ -42,9 +42,9 @@ double random_uniform(unsigned int seed, unsigned int n, double a, double b) {
 // ------------------------------------------------------------------ Particle
 Particle::Particle(int kind, long n, long nv) : kind_(kind) { allocate(n, nv); }
 
-void Particle::allocate(long n, long nv) {
+void Particle::allocate(long n, long nv, bool fresh) {
     if (n <= 0) throw std::invalid_argument("Particle: n_particles must be positive");
-    if (sys_ && n == n_particles && nv == n_vertices) return;
+    if (sys_ && n == n_particles && nv == n_vertices && !fresh) return;
     if (sys_) dpm_destroy(sys_);
     sys_ = dpm_create(kind_, n, nv);
     if (!sys_) throw std::runtime_error(std::string("Particle: ") + dpm_last_error());
@@ -452,9 +452,11 @@ double RigidBumpy::calculateVertexRadius(double d_min, long n_small, double segm
     return d_min / (1.0 + segment / std::sin(ang / 2.0)) / 2.0;
 }
 
+// RigidBumpy::getFundamentalValues (rigid_bumpy.h:32)
 std::vector<std::string> RigidBumpy::getFundamentalValues() {
-    return {"positions", "velocities", "angles", "angular_velocities", "box_size", "radii",
-            "masses", "vertex_positions", "num_vertices_in_particle"};
+    return {"radii",           "positions",          "angles",        "velocities",
+            "angular_velocities", "num_vertices_in_particle", "vertex_positions", "masses",
+            "moments_of_inertia", "box_size",         "vertex_masses"};
 }
 
 void RigidBumpy::setVertexCounts(const std::vector<long>& nvp) {

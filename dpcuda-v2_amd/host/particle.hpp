@@ -11,11 +11,16 @@
 // failures reported by the device library throw std::runtime_error with the message
 // of dpm_last_error() (the reference prints and exits on CUDA errors).
 #pragma once
+#include <filesystem>
+#include <map>
+#include <set>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
 #include "../../include/dpm.h"
+#include "array_data.hpp"
 #include "config_dict.hpp"
 
 class Particle {
@@ -110,6 +115,7 @@ class Particle {
     virtual void calculateKineticEnergy() {}
     double totalPotentialEnergy();
     double totalKineticEnergy();
+    double totalEnergy() { return totalKineticEnergy() + totalPotentialEnergy(); }
     double getForceBalance();
     virtual double getPowerFire();
     virtual void setVelocitiesToZero();
@@ -121,6 +127,9 @@ class Particle {
     std::vector<double> getStressTensor();
     virtual void countContacts();
     double getContactCount();
+    // Disk/RigidBumpy::calculateForceDistancePairs (disk.cu:228-248, rigid_bumpy.cu:1383-1470):
+    // per-slot pair arrays of the current state, read through getArrayData
+    void calculateForceDistancePairs();
     virtual double getVertexContactCount() { return 0.0; }
 
     // ---- box scaling and saved state ----
@@ -132,17 +141,52 @@ class Particle {
     virtual void setLastState();
     virtual void revertToLastState();
 
+    // ---- state files (particle.cu:45-229) ----
+    // Particle::load / Disk::load / RigidBumpy::load (disk.cu:35-96, rigid_bumpy.cu:37-132):
+    // the particle described by <root>/system/particle_config.json with its arrays read
+    // from <root>/system/init ("init"), <root>/system/restart ("restart", step from
+    // step.dat) or <root>/trajectories/t<frame> ("frame", frame -1 = the latest; arrays
+    // missing there come from restart).  Returns the step (0 for init, the frame for
+    // frame).  The neighbour set-up then runs as in initializeFromConfig.
+    virtual long load(std::filesystem::path root_path, std::string source, long frame = -2) = 0;
+    std::tuple<std::filesystem::path, std::filesystem::path, std::filesystem::path, long>
+    getPaths(std::filesystem::path root_path, std::string source, long frame);
+    // reads one <name>.dat into the particle (false if the file is absent or unreadable)
+    virtual bool tryLoadArrayData(std::filesystem::path path);
+    void tryLoadData(const std::filesystem::path& frame_path,
+                     const std::filesystem::path& restart_path,
+                     const std::filesystem::path& init_path, const std::string& source,
+                     const std::string& file_name_without_extension);
+    // every known <name><extension> file of a directory
+    virtual void loadDataFromPath(std::filesystem::path root_path, std::string data_file_extension);
+
+    // ---- named arrays and values for the I/O layer (particle.cu:485-692, particle.h:61-104) ----
+    virtual ArrayData getArrayData(const std::string& array_name);
+    virtual std::vector<std::string> get_reorder_arrays() { return {"static_particle_index"}; }
+    // log names whose value needs a calculation before it is read
+    bool is_dependent(const std::string& log_name) const;
+    void calculate_dependencies(const std::string& log_name);
+    void reset_dependency_status() { dependency_done_.clear(); }
+
    protected:
     explicit Particle(int kind) : kind_(kind) {}
     Particle(int kind, long n_particles, long n_vertices);
-    // (re)creates the device system for n particles and nv vertices
-    void allocate(long n_particles, long n_vertices);
+    // (re)creates the device system for n particles and nv vertices; an existing system of
+    // that size is kept unless `fresh` (a new system: static order = upload order again)
+    void allocate(long n_particles, long n_vertices, bool fresh = false);
     dpm_system* sys() const;
     void check(int status) const;
     std::vector<double> download(int field, int64_t count);
     void upload(int field, const std::vector<double>& v);
     virtual void applyNeighborConfig();
     double reduce(int what);
+    virtual void calculateDependency(const std::string& calculation);
+    virtual const std::map<std::string, std::string>& calculationDependencies() const;
+    // arrays read by tryLoadArrayData, uploaded once the particle is set up
+    std::map<std::string, std::vector<double>> staged_;
+    std::set<std::string> dependency_done_;
+    long pair_row_ = 0;  // row length of the last calculateForceDistancePairs
+    ArrayData pairArray(const std::string& array_name);
 
     dpm_system* sys_ = nullptr;
     int kind_ = DPM_DISK;
@@ -155,6 +199,7 @@ class Disk : public Particle {
     Disk() : Particle(DPM_DISK) {}
     explicit Disk(long n_particles);
     void initializeFromConfig(ConfigDict& config, bool minimize = false) override;
+    long load(std::filesystem::path root_path, std::string source, long frame = -2) override;
 };
 
 class RigidBumpy : public Particle {
@@ -186,8 +231,18 @@ class RigidBumpy : public Particle {
     std::vector<double> getTorques();
     double getVertexContactCount() override;
 
+    long load(std::filesystem::path root_path, std::string source, long frame = -2) override;
+    bool tryLoadArrayData(std::filesystem::path path) override;
+    void loadDataFromPath(std::filesystem::path root_path, std::string data_file_extension) override;
+    ArrayData getArrayData(const std::string& array_name) override;
+    std::vector<std::string> get_reorder_arrays() override {
+        return {"static_particle_index", "static_vertex_index"};
+    }
+
    protected:
     void applyNeighborConfig() override;
+    // uploads the staged rotation and vertex arrays after the vertex set-up
+    void applyStaged();
     std::vector<long> nv_;
 };
 

@@ -184,6 +184,39 @@ typedef struct {
 } dpm_jam_result;
 int dpm_jam(dpm_system* s, const dpm_jam_params* p, dpm_jam_result* out);
 
+/* ---- pair analytics -------------------------------------------------------------------
+ * Disk::calculateForceDistancePairs (disk.cu:228-248, kernelCalcDiskForceDistancePairs
+ * disk/kernels.cu:73-175) and RigidBumpy::calculateForceDistancePairs
+ * (rigid_bumpy.cu:1383-1470, kernelCalcRigidBumpyForceDistancePairs
+ * rigid_bumpy/kernels.cu:482-702): one entry per neighbour-list slot, laid out as the
+ * reference lays them out: row i = the i-th particle in the library's current (cell)
+ * order, row length = the max_alloc of dpm_neighbor_stats, entry k = the k-th neighbour
+ * of that row; unused entries hold the reference's fill values.  dpm_pair_analytics
+ * computes every field for the current state and returns the row length;
+ * dpm_pair_download copies one field (n_particles * row_length values: int64 for the id
+ * and vertex-count fields, double otherwise).  The *_T* Hessian entries, the angles, the
+ * vertex-contact counts, the friction coefficient and the vertex overlaps exist for
+ * rigid bumpy systems only. */
+enum {
+    DPM_P_POTENTIAL = 0, DPM_P_FORCE_X = 1, DPM_P_FORCE_Y = 2, DPM_P_DIST_X = 3,
+    DPM_P_DIST_Y = 4, DPM_P_ID_I = 5, DPM_P_ID_J = 6, DPM_P_OVERLAP = 7, DPM_P_RADSUM = 8,
+    DPM_P_ANGLE = 9,
+    DPM_P_H_XX = 10, DPM_P_H_XY = 11, DPM_P_H_YX = 12, DPM_P_H_YY = 13,
+    DPM_P_HII_XX = 14, DPM_P_HII_XY = 15, DPM_P_HII_YX = 16, DPM_P_HII_YY = 17,
+    DPM_P_ANGLE_I = 18, DPM_P_ANGLE_J = 19, DPM_P_VERTEX_CONTACTS = 20, DPM_P_FRICTION = 21,
+    DPM_P_VERTEX_OVERLAP = 22,
+    DPM_P_H_XT = 23, DPM_P_H_YT = 24, DPM_P_H_TX = 25, DPM_P_H_TY = 26, DPM_P_H_TT = 27,
+    DPM_P_HII_XT = 28, DPM_P_HII_YT = 29, DPM_P_HII_TX = 30, DPM_P_HII_TY = 31,
+    DPM_P_HII_TT = 32,
+    DPM_P_NFIELDS = 33
+};
+int dpm_pair_analytics(dpm_system* s, int64_t* row_length);
+int dpm_pair_download(dpm_system* s, int field, void* out, int64_t count);
+/* RigidBumpy::calculateParticleArea (rigid_bumpy.cu:1018-1024 with
+ * kernelCalculateBumpyParticleAreaFull, rigid_bumpy/kernels.cu:1226-1283): the area of each
+ * particle's vertex ring (polygon plus the exposed parts of the vertex disks), static order */
+int dpm_bumpy_area(dpm_system* s, double* out, int64_t count);
+
 /* ---- rigid bumpy set-up --------------------------------------------------------------------
  * RigidBumpy::setParticleStartIndex/setVertexParticleIndex/setVertexPositions/setMass/
  * setDegreesOfFreedom (rigid_bumpy.cu:476-478, :562-568, :783-787, :1068-1081, :618-624). */

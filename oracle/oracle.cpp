@@ -173,6 +173,10 @@ struct orc_system {
     std::vector<long> vpart, svidx;
     std::vector<long> vnbr, vnum;
     long vmax_alloc = 4;
+    // pair analytics: pair_fields blocks of N * pair_row values (ids as int64 bits)
+    std::vector<double> pairs;
+    long pair_row = 0;
+    int pair_fields = 0;
     // snapshot
     struct Snap {
         bool valid = false;
@@ -1294,6 +1298,266 @@ int orc_vertex_pairs(orc_system* s, int64_t* oi, int64_t* oj, int64_t cap, int64
         }
     *count = c;
     return c > cap ? fail("vertex_pairs: capacity too small") : 0;
+}
+
+// ------------------------------------------------------------ pair analytics
+namespace {
+
+void put_i64(std::vector<double>& out, long S, int f, long t, long long v) {
+    std::memcpy(&out[(size_t)(f * S + t)], &v, sizeof v);
+}
+
+// the fill values of Disk::calculateForceDistancePairs (disk.cu:229-245) and
+// RigidBumpy::calculateForceDistancePairs (rigid_bumpy.cu:1384-1415)
+void pair_fill(orc_system* s, long S, int nf) {
+    s->pairs.assign((size_t)(nf * S), 0.0);
+    for (long t = 0; t < S; ++t) {
+        for (int f : {ORC_P_POTENTIAL, ORC_P_DIST_X, ORC_P_DIST_Y, ORC_P_OVERLAP, ORC_P_RADSUM,
+                      ORC_P_ANGLE})
+            s->pairs[(size_t)(f * S + t)] = -1.0;
+        put_i64(s->pairs, S, ORC_P_ID_I, t, -1);
+        put_i64(s->pairs, S, ORC_P_ID_J, t, -1);
+        if (nf > ORC_P_ANGLE_I) {
+            s->pairs[(size_t)(ORC_P_ANGLE_I * S + t)] = -1.0;
+            s->pairs[(size_t)(ORC_P_ANGLE_J * S + t)] = -1.0;
+            s->pairs[(size_t)(ORC_P_FRICTION * S + t)] = -1.0;
+            put_i64(s->pairs, S, ORC_P_VERTEX_CONTACTS, t, -1);
+        }
+    }
+}
+
+// kernelCalcDiskForceDistancePairs: src/particles/disk/kernels.cu:73-175
+void disk_pairs(orc_system* s, long S) {
+    const long M = s->max_alloc;
+    const double e_c = s->e_c;
+    std::vector<double>& P = s->pairs;
+    for (long i = 0; i < s->N; ++i) {
+        if (s->nnum[i] == 0) continue;
+        for (long n = 0; n < M; ++n) {
+            long j = s->nbr[(size_t)(i * M + n)];
+            if (j == -1 || j == i) continue;
+            long t = i * M + n;
+            double fx, fy;
+            double e = pair_law(s->x[i], s->y[i], s->r[i], s->x[j], s->y[j], s->r[j], s->box[0],
+                                s->box[1], s->e_c, s->n_c, fx, fy);
+            P[ORC_P_POTENTIAL * S + t] = e;
+            P[ORC_P_FORCE_X * S + t] = fx;
+            P[ORC_P_FORCE_Y * S + t] = fy;
+            double dx = pbc(s->x[i], s->x[j], s->box[0]);
+            double dy = pbc(s->y[i], s->y[j], s->box[1]);
+            P[ORC_P_DIST_X * S + t] = dx;
+            P[ORC_P_DIST_Y * S + t] = dy;
+            put_i64(P, S, ORC_P_ID_I, t, s->sidx[i]);
+            put_i64(P, S, ORC_P_ID_J, t, s->sidx[j]);
+            double dist = std::sqrt(dx * dx + dy * dy);
+            P[ORC_P_OVERLAP * S + t] = dist - (s->r[i] + s->r[j]);
+            P[ORC_P_RADSUM * S + t] = s->r[i] + s->r[j];
+            P[ORC_P_ANGLE * S + t] = std::atan2(dy, dx);
+            if (dist < s->r[i] + s->r[j]) {
+                double rs = s->r[i] + s->r[j];
+                double t_ij = -e_c / rs * (1 - dist / rs);
+                double c_ij = e_c / (rs * rs);
+                double hij[4] = {0, 0, 0, 0}, hii[4] = {0, 0, 0, 0};
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        // unit derivatives of (dx, dy) by coordinate a of i and b of j
+                        double ax = 1 * (a == 0), ay = 1 * (a == 1);
+                        double bx = -1 * (b == 0), by = -1 * (b == 1);
+                        double zx = 0, zy = 0;
+                        double qa = dx * ax + dy * ay;
+                        double qb = dx * bx + dy * by;
+                        double dq = ax * bx + dx * zx + ay * by + dy * zy;
+                        hij[a * 2 + b] += c_ij * (qa * qb) / (dist * dist) +
+                                          t_ij * (dq / dist - (qa * qb) / (dist * dist * dist));
+                        double cx = 1 * (b == 0), cy = 1 * (b == 1);
+                        double wx = 0, wy = 0;
+                        double qc = dx * cx + dy * cy;
+                        double dqc = ax * cx + dx * wx + ay * cy + dy * wy;
+                        hii[a * 2 + b] += c_ij * (qa * qc) / (dist * dist) +
+                                          t_ij * (dqc / dist - (qa * qc) / (dist * dist * dist));
+                    }
+                for (int c = 0; c < 4; ++c) {
+                    P[(ORC_P_H_XX + c) * S + t] += hij[c];
+                    P[(ORC_P_HII_XX + c) * S + t] += hii[c];
+                }
+            }
+        }
+    }
+}
+
+// kernelCalcRigidBumpyForceDistancePairs: src/particles/rigid_bumpy/kernels.cu:482-702
+void bumpy_pairs(orc_system* s, long S) {
+    const long M = s->max_alloc;
+    const double rv = s->rv, e_c = s->e_c;
+    const double PI_R = 3.14159265358979323846;  // M_PI
+    std::vector<double>& P = s->pairs;
+    for (long i = 0; i < s->N; ++i) {
+        long nn = s->nnum[i];
+        for (long n = 0; n < nn; ++n) {
+            long j = s->nbr[(size_t)(i * M + n)];
+            if (j == -1 || j == i) continue;
+            double ti = s->th[i], tj = s->th[j];
+            double fx = 0.0, fy = 0.0, energy = 0.0, voverlap = 0.0;
+            long count_i = 0;
+            double hij[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, hii[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (long u = 0; u < s->nverts[i]; ++u) {
+                long vi = s->vstart[i] + u;
+                double phi_u = u * 2 * PI_R / s->nverts[i];
+                double ex = s->vxp[vi] - s->x[i], ey = s->vyp[vi] - s->y[i];
+                double Ri = std::sqrt(ex * ex + ey * ey);
+                double si, ci;
+                dpm_sincos(ti + phi_u, &si, &ci);
+                bool contact = false;
+                for (long m = 0; m < s->vnum[vi]; ++m) {
+                    long vo = s->vnbr[(size_t)(vi * s->vmax_alloc + m)];
+                    if (vo == -1 || vo == vi || s->vpart[vo] != j) continue;
+                    long v = vo - s->vstart[j];
+                    double phi_v = v * 2 * PI_R / s->nverts[j];
+                    double gx = s->vxp[vo] - s->x[j], gy = s->vyp[vo] - s->y[j];
+                    double Rj = std::sqrt(gx * gx + gy * gy);
+                    double tfx, tfy;
+                    double te = pair_law(s->vxp[vi], s->vyp[vi], rv, s->vxp[vo], s->vyp[vo], rv,
+                                         s->box[0], s->box[1], e_c, s->n_c, tfx, tfy);
+                    energy += te;
+                    fx += tfx;
+                    fy += tfy;
+                    if (!(te > 0.0)) continue;
+                    contact = true;
+                    double dx = pbc(s->vxp[vi], s->vxp[vo], s->box[0]);
+                    double dy = pbc(s->vyp[vi], s->vyp[vo], s->box[1]);
+                    voverlap += (rv * 2) - std::sqrt(dx * dx + dy * dy);
+                    double dist = std::sqrt(dx * dx + dy * dy);
+                    double rs = rv * 2.0;
+                    double t_ij = -e_c / rs * (1 - dist / rs);
+                    double c_ij = e_c / (rs * rs);
+                    double sj, cj;
+                    dpm_sincos(tj + phi_v, &sj, &cj);
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            // derivatives of the vertex separation by (x, y, theta) of i and j
+                            double ax = (a == 0) - Ri * si * (a == 2);
+                            double ay = (a == 1) + Ri * ci * (a == 2);
+                            double bx = -1.0 * ((b == 0) - Rj * sj * (b == 2));
+                            double by = -1.0 * ((b == 1) + Rj * cj * (b == 2));
+                            double qa = dx * ax + dy * ay;
+                            double qb = dx * bx + dy * by;
+                            double dq = (ax * bx + ay * by) / dist;
+                            hij[a * 3 + b] += c_ij * qa * qb / (dist * dist) +
+                                              t_ij * (dq - qa * qb / (dist * dist * dist));
+                            double cx = (b == 0) - Ri * si * (b == 2);
+                            double cy = (b == 1) + Ri * ci * (b == 2);
+                            double wx = -Ri * ci * (a == 2) * (b == 2);
+                            double wy = -Ri * si * (a == 2) * (b == 2);
+                            double qc = dx * cx + dy * cy;
+                            double dqc = (ax * cx + dx * wx + ay * cy + dy * wy) / dist;
+                            hii[a * 3 + b] += c_ij * qa * qc / (dist * dist) +
+                                              t_ij * (dqc - qa * qc / (dist * dist * dist));
+                        }
+                }
+                if (contact) count_i++;
+            }
+            long t = i * M + n;
+            P[ORC_P_POTENTIAL * S + t] = energy;
+            P[ORC_P_FORCE_X * S + t] = fx;
+            P[ORC_P_FORCE_Y * S + t] = fy;
+            double xd = pbc(s->x[i], s->x[j], s->box[0]);
+            double yd = pbc(s->y[i], s->y[j], s->box[1]);
+            P[ORC_P_DIST_X * S + t] = xd;
+            P[ORC_P_DIST_Y * S + t] = yd;
+            put_i64(P, S, ORC_P_ID_I, t, s->sidx[i]);
+            put_i64(P, S, ORC_P_ID_J, t, s->sidx[j]);
+            P[ORC_P_VERTEX_OVERLAP * S + t] = voverlap;
+            double dist = std::sqrt(xd * xd + yd * yd);
+            P[ORC_P_OVERLAP * S + t] = dist - (s->r[i] + s->r[j]);
+            P[ORC_P_RADSUM * S + t] = s->r[i] + s->r[j];
+            P[ORC_P_ANGLE * S + t] = std::atan2(yd, xd);
+            P[ORC_P_ANGLE_I * S + t] = ti;
+            P[ORC_P_ANGLE_J * S + t] = tj;
+            put_i64(P, S, ORC_P_VERTEX_CONTACTS, t, count_i);
+            double mu = 0.0;
+            if (fx != 0.0 || fy != 0.0) {  // |tangential| / |normal| force
+                double nx = fx * xd / dist, ny = fy * yd / dist;
+                double nf = std::sqrt(nx * nx + ny * ny);
+                double qx = fx - nx, qy = fy - ny;
+                double tf = std::sqrt(qx * qx + qy * qy);
+                mu = tf / nf;
+            }
+            P[ORC_P_FRICTION * S + t] = mu;
+            const int hp[9] = {ORC_P_H_XX, ORC_P_H_XY, ORC_P_H_XT, ORC_P_H_YX, ORC_P_H_YY,
+                               ORC_P_H_YT, ORC_P_H_TX, ORC_P_H_TY, ORC_P_H_TT};
+            const int hs[9] = {ORC_P_HII_XX, ORC_P_HII_XY, ORC_P_HII_XT, ORC_P_HII_YX, ORC_P_HII_YY,
+                               ORC_P_HII_YT, ORC_P_HII_TX, ORC_P_HII_TY, ORC_P_HII_TT};
+            for (int c = 0; c < 9; ++c) {
+                P[hp[c] * S + t] += hij[c];
+                P[hs[c] * S + t] += hii[c];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int orc_pair_analytics(orc_system* s, int64_t* row_length) {
+    if (s->method == ORC_NBR_NONE) return fail("orc_pair_analytics: neighbour list not initialised");
+    const long row = std::max(1L, s->max_alloc);
+    const long S = s->N * row;
+    const int nf = s->kind == ORC_BUMPY ? ORC_P_NFIELDS : ORC_P_HII_YY + 1;
+    pair_fill(s, S, nf);
+    if (s->kind == ORC_BUMPY) bumpy_pairs(s, S);
+    else disk_pairs(s, S);
+    s->pair_row = row;
+    s->pair_fields = nf;
+    if (row_length) *row_length = row;
+    return 0;
+}
+
+int orc_pair_download(orc_system* s, int field, void* out, int64_t count) {
+    if (s->pair_row == 0) return fail("orc_pair_download: call orc_pair_analytics first");
+    if (field < 0 || field >= s->pair_fields) return fail("orc_pair_download: unknown field");
+    const long S = s->N * s->pair_row;
+    if (count != S) return fail("orc_pair_download: count must be n_particles * row_length");
+    std::memcpy(out, s->pairs.data() + (size_t)field * S, sizeof(double) * (size_t)S);
+    return 0;
+}
+
+// RigidBumpy::calculateParticleArea: rigid_bumpy.cu:1018-1024 with
+// kernelCalculateBumpyParticleAreaFull rigid_bumpy/kernels.cu:1226-1283, angleBetweenVectors
+// and calcOverlapLenseArea include/particles/base/kernels.cuh:367-381
+int orc_bumpy_area(orc_system* s, double* out, int64_t count) {
+    if (s->kind != ORC_BUMPY) return fail("orc_bumpy_area: not a rigid bumpy system");
+    if (count != s->N) return fail("orc_bumpy_area: count must be n_particles");
+    const double PI_R = 3.14159265358979323846;
+    const double rv = s->rv;
+    auto seg = [](double R, double d) { return R * R * std::acos(d / R) - d * std::sqrt(R * R - d * d); };
+    for (long i = 0; i < s->N; ++i) {
+        double poly = 0.0, exposed = 0.0, va = PI_R * rv * rv;
+        long f = s->vstart[i], nv = s->nverts[i];
+        double cx = s->vxp[f], cy = s->vyp[f];
+        double px = s->vxp[f + nv - 1], py = s->vyp[f + nv - 1];
+        for (long k = 0; k < nv; ++k) {
+            double own = 0.0;
+            long nx_i = f + ((k + 1) % nv);
+            double nx = s->vxp[nx_i], ny = s->vyp[nx_i];
+            poly += cx * ny - nx * cy;
+            double ms = (nx - cx) * (cy - py) - (ny - cy) * (cx - px);
+            double mc = (nx - cx) * (cx - px) + (ny - cy) * (cy - py);
+            double ang = std::atan2(ms, mc);
+            double dx = nx - cx, dy = ny - cy;
+            double r = std::sqrt(dx * dx + dy * dy);
+            if (r < (2 * rv - 1e-10)) {
+                double d1 = (r * r - rv * rv + rv * rv) / (2 * r);
+                double d2 = r - d1;
+                own -= (seg(rv, d1) + seg(rv, d2)) / 2.0;
+            }
+            exposed += va * (PI_R - ang) / (2.0 * PI_R) - own;
+            px = cx;
+            py = cy;
+            cx = nx;
+            cy = ny;
+        }
+        out[s->sidx[i]] = std::fabs(poly) * 0.5 + exposed;
+    }
+    return 0;
 }
 
 int orc_neighbor_stats(orc_system* s, int64_t* ma, int64_t* tot, int64_t* nr, int64_t* ncr) {

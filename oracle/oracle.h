@@ -132,6 +132,25 @@ int orc_jam(orc_system* s, const orc_jam_params* p, orc_jam_result* out);
 int orc_bumpy_setup(orc_system* s, const int64_t* n_vertices_per_particle, double vertex_radius,
                     int rotation);
 
+/* pair analytics (Disk/RigidBumpy::calculateForceDistancePairs): same fields, layout and
+ * fill values as dpm_pair_analytics / dpm_pair_download */
+enum {
+    ORC_P_POTENTIAL = 0, ORC_P_FORCE_X = 1, ORC_P_FORCE_Y = 2, ORC_P_DIST_X = 3,
+    ORC_P_DIST_Y = 4, ORC_P_ID_I = 5, ORC_P_ID_J = 6, ORC_P_OVERLAP = 7, ORC_P_RADSUM = 8,
+    ORC_P_ANGLE = 9,
+    ORC_P_H_XX = 10, ORC_P_H_XY = 11, ORC_P_H_YX = 12, ORC_P_H_YY = 13,
+    ORC_P_HII_XX = 14, ORC_P_HII_XY = 15, ORC_P_HII_YX = 16, ORC_P_HII_YY = 17,
+    ORC_P_ANGLE_I = 18, ORC_P_ANGLE_J = 19, ORC_P_VERTEX_CONTACTS = 20, ORC_P_FRICTION = 21,
+    ORC_P_VERTEX_OVERLAP = 22,
+    ORC_P_H_XT = 23, ORC_P_H_YT = 24, ORC_P_H_TX = 25, ORC_P_H_TY = 26, ORC_P_H_TT = 27,
+    ORC_P_HII_XT = 28, ORC_P_HII_YT = 29, ORC_P_HII_TX = 30, ORC_P_HII_TY = 31,
+    ORC_P_HII_TT = 32,
+    ORC_P_NFIELDS = 33
+};
+int orc_pair_analytics(orc_system* s, int64_t* row_length);
+int orc_pair_download(orc_system* s, int field, void* out, int64_t count);
+int orc_bumpy_area(orc_system* s, double* out, int64_t count);
+
 /* canonical fixed-order reduction used by both sides (exposed for tests) */
 double orc_canonical_sum(const double* a, int64_t n);
 

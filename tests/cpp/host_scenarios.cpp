@@ -7,7 +7,8 @@
 // compare the lines key by key.
 //
 //     host_scenarios <scenario>      scenarios: rng config errors disk_nve disk_fire
-//                                    disk_adam disk_compress disk_jam_fire bumpy_init
+//                                    disk_adam disk_compress disk_jam_fire bumpy_init json
+//     host_scenarios <scenario> <dir>   io_disk io_bumpy (files written under <dir>)
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 
 #include "factory.hpp"
 #include "integrator.hpp"
+#include "io.hpp"
 #include "particle.hpp"
 #include "routines.hpp"
 
@@ -225,9 +227,179 @@ void scenario_bumpy_init() {
     put("bumpy.angles100", rb->getAngles());
 }
 
+// JSON text of ConfigDict values: one "@ json.<k> <dump>" line per case (no spaces in the
+// compact dumps), and the indented dump of a nested config written to stdout between
+// markers for the test to parse
+void scenario_json() {
+    std::vector<std::pair<std::string, ConfigValue>> cases = {
+        {"one", 1.0},          {"tenth", 0.1},        {"third", 1.0 / 3.0},
+        {"tiny", 1e-7},        {"small", 1e-4},       {"smaller", 1e-5},
+        {"e14", 1e14},         {"e15", 1e15},         {"e16", 1e16},
+        {"big", 1.5e300},      {"neg", -2.5e-8},      {"zero", 0.0},
+        {"negzero", -0.0},     {"int", 42L},          {"negint", -7L},
+        {"t", true},           {"f", false},          {"str", "a\"b\\c\n\t\x01"},
+        {"vec", std::vector<std::string>{"x", "y/N"}}, {"empty_vec", std::vector<double>{}},
+    };
+    for (auto& kv : cases) std::printf("@ json.%s %s\n", kv.first.c_str(), kv.second.dump().c_str());
+    ConfigDict c = get_standard_rigid_bumpy_config(32, 0.75);
+    c["log_names"] = std::vector<std::string>{"step", "PE/N"};
+    c["nested"] = {{"deep", {{"x", 1e-12}}}, {"empty", ConfigValue::object_value()}};
+    std::printf("=== dump4\n%s\n=== end\n", c.dump(4).c_str());
+    ConfigValue back = ConfigValue::parse(c.dump(4));
+    std::printf("@ json.roundtrip %d\n", back.dump(4) == c.dump(4) ? 1 : 0);
+    ConfigValue spaced = ConfigValue::parse(" { \"a\" : [ 1 , 2.5e3 , -0.0 , \"\\u00e9\\ud83d\\ude00\" ] , \"b\" : null } ");
+    std::printf("@ json.parsed %s\n", spaced.dump().c_str());
+    expect_throw<std::invalid_argument>("json_trailing", [&] { ConfigValue::parse("{} x"); });
+    expect_throw<std::invalid_argument>("json_unterminated", [&] { ConfigValue::parse("{\"a\": [1, 2"); });
+    expect_throw<std::invalid_argument>("json_bad_escape", [&] { ConfigValue::parse("\"\\q\""); });
+}
+
+std::vector<std::string> with_pairs(Particle& p, bool bumpy) {
+    std::vector<std::string> n = p.getFundamentalValues();
+    for (const char* x : {"force_pairs", "distance_pairs", "overlap_pairs", "radsum_pairs",
+                          "pair_separation_angle", "pair_ids", "potential_pairs", "contact_counts",
+                          "hessian_pairs_xx", "hessian_pairs_xy", "hessian_pairs_yx",
+                          "hessian_pairs_yy", "hessian_ii_xx", "hessian_ii_xy", "hessian_ii_yx",
+                          "hessian_ii_yy"})
+        n.push_back(x);
+    if (bumpy)
+        for (const char* x : {"angle_pairs_i", "angle_pairs_j", "this_vertex_contact_counts",
+                              "pair_friction_coefficient", "pair_vertex_overlaps",
+                              "hessian_pairs_xt", "hessian_pairs_yt", "hessian_pairs_tt",
+                              "hessian_pairs_tx", "hessian_pairs_ty", "hessian_ii_xt",
+                              "hessian_ii_yt", "hessian_ii_tt", "hessian_ii_tx", "hessian_ii_ty",
+                              "area"})
+            n.push_back(x);
+    return n;
+}
+
+void compare_state(Particle& a, Particle& b, const std::string& tag) {
+    auto pa = a.getPositions(), pb = b.getPositions();
+    auto va = a.getVelocities(), vb = b.getVelocities();
+    long flags = (pa == pb) | (va == vb) << 1 | (a.getRadii() == b.getRadii()) << 2 |
+                 (a.getMasses() == b.getMasses()) << 3 | (a.getBoxSize() == b.getBoxSize()) << 4;
+    auto* ra = dynamic_cast<RigidBumpy*>(&a);
+    auto* rb = dynamic_cast<RigidBumpy*>(&b);
+    if (ra && rb)  // orientations and vertex rings too
+        flags &= ~(long)((ra->getAngles() != rb->getAngles() ||
+                          ra->getVertexPositions() != rb->getVertexPositions())
+                             ? 1
+                             : 0);
+    put_int((tag + ".same_state").c_str(), flags == 31);
+    put_int((tag + ".same_flags").c_str(), flags);
+}
+
+// A run written through IOManager (energy, console, state and restart groups; NVE with the
+// steps between logs run as one device loop), then read back with loadParticle from the
+// restart, init and frame sources and with load_configs.
+void scenario_io(const std::string& dir, bool bumpy) {
+    namespace fs = std::filesystem;
+    fs::path root = fs::path(dir);
+    fs::path out = root / "out";
+    ConfigDict pc;
+    if (bumpy) {
+        pc = get_standard_rigid_bumpy_config(48, 0.7);
+        pc["n_vertices_per_small_particle"] = 12;
+        pc["n_vertices_per_large_particle"] = 16;
+        pc["vertex_radius"] = RigidBumpy::calculateVertexRadius(1.0, 12, 1.0);
+        pc["n_vertices"] = 24L * 12 + 24L * 16;
+        pc["neighbor_list_config"]["neighbor_list_update_method"] = "verlet";
+    } else {
+        pc = get_standard_disk_config(96, 0.84);
+    }
+    pc["seed"] = 17;
+    auto p = createParticle(pc);
+    minimizeFire(*p, 0.1, 1e-2, 200, 1000);
+    p->setRandomVelocities(1e-3);
+    ConfigDict nve_config = get_nve_config_dict(0.01 / p->getTimeUnit());
+    NVE nve(*p, nve_config);
+    const long num_steps = 120;
+    ConfigDict console = config_from_names_lin({"step", "PE/N", "KE/N", "T"}, num_steps, 4, "console");
+    ConfigDict energy = config_from_names_lin(
+        {"step", "PE/N", "KE/N", "TE/N", "T", "phi", "P", "Zp", "PE/dof", "KE"}, num_steps, 12, "energy");
+    std::vector<std::string> state_names = {"positions", "velocities", "forces", "radii",
+                                            "kinetic_energy", "potential_energy", "contact_counts",
+                                            "stress_tensor", "static_particle_index", "box_size"};
+    if (bumpy)
+        for (const char* x : {"angles", "angular_velocities", "torques", "vertex_positions",
+                              "num_vertices_in_particle", "vertex_masses", "moments_of_inertia"})
+            state_names.push_back(x);
+    ConfigDict state_cfg = config_from_names_log(state_names, num_steps, 1, 10, "state");
+    ConfigDict restart = config_from_names_lin_everyN(with_pairs(*p, bumpy), 50, "restart");
+    {
+        IOManager io({console, energy, state_cfg, restart}, *p, &nve, out.string(), 4, true);
+        io.write_params();
+        long step = 0, pending = 0;
+        while (step < num_steps) {
+            ++pending;
+            if (io.prepare(step)) {
+                nve.run(pending);
+                pending = 0;
+                io.write(step);
+            }
+            step++;
+        }
+        if (pending) nve.run(pending);
+        io.log(step, true);
+    }
+    state(*p, "io.final");
+    put_int("io.row", p->max_neighbors_allocated);
+    auto [q, step_r] = loadParticle(out, "restart");
+    put_int("io.restart_step", step_r);
+    compare_state(*p, *q, "io.restart");
+    put("io.restart.pe", q->totalPotentialEnergy());
+    auto [r, step_i] = loadParticle(out, "init");
+    put_int("io.init_step", step_i);
+    state(*r, "io.init");
+    auto [f, step_f] = loadParticle(out, "frame", -1);
+    put_int("io.frame_step", step_f);
+    compare_state(*p, *f, "io.frame");
+    // resume through a run directory that names only the input
+    fs::path run = root / "resume";
+    fs::create_directories(run / "system");
+    ConfigDict rc;
+    rc["minimize"] = false;
+    rc["input_dir"] = out.string();
+    rc["output_dir"] = "";
+    rc.save(run / "system" / "run_config.json");
+    std::string a0 = "host_scenarios", a1 = run.string();
+    char* argv[] = {a0.data(), a1.data()};
+    auto [lp, lstep, lcon, len, lst, lrun] = load_configs(2, argv);
+    put_int("io.resume_step", lstep);
+    compare_state(*p, *lp, "io.resume");
+    put_int("io.resume_energy_group", len.at("group_name").get<std::string>() == "energy");
+    // a state written by write_restart_file and read back from a named source
+    {
+        IOManager io2({restart}, *p, nullptr, (root / "out2").string(), 1, true);
+        io2.write_params();
+        io2.write_restart_file(0, "last_state");
+    }
+    p->scaleBox(1.01);
+    long before = p->n_particles;
+    p->load(root / "out2", "last_state");
+    put_int("io.last_state_n", p->n_particles == before);
+    compare_state(*p, *q, "io.last_state");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc == 3) {
+        std::string s = argv[1];
+        try {
+            if (s == "io_disk") scenario_io(argv[2], false);
+            else if (s == "io_bumpy") scenario_io(argv[2], true);
+            else {
+                std::fprintf(stderr, "unknown scenario %s\n", s.c_str());
+                return 2;
+            }
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "scenario %s failed: %s\n", s.c_str(), e.what());
+            return 1;
+        }
+        std::printf("@ done 1\n");
+        return 0;
+    }
     if (argc != 2) {
         std::fprintf(stderr, "usage: %s <scenario>\n", argv[0]);
         return 2;
@@ -243,6 +415,7 @@ int main(int argc, char** argv) {
         else if (s == "disk_compress") scenario_disk_compress();
         else if (s == "disk_jam_fire") scenario_disk_jam_fire();
         else if (s == "bumpy_init") scenario_bumpy_init();
+        else if (s == "json") scenario_json();
         else {
             std::fprintf(stderr, "unknown scenario %s\n", s.c_str());
             return 2;
