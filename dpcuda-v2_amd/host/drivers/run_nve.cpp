@@ -34,8 +34,16 @@ static int run_directory(int argc, char** argv) {
     io.write_params();
     run_config.save(output_dir / "system" / "run_config.json");
     auto start_time = std::chrono::high_resolution_clock::now();
+    // "loop": "host" in the run config steps one NVE::step per host call, as the reference
+    const bool per_step = run_config.contains("loop") && run_config["loop"].get<std::string>() == "host";
     long pending = 0;
     while (step < num_steps) {
+        if (per_step) {
+            nve.step();
+            io.log(step);
+            step++;
+            continue;
+        }
         ++pending;  // nve.step() of this step
         if (io.prepare(step)) {
             nve.run(pending);

@@ -180,3 +180,103 @@ def test_drivers_run(argv):
                        text=True, timeout=300)
     assert p.returncode == 0, p.stderr[-2000:]
     assert "Time taken:" in p.stdout
+
+
+# ------------------------------------------------------------------ reference-form drivers
+def _log_config(names, group, freq=1, style="lin"):
+    return {"log_names": names, "save_style": style, "save_freq": freq, "reset_save_decade": 10,
+            "min_save_decade": 10, "multiple": 0, "group_name": group, "decade": 10}
+
+
+def _prepare(out_dir, run_config, particle_config=None):
+    """A run directory as the upstream tooling lays it out: system/run_config.json, and in
+    the output directory the log configs (and a particle config for a new particle)."""
+    import json
+    sysd = os.path.join(out_dir, "system")
+    os.makedirs(sysd, exist_ok=True)
+    logs = {"console": _log_config(["step", "PE/N", "KE/N", "T"], "console", 100),
+            "energy": _log_config(["step", "PE/N", "KE/N", "TE/N", "T", "phi", "P"], "energy", 50),
+            "state": _log_config(["positions", "velocities"], "state", 250)}
+    for g, c in logs.items():
+        json.dump(c, open(os.path.join(sysd, f"{g}_log_config.json"), "w"), indent=4)
+    if particle_config is not None:
+        json.dump(particle_config, open(os.path.join(sysd, "particle_config.json"), "w"), indent=4)
+    return sysd
+
+
+def _disk_particle_config(n, phi, seed):
+    return {"particle_type": "Disk", "n_particles": n, "packing_fraction": phi,
+            "particle_dim_block": 256, "e_c": 1.0, "n_c": 2.0, "mass": 1.0, "size_ratio": 1.4,
+            "count_ratio": 0.5, "seed": seed,
+            "neighbor_list_config": {"neighbor_cutoff_multiplier": 1.5,
+                                     "neighbor_displacement_multiplier": 0.2,
+                                     "num_particles_per_cell": 8.0,
+                                     "cell_displacement_multiplier": 0.5,
+                                     "neighbor_list_update_method": "cell"}}
+
+
+def _drive(name, run_dir, run_config, timeout=300):
+    import json
+    os.makedirs(os.path.join(run_dir, "system"), exist_ok=True)
+    json.dump(run_config, open(os.path.join(run_dir, "system", "run_config.json"), "w"), indent=4)
+    print("[driver]", name, run_dir, flush=True)
+    p = subprocess.run([os.path.join(DRIVERS, name), run_dir], capture_output=True, text=True,
+                       timeout=timeout)
+    assert p.returncode == 0, (name, p.stdout[-2000:], p.stderr[-2000:])
+    return p.stdout
+
+
+@pytest.mark.gpu
+def test_reference_form_drivers(tmp_path):
+    """create -> run_nve (device loop between logs and the per-step host loop give the same
+    files) -> run_fire -> run_jam -> run_jam_with_fire, each started as
+    `driver <run directory>` and writing the reference's directory layout."""
+    base = str(tmp_path)
+    src = os.path.join(base, "src")
+    _prepare(src, None, _disk_particle_config(256, 0.8, 7))
+    _drive("create", src, {"input_dir": "", "output_dir": src, "minimize": False})
+    for sub in ("init", "restart"):
+        assert os.path.exists(os.path.join(src, "system", sub, "positions.dat"))
+    outs = {}
+    for loop in ("device", "host"):
+        out = os.path.join(base, f"nve_{loop}")
+        _prepare(out, None)
+        _drive("run_nve", os.path.join(base, f"run_nve_{loop}"),
+               {"input_dir": src, "output_dir": out, "minimize": False, "num_steps": 600,
+                "dt_dimless": 0.01, "temperature": 1e-4, "loop": loop})
+        outs[loop] = out
+    for rel in ("system/energy.csv", "system/restart/positions.dat",
+                "system/restart/velocities.dat", "trajectories/t500/positions.dat"):
+        a = open(os.path.join(outs["device"], rel)).read()
+        assert a == open(os.path.join(outs["host"], rel)).read(), rel
+    rows = open(os.path.join(outs["device"], "system", "energy.csv")).read().split("\n")[1:-1]
+    assert [float(r.split(",")[0]) for r in rows] == list(range(0, 600, 50)) + [600]
+    assert open(os.path.join(outs["device"], "system", "restart", "step.dat")).read() == "600"
+    fire_out = os.path.join(base, "fire")
+    _prepare(fire_out, None)
+    _drive("run_fire", os.path.join(base, "run_fire"),
+           {"input_dir": src, "output_dir": fire_out, "minimize": False, "fire_alpha_init": 0.1,
+            "fire_dt": 0.01, "fire_num_steps": 2000, "fire_log_every_n": 1000})
+    assert os.path.exists(os.path.join(fire_out, "system", "restart", "force_pairs.dat"))
+    jam_out = os.path.join(base, "jam")
+    _prepare(jam_out, None)
+    _drive("run_jam", os.path.join(base, "run_jam"),
+           {"input_dir": src, "output_dir": jam_out, "minimize": False,
+            "num_compression_steps": 40, "compression_increment": 2e-3,
+            "avg_pe_tolerance": 1e-16, "avg_pe_diff_tolerance": 1e-16, "phi_tolerance": 1e-6,
+            "delta_phi_stability": 1e-4}, timeout=600)
+    assert os.path.exists(os.path.join(jam_out, "system", "restart", "hessian_ii_yy.dat"))
+    small = os.path.join(base, "small")
+    _prepare(small, None, _disk_particle_config(64, 0.8, 3))
+    _drive("create", small, {"input_dir": "", "output_dir": small, "minimize": False})
+    jf_out = os.path.join(base, "jam_fire")
+    _prepare(jf_out, None)
+    stdout = _drive("run_jam_with_fire", os.path.join(base, "run_jam_fire"),
+                    {"input_dir": small, "output_dir": jf_out, "minimize": False,
+                     "num_compression_steps": 300, "compression_rate": 0.99,
+                     "pressure_tolerance_low": 1e-4, "pressure_tolerance_high": 1.5e-4,
+                     "length_tolerance": 1e-12}, timeout=600)
+    assert "Time taken:" in stdout
+    assert os.path.exists(os.path.join(jf_out, "system", "last_state", "positions.dat"))
+    rows = open(os.path.join(jf_out, "system", "energy.csv")).read().split("\n")[1:-1]
+    assert len(rows) >= 2
