@@ -203,6 +203,205 @@ def rigid_fire(args, P, W):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- NVE with logging
+def nve_logged(args, P, W):
+    """Config-2 NVE run by the run_nve driver in the reference's run-directory form: the
+    FIRE-relaxed state is written here as a reference-format input directory
+    (system/{init,restart}/*.dat in %.17e, particle and log configs), loaded by
+    load_configs / loadParticle, and run with the reference's log groups (energy.csv every
+    100 steps, positions and velocities every 1000, restart every 10000).  value =
+    particle-steps/s with every log written; the same driver run with only the final log
+    gives the rate without logging."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    n = args.n
+    steps = args.steps
+    print(f"[bench] preparing N={n} run directory", file=sys.stderr, flush=True)
+    s = prepare(P, W, n, args.seed)
+    cols = {"positions": (s.download(P.F_X), s.download(P.F_Y)),
+            "velocities": (s.download(P.F_VX), s.download(P.F_VY))}
+    rad, mass = s.download(P.F_RAD), s.download(P.F_MASS)
+    side = s.get_box()[0]
+    del s
+    tmp = tempfile.mkdtemp(prefix="nve_logged_", dir=os.environ.get("TMPDIR", "/tmp"))
+    drv = os.path.join(ROOT, "dpcuda-v2_amd", "_build", "drivers", "run_nve")
+
+    def logc(names, group, freq):
+        return {"log_names": names, "save_style": "lin", "save_freq": freq,
+                "reset_save_decade": 10, "min_save_decade": 10, "multiple": 0,
+                "group_name": group, "decade": 10}
+
+    def run(tag, energy_every, state_every):
+        out = os.path.join(tmp, "out_" + tag)
+        os.makedirs(os.path.join(out, "system"))
+        for g, c in {"console": logc(["step", "PE/N", "KE/N", "T"], "console", 10**9),
+                     "energy": logc(["step", "PE/N", "KE/N", "TE/N", "T", "phi", "P", "Zp"],
+                                    "energy", energy_every),
+                     "state": logc(["positions", "velocities"], "state", state_every)}.items():
+            json.dump(c, open(os.path.join(out, "system", f"{g}_log_config.json"), "w"),
+                      indent=4)
+        rd = os.path.join(tmp, "run_" + tag)
+        os.makedirs(os.path.join(rd, "system"))
+        json.dump({"input_dir": src, "output_dir": out, "minimize": False, "num_steps": steps,
+                   "dt_dimless": 0.01, "temperature": 0.0},
+                  open(os.path.join(rd, "system", "run_config.json"), "w"), indent=4)
+        print(f"[bench] run_nve {tag} ({steps} steps)", file=sys.stderr, flush=True)
+        p = subprocess.run([drv, rd], capture_output=True, text=True, timeout=900)
+        if p.returncode != 0:
+            raise SystemExit("run_nve failed: " + p.stderr[-2000:])
+        line = [l for l in p.stdout.splitlines() if l.startswith("particle-steps/s")][-1]
+        frames = len(os.listdir(os.path.join(out, "trajectories")))
+        rows = len(open(os.path.join(out, "system", "energy.csv")).read().splitlines()) - 1
+        mb = sum(os.path.getsize(os.path.join(dp, f))
+                 for dp, _, fs in os.walk(out) for f in fs) / 1e6
+        return (float(line.split()[1]), float(line.split("(logging included, ")[1].split()[0]),
+                frames, rows, mb)
+
+    try:
+        src = os.path.join(tmp, "src")
+        for sub in ("init", "restart"):
+            d = os.path.join(src, "system", sub)
+            os.makedirs(d)
+            for name, (a, b) in cols.items():
+                np.savetxt(os.path.join(d, name + ".dat"), np.column_stack([a, b]),
+                           fmt="%.17e", delimiter="\t")
+            np.savetxt(os.path.join(d, "radii.dat"), rad, fmt="%.17e")
+            np.savetxt(os.path.join(d, "masses.dat"), mass, fmt="%.17e")
+            np.savetxt(os.path.join(d, "box_size.dat"), [side, side], fmt="%.17e")
+            with open(os.path.join(d, "step.dat"), "w") as f:
+                f.write("0")
+        pc = {"particle_type": "Disk", "n_particles": n, "packing_fraction": 0.85,
+              "particle_dim_block": 256, "e_c": 1.0, "n_c": 2.0, "mass": 1.0,
+              "size_ratio": 1.4, "count_ratio": 0.5, "seed": args.seed,
+              "neighbor_list_config": {"neighbor_cutoff_multiplier": 1.5,
+                                       "neighbor_displacement_multiplier": 0.2,
+                                       "num_particles_per_cell": 8.0,
+                                       "cell_displacement_multiplier": 0.5,
+                                       "neighbor_list_update_method": "cell"}}
+        json.dump(pc, open(os.path.join(src, "system", "particle_config.json"), "w"), indent=4)
+        quiet = run("final_log_only", 10**9, 10**9)
+        logged = run("logged", 100, 1000)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    value, secs, frames, rows, mb = logged
+    print(json.dumps({
+        "metric": "FP64 NVE particle-steps/s at N=262,144 with the reference's logs written",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": secs / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (config-2 state written as a reference-format input directory)",
+        "config": {
+            "workload": "config-2 disks (N=262144, phi=0.85, FIRE-relaxed) NVE dt_dimless "
+                        "0.01 through run_nve <run directory> with the standard neighbour "
+                        "config (cutoff 1.5 d_max, cell list): energy.csv every 100 steps "
+                        "(PE/N, KE/N, TE/N, T, phi, P, Zp), positions + velocities every "
+                        "1000, restart every 10000, final forced log",
+            "n_particles": n,
+            "final_log_only_particle_steps_per_s": quiet[0],
+            "logging_overhead_fraction": 1 - value / quiet[0],
+            "trajectory_frames": frames,
+            "energy_rows": rows,
+            "megabytes_written": mb,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }))
+
+
+# ---------------------------------------------------------------- pair analytics
+def pairs(args, P, W):
+    """Per-pair analytics (dpm_pair_analytics: energies, forces, separations, ids, overlaps,
+    angles, 2x2 Hessian blocks for every neighbour-list slot, in the reference's layout) on
+    the config-2 state.  value = neighbour-list slots per second; the kernel writes every
+    slot of every field, so it is bound by HBM writes."""
+    import numpy as np
+    n = args.n
+    print(f"[bench] preparing N={n} for pair analytics", file=sys.stderr, flush=True)
+    s = prepare(P, W, n, args.seed)
+    s.nve_run(0.01, 200)
+    import ctypes
+    r = ctypes.c_int64(0)
+    for _ in range(3):
+        s._c("pair_analytics", ctypes.byref(r))
+    row = r.value
+    st = s.neighbor_stats()
+    used = st["total_entries"]
+    k = args.steps if args.steps < 5000 else 200
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        s._c("pair_analytics", ctypes.byref(r))
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    slots = n * row
+    # algorithmic bytes: 18 fields x 8 B written per slot (fill values included, as the
+    # reference lays them out); per used slot the neighbour index (4), P0[j] (32) and its
+    # static id (4); per row nnum (4), P0[i] (32) and sidx[i] (4)
+    alg = slots * 144.0 + used * 40.0 + n * 40.0
+    us = dt / k * 1e6
+    gbs = alg / (us * 1e-6) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
+        if os.path.exists(oracle_so):
+            be = P.Backend(oracle_so, "orc_", gpu=False)
+            so = prepare(P, W, n, args.seed, backend=be)
+            so.nve_run(0.01, 200)
+            kc = 3
+            c0 = time.perf_counter()
+            for _ in range(kc):
+                so._c("pair_analytics", ctypes.byref(r))
+            cdt = time.perf_counter() - c0
+            cpu = {"value": slots * kc / cdt, "unit": "pair-slots/s", "cores": 1,
+                   "kind": "port",
+                   "sample": f"{kc} full pair-analytics passes of the same N={n} state, "
+                             f"C++ oracle (one thread), {cdt:.2f} s"}
+    out = {
+        "metric": "FP64 per-pair analytics (force/distance/Hessian pairs) neighbour slots/s at N=262,144",
+        "value": slots / (dt / k),
+        "unit": "pair-slots/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": 3,
+        "ms_per_step": dt / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (config-2 state after 200 FIRE and 200 NVE steps)",
+        "config": {
+            "workload": "Disk::calculateForceDistancePairs on config 2: N=262144 bidisperse "
+                        "disks, phi=0.85, cell list rc=1.7; 18 per-slot fields",
+            "n_particles": n,
+            "row_length": row,
+            "used_slots": used,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_pairs_disk",
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": alg,
+            "call_us": us,
+            "note": "call time includes the launch and one stream synchronisation",
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 # ---------------------------------------------------------------- batched jamming
 def jam_batch(args, P, W):
     """Jamming of many independent disk systems of uneven N: the batched engine against
@@ -464,7 +663,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1 << 18)
     ap.add_argument("--serve-budget", type=int, default=16 << 30)
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch", "serve"),
+    ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch", "serve",
+                                           "pairs", "nve-logged"),
                     default="nve-disk",
                     help="nve-disk: the headline NVE benchmark (default); fire-rigid: FIRE "
                          "of N=65,536 rigid bumpy particles to |F|max < 1e-12")
@@ -502,6 +702,12 @@ def main():
         return
     if args.workload == "serve":
         serve(args, P, W)
+        return
+    if args.workload == "pairs":
+        pairs(args, P, W)
+        return
+    if args.workload == "nve-logged":
+        nve_logged(args, P, W)
         return
     n = args.n
     print(f"[bench] preparing N={n} (FIRE 200 steps, T=1e-5)", file=sys.stderr, flush=True)

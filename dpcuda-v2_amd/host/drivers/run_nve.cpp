@@ -30,9 +30,12 @@ static int run_directory(int argc, char** argv) {
     std::vector<ConfigDict> log_group_configs = {
         console_config, energy_config, state_config,
         config_from_names_lin_everyN(init_names, 10000, "restart")};
-    IOManager io(log_group_configs, *particle, &nve, output_dir.string(), 20, overwrite);
+    auto io_ptr = std::make_unique<IOManager>(log_group_configs, *particle, &nve,
+                                              output_dir.string(), 20, overwrite);
+    IOManager& io = *io_ptr;
     io.write_params();
     run_config.save(output_dir / "system" / "run_config.json");
+    const long first_step = step;
     auto start_time = std::chrono::high_resolution_clock::now();
     // "loop": "host" in the run config steps one NVE::step per host call, as the reference
     const bool per_step = run_config.contains("loop") && run_config["loop"].get<std::string>() == "host";
@@ -54,10 +57,14 @@ static int run_directory(int argc, char** argv) {
     }
     if (pending) nve.run(pending);
     io.log(step, true);
+    io_ptr.reset();  // queued state writes finish here
     auto end_time = std::chrono::high_resolution_clock::now();
     std::cout << "Time taken: "
               << std::chrono::duration_cast<std::chrono::seconds>(end_time - start_time).count()
               << " seconds" << std::endl;
+    double secs = std::chrono::duration<double>(end_time - start_time).count();
+    std::cout << "particle-steps/s " << particle->n_particles * (num_steps - first_step) / secs
+              << " (logging included, " << secs << " s)" << std::endl;
     return 0;
 }
 
