@@ -223,7 +223,7 @@ def nve_logged(args, P, W):
     cols = {"positions": (s.download(P.F_X), s.download(P.F_Y)),
             "velocities": (s.download(P.F_VX), s.download(P.F_VY))}
     rad, mass = s.download(P.F_RAD), s.download(P.F_MASS)
-    side = s.get_box()[0]
+    side = s.box()[0]
     del s
     tmp = tempfile.mkdtemp(prefix="nve_logged_", dir=os.environ.get("TMPDIR", "/tmp"))
     drv = os.path.join(ROOT, "dpcuda-v2_amd", "_build", "drivers", "run_nve")

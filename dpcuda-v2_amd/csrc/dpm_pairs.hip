@@ -54,75 +54,94 @@ __device__ void fill_slot(double* out, int64_t S, int64_t t, int bumpy) {
     for (int f = DPM_P_H_XT; f <= DPM_P_HII_TT; ++f) fld(out, S, f, t) = 0.0;
 }
 
-// kernelCalcDiskForceDistancePairs, disk/kernels.cu:73-175
+// slot t -> (row i, entry k); rows are a power of two long except for all-pairs lists
+__device__ __forceinline__ int64_t row_of(int64_t t, int64_t row, int shift) {
+    return shift >= 0 ? (t >> shift) : t / row;
+}
+
+// kernelCalcDiskForceDistancePairs, disk/kernels.cu:73-175.  Every field of the slot is
+// formed in registers (the reference's fill value where the slot is unused) and stored
+// once.
 __global__ __launch_bounds__(BLOCK) void k_pairs_disk(Dev d, PairParams p, double* out,
-                                                      int64_t row, int64_t S) {
+                                                      int64_t row, int shift, int64_t S) {
     int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S) return;
-    int64_t i = t / row;
+    int64_t i = row_of(t, row, shift);
     int k = (int)(t - i * row);
     int nn = min(d.nnum[i], d.cap);
     int j = k < nn ? d.nbr[(int64_t)k * d.N + i] : -1;
-    fill_slot(out, S, t, 0);
-    if (j < 0 || j == i) return;
-    double4 qi = d.P0[i], qj = d.P0[j];
-    double dx = pbc_div(qi.x, qj.x, p.Lx);
-    double dy = pbc_div(qi.y, qj.y, p.Ly);
-    double fx, fy;
-    double e = pair_core(dx, dy, qi.z, qj.z, p, fx, fy);
+    double e = -1.0, fx = 0.0, fy = 0.0, dx = -1.0, dy = -1.0, ovl = -1.0, rsum = -1.0,
+           ang = -1.0;
+    long long id_i = -1, id_j = -1;
+    double hij[4] = {0.0, 0.0, 0.0, 0.0}, hii[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j >= 0 && j != i) {
+        double4 qi = d.P0[i], qj = d.P0[j];
+        dx = pbc_div(qi.x, qj.x, p.Lx);
+        dy = pbc_div(qi.y, qj.y, p.Ly);
+        e = pair_core(dx, dy, qi.z, qj.z, p, fx, fy);
+        id_i = d.sidx[i];
+        id_j = d.sidx[j];
+        double dist = sqrt(dx * dx + dy * dy);
+        ovl = dist - (qi.z + qj.z);
+        rsum = qi.z + qj.z;
+        ang = atan2(dy, dx);
+        if (dist < qi.z + qj.z) {
+            double radsum = qi.z + qj.z;
+            double t_ij = -p.e_c / radsum * (1 - dist / radsum);
+            double c_ij = p.e_c / (radsum * radsum);
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    double d_ia_dx = 1 * (a == 0);
+                    double d_ia_dy = 1 * (a == 1);
+                    double d_jb_dx = -1 * (b == 0);
+                    double d_jb_dy = -1 * (b == 1);
+                    double d_ia_d_jb_dx = 0;
+                    double d_ia_d_jb_dy = 0;
+                    double q_ia = dx * d_ia_dx + dy * d_ia_dy;
+                    double q_jb = dx * d_jb_dx + dy * d_jb_dy;
+                    double d_ia_q_jb = d_ia_dx * d_jb_dx + dx * d_ia_d_jb_dx + d_ia_dy * d_jb_dy +
+                                       dy * d_ia_d_jb_dy;
+                    hij[a * 2 + b] += c_ij * (q_ia * q_jb) / (dist * dist) +
+                                      t_ij * (d_ia_q_jb / dist - (q_ia * q_jb) / (dist * dist * dist));
+                    double d_ib_dx = 1 * (b == 0);
+                    double d_ib_dy = 1 * (b == 1);
+                    double d_ia_d_ib_dx = 0;
+                    double d_ia_d_ib_dy = 0;
+                    double q_ib = dx * d_ib_dx + dy * d_ib_dy;
+                    double d_ia_q_ib = d_ia_dx * d_ib_dx + dx * d_ia_d_ib_dx + d_ia_dy * d_ib_dy +
+                                       dy * d_ia_d_ib_dy;
+                    hii[a * 2 + b] += c_ij * (q_ia * q_ib) / (dist * dist) +
+                                      t_ij * (d_ia_q_ib / dist - (q_ia * q_ib) / (dist * dist * dist));
+                }
+            for (int c = 0; c < 4; ++c) {
+                hij[c] = 0.0 + hij[c];
+                hii[c] = 0.0 + hii[c];
+            }
+        }
+    }
     fld(out, S, DPM_P_POTENTIAL, t) = e;
     fld(out, S, DPM_P_FORCE_X, t) = fx;
     fld(out, S, DPM_P_FORCE_Y, t) = fy;
     fld(out, S, DPM_P_DIST_X, t) = dx;
     fld(out, S, DPM_P_DIST_Y, t) = dy;
-    put_id(out, S, DPM_P_ID_I, t, d.sidx[i]);
-    put_id(out, S, DPM_P_ID_J, t, d.sidx[j]);
-    double dist = sqrt(dx * dx + dy * dy);
-    fld(out, S, DPM_P_OVERLAP, t) = dist - (qi.z + qj.z);
-    fld(out, S, DPM_P_RADSUM, t) = qi.z + qj.z;
-    fld(out, S, DPM_P_ANGLE, t) = atan2(dy, dx);
-    if (dist < qi.z + qj.z) {
-        double radsum = qi.z + qj.z;
-        double t_ij = -p.e_c / radsum * (1 - dist / radsum);
-        double c_ij = p.e_c / (radsum * radsum);
-        double hij[4] = {0.0, 0.0, 0.0, 0.0}, hii[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) {
-                double d_ia_dx = 1 * (a == 0);
-                double d_ia_dy = 1 * (a == 1);
-                double d_jb_dx = -1 * (b == 0);
-                double d_jb_dy = -1 * (b == 1);
-                double d_ia_d_jb_dx = 0;
-                double d_ia_d_jb_dy = 0;
-                double q_ia = dx * d_ia_dx + dy * d_ia_dy;
-                double q_jb = dx * d_jb_dx + dy * d_jb_dy;
-                double d_ia_q_jb =
-                    d_ia_dx * d_jb_dx + dx * d_ia_d_jb_dx + d_ia_dy * d_jb_dy + dy * d_ia_d_jb_dy;
-                hij[a * 2 + b] += c_ij * (q_ia * q_jb) / (dist * dist) +
-                                  t_ij * (d_ia_q_jb / dist - (q_ia * q_jb) / (dist * dist * dist));
-                double d_ib_dx = 1 * (b == 0);
-                double d_ib_dy = 1 * (b == 1);
-                double d_ia_d_ib_dx = 0;
-                double d_ia_d_ib_dy = 0;
-                double q_ib = dx * d_ib_dx + dy * d_ib_dy;
-                double d_ia_q_ib =
-                    d_ia_dx * d_ib_dx + dx * d_ia_d_ib_dx + d_ia_dy * d_ib_dy + dy * d_ia_d_ib_dy;
-                hii[a * 2 + b] += c_ij * (q_ia * q_ib) / (dist * dist) +
-                                  t_ij * (d_ia_q_ib / dist - (q_ia * q_ib) / (dist * dist * dist));
-            }
-        for (int c = 0; c < 4; ++c) {
-            fld(out, S, DPM_P_H_XX + c, t) = 0.0 + hij[c];
-            fld(out, S, DPM_P_HII_XX + c, t) = 0.0 + hii[c];
-        }
+    put_id(out, S, DPM_P_ID_I, t, id_i);
+    put_id(out, S, DPM_P_ID_J, t, id_j);
+    fld(out, S, DPM_P_OVERLAP, t) = ovl;
+    fld(out, S, DPM_P_RADSUM, t) = rsum;
+    fld(out, S, DPM_P_ANGLE, t) = ang;
+    for (int c = 0; c < 4; ++c) {
+        fld(out, S, DPM_P_H_XX + c, t) = hij[c];
+        fld(out, S, DPM_P_HII_XX + c, t) = hii[c];
     }
 }
 
 // kernelCalcRigidBumpyForceDistancePairs, rigid_bumpy/kernels.cu:482-702
 __global__ __launch_bounds__(BLOCK) void k_pairs_bumpy(Dev d, PairParams p, double rv,
-                                                       double* out, int64_t row, int64_t S) {
+                                                       double* out, int64_t row, int shift,
+                                                       int64_t S) {
     int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S) return;
-    int64_t i = t / row;
+    int64_t i = row_of(t, row, shift);
     int k = (int)(t - i * row);
     int nn = min(d.nnum[i], d.cap);
     int j = k < nn ? d.nbr[(int64_t)k * d.N + i] : -1;
@@ -304,11 +323,16 @@ int dpm_pair_analytics(dpm_system* s, int64_t* row_length) {
     PairParams p = params_of(s);
     p.fast = 0;  // the reference's minimum image for every slot
     dim3 grid((unsigned)std::max<int64_t>(1, (S + BLOCK - 1) / BLOCK));
+    int shift = -1;
+    if ((row & (row - 1)) == 0)
+        for (shift = 0; ((int64_t)1 << shift) < row; ++shift) {
+        }
     if (s->kind == DPM_BUMPY)
         hipLaunchKernelGGL(k_pairs_bumpy, grid, dim3(BLOCK), 0, s->stream, s->d, p, s->rv, s->pairs,
-                           row, S);
+                           row, shift, S);
     else
-        hipLaunchKernelGGL(k_pairs_disk, grid, dim3(BLOCK), 0, s->stream, s->d, p, s->pairs, row, S);
+        hipLaunchKernelGGL(k_pairs_disk, grid, dim3(BLOCK), 0, s->stream, s->d, p, s->pairs, row,
+                           shift, S);
     if (check_launch()) return -1;
     DPM_HIP(hipStreamSynchronize(s->stream));
     s->pair_row = row;

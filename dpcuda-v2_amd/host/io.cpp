@@ -393,10 +393,7 @@ std::unique_ptr<BaseLogGroup> ConsoleLog::snapshot() const {
 StateLog::StateLog(ConfigDict c, Orchestrator& o, const std::string& r, const std::string& prefix,
                    const std::string& ext, bool restart)
     : BaseLogGroup(c, o), root(r), indexed_file_prefix(prefix), extension(ext), is_restart(restart) {
-    // trajectory frames go to a directory per step and may be written by the workers; the
-    // restart group rewrites one directory, so it is written in call order on the caller's
-    // thread (two queued restart writes could otherwise interleave their files)
-    parallel = !is_restart;
+    parallel = true;
 }
 
 void StateLog::gather_data(long step) {
@@ -427,8 +424,18 @@ void StateLog::log(long step) {
                                     ? std::filesystem::path(root) / "restart"
                                     : std::filesystem::path(root) /
                                           (indexed_file_prefix + std::to_string(step));
-    make_dir(dir.string(), true);
-    write_values(dir, step);
+    if (is_restart) {
+        // queued restart writes may run on several workers: one at a time, newest wins
+        std::lock_guard<std::mutex> g(order_->mu);
+        if (step >= order_->last) {
+            make_dir(dir.string(), true);
+            write_values(dir, step);
+            order_->last = step;
+        }
+    } else {
+        make_dir(dir.string(), true);
+        write_values(dir, step);
+    }
     gathered_data.clear();
     reorder_index_data.clear();
 }

@@ -28,6 +28,7 @@
 #include <functional>
 #include <iomanip>
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -205,6 +206,15 @@ class StateLog : public BaseLogGroup {
     std::unordered_map<std::string, ArrayData> gathered_data;
     std::unordered_map<std::string, ArrayData> reorder_index_data;
     bool is_restart;
+
+   private:
+    // shared by a restart group and its queued snapshots: restart writes into the one
+    // restart directory one at a time, and a write older than the last one is dropped
+    struct RestartOrder {
+        std::mutex mu;
+        long last = std::numeric_limits<long>::min();
+    };
+    std::shared_ptr<RestartOrder> order_ = std::make_shared<RestartOrder>();
 };
 
 // a fixed set of worker threads; shutdown() (also the destructor) runs every queued task
