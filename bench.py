@@ -662,7 +662,7 @@ def main():
     ap.add_argument("--serve-systems", type=int, default=1024)
     ap.add_argument("--serve-slots", type=int, default=1 << 18)
     ap.add_argument("--serve-budget", type=int, default=16 << 30)
-    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch", "serve",
                                            "pairs", "nve-logged"),
                     default="nve-disk",
