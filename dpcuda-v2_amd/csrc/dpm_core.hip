@@ -266,7 +266,10 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
         if (blockIdx.x == 0 && threadIdx.x == 0) c->n_rebuilds += 1;
     }
     long long st = last ? stamp(e + 1, 0) : stamp(e, pos + 1);
-    disp_epilogue(d, nb, tile, nd, cd, (pos + 1) & 1, st);
+    if (last)  // the maxima of the chunk's final state, for the checks and queries after it
+        disp_epilogue(d, nb, tile, nd, cd, (pos + 1) & 1, st);
+    else
+        disp_flags(d, nb, nd, cd, (pos + 1) & 1, st);
     if (blockIdx.x == 0 && threadIdx.x == 0) c->progress = pos + 1;
 }
 

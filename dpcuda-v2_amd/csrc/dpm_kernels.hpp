@@ -78,6 +78,23 @@ __device__ __forceinline__ void disp_epilogue(const Dev& d, const NbrParams& nb,
     }
 }
 
+// The rebuild decision of disp_epilogue without the block maxima: 3 max d^2 > thr holds
+// exactly when 3 d^2 > thr holds for some particle (3x is monotone under rounding), so a
+// wave ballot per test replaces the block reduction.  For the steps inside a chunk, whose
+// maxima nothing reads; the partials are written at the end of the chunk.
+__device__ __forceinline__ void disp_flags(const Dev& d, const NbrParams& nb, double nd,
+                                           double cd, int wr, long long st) {
+    if (nb.method != DPM_NBR_CELL && nb.method != DPM_NBR_VERLET) return;
+    const bool cf = nb.method == DPM_NBR_CELL && 3.0 * cd > nb.cthr;
+    const bool lf = 3.0 * nd > nb.thr;
+    const uint64_t bc = __builtin_amdgcn_ballot_w64(cf);
+    const uint64_t bl = __builtin_amdgcn_ballot_w64(lf);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        if (bc) d.ctl->cflag[wr] = st;
+        if (bl) d.ctl->flag[wr] = st;
+    }
+}
+
 __device__ __forceinline__ void disp_of(const Dev& d, int64_t i, double x, double y, double& nd,
                                         double& cd) {
     double2 ln = d.lref[i], lc = d.cref[i];
