@@ -472,7 +472,8 @@ __global__ __launch_bounds__(BLOCK) void k_bfire_move(Dev d, NbrParams nb, int p
         d.omega[i] = w;
         disp_of(d, i, nx, ny, nd, cd);
     }
-    disp_epilogue(d, nb, tile, nd, cd, pos & 1, stamp(e, pos));
+    (void)tile;
+    disp_flags(d, nb, nd, cd, pos & 1, stamp(e, pos));  // maxima: k_disp_partials at the end
 }
 
 // kernelTranslateAndRotateVertices1 (rigid_bumpy/kernels.cu:152-201) for the back-step

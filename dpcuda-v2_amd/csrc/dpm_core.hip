@@ -886,6 +886,21 @@ __global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, NbrParams nb, int
     }
 }
 
+// The per-block displacement maxima of the current positions (P0) against the list and
+// cell reference positions: what disp_epilogue leaves behind, rebuilt after a loop whose
+// steps decide their rebuilds by wave ballots (disp_flags), for the checks and queries
+// that follow it.
+__global__ __launch_bounds__(BLOCK) void k_disp_partials(Dev d, NbrParams nb) {
+    int tile;
+    const int64_t i = particle_of_thread(tile);
+    double nd = 0.0, cd = 0.0;
+    if (i < d.N) {
+        double4 q = d.P0[i];
+        disp_of(d, i, q.x, q.y, nd, cd);
+    }
+    disp_epilogue(d, nb, tile, nd, cd, -1, 0);
+}
+
 // ------------------------------------------------------------- rebuild chain
 // mode 0: end of a chunk (acts only if a step halted); 1: check (decide from the
 // current displacement maxima); 2: forced full rebuild (initialisation, restore).
@@ -1027,7 +1042,8 @@ __global__ __launch_bounds__(BLOCK) void k_fire_move(Dev d, NbrParams nb, int po
         d.vel[i] = v;
         disp_of(d, i, q.x, q.y, nd, cd);
     }
-    disp_epilogue(d, nb, tile, nd, cd, pos & 1, stamp(e, pos));
+    (void)tile;
+    disp_flags(d, nb, nd, cd, pos & 1, stamp(e, pos));  // maxima: k_disp_partials at the end
 }
 
 // kick: 0 = forces only (minimizeFire's initial evaluation), 1 = + closing kick
@@ -1784,6 +1800,7 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
         run_chunks(s, prm->max_steps, 0.0, rigid ? 3 : 2, rigid ? bumpy_fire_chunk : fire_chunk,
                    s->fire_graphs, true, false))
         return -1;
+    hipLaunchKernelGGL(k_disp_partials, pgrid(s->N), dim3(BLOCK), 0, st, d, nbr_params(s));
     hipLaunchKernelGGL(k_zero_velocities, pgrid(s->N), dim3(BLOCK), 0, st, d);
     if (scopy(s, &f, d.fire, sizeof(FireCtl), hipMemcpyDeviceToHost)) return -1;
     if (out) {
