@@ -636,7 +636,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--systems", type=int, default=4096)
     ap.add_argument("--jam-phi0", type=float, default=0.80)
     ap.add_argument("--jam-pressure", type=float, default=1e-7)

@@ -165,7 +165,7 @@ struct dpm_system {
     double snap_box[2] = {0, 0};
     // execution options
     bool use_graph = true;
-    int64_t chunk = 64;
+    int64_t chunk = 32;
     int timing = 0;             // 0 off, 1 per launch, 2 per chunk
     uint64_t gen = 0;           // bumped whenever baked kernel arguments change
     std::map<int64_t, dpm::Graph> nve_graphs, fire_graphs;

@@ -290,7 +290,7 @@ int dpm_server_result(dpm_server* s, int64_t id, int32_t* state, dpm_jam_result*
 
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
- * DPM_OPT_CHUNK: steps per host poll of the device status (default 64).
+ * DPM_OPT_CHUNK: steps per host poll of the device status (default 32).
  * DPM_OPT_TIMING: 0 off (default); 1 HIP events around every launch of the hot kernels
  *   (disables graph capture); 2 HIP events around every graph-launched chunk of the NVE and
  *   FIRE loops (chunk time is charged to DPM_K_FORCE, recovery chains to DPM_K_REBUILD).
