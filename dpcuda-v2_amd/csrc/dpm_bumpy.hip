@@ -569,8 +569,22 @@ __device__ __forceinline__ int first_particle_from(const Dev& d, int64_t v) {
 
 template <int LDSV, bool HOT>
 __global__ __launch_bounds__(BF_TILE) void k_bfire_forces(Dev d, PairParams p, double rv,
-                                                          int kick, int gate, int rotation) {
+                                                          int kick, int gate, int rotation,
+                                                          int check_pos) {
     if (bfire_skip(d, gate)) return;
+    if (check_pos >= 0) {  // k_bfire_check's test, made by every block (same words read)
+        Ctl* c = d.ctl;
+        if (c->kind >= 1 && (c->max_nnum > d.cap || c->max_vnum > d.vcap)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                c->overflow = 1;
+                c->halt = 1;
+                c->halt_pos = check_pos;
+                c->halt_mode = H_FIRE;
+                c->halt_kind = 1;
+            }
+            return;
+        }
+    }
     __shared__ double sv[4][LDSV];       // per vertex: fx, fy, torque, energy
     __shared__ double sres[4][BF_TILE];  // per particle: the sums of those
     const int64_t b0 = (int64_t)blockIdx.x * BF_TILE;
@@ -842,7 +856,7 @@ int bumpy_nve_run(dpm_system* s, double dt, int64_t n) {
 
 
 // ---- device-resident FIRE (the loop driver is minimize_fire in dpm_core.hip) ----
-void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate) {
+void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate, int check_pos) {
     PairParams p = params_of(s);
     if (BF_TILE + s->nv_max - 1 <= BF_LDS) {
         unsigned nb = (unsigned)((s->Nv + BF_TILE - 1) / BF_TILE);
@@ -851,11 +865,13 @@ void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate) {
                               : k_bfire_forces<BF_LDS_SMALL, false>)
                        : (hot ? k_bfire_forces<BF_LDS, true> : k_bfire_forces<BF_LDS, false>);
         hipLaunchKernelGGL(k, dim3(nb), dim3(BF_TILE), 0, s->stream, s->d, p, s->rv, kick, gate,
-                           (int)s->rotation);
+                           (int)s->rotation, check_pos);
         hipLaunchKernelGGL(k_bfire_partials, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream,
                            s->d, gate);
         return;
     }
+    if (check_pos >= 0)
+        hipLaunchKernelGGL(k_bfire_check, dim3(1), dim3(1), 0, s->stream, s->d, check_pos);
     hipLaunchKernelGGL(k_bfire_vforce, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
                        gate);
     hipLaunchKernelGGL(k_bfire_reduce, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream, s->d,
@@ -877,10 +893,11 @@ void bumpy_enqueue_fire_step(dpm_system* s, int pos) {
         hipLaunchKernelGGL(k_bfire_decide, dim3(1), dim3(1), 0, st, d, pos);
         enqueue_list_build(s, 1);
         bumpy_enqueue_vertex_build(s, 1);
-        hipLaunchKernelGGL(k_bfire_check, dim3(1), dim3(1), 0, st, d, pos);
     }
     Timed t(s, DPM_K_FORCE);
-    bumpy_enqueue_fire_forces(s, 1, G_NORMAL);
+    // the overflow test of an in-step rebuild (k_bfire_check) runs in the force kernel
+    bumpy_enqueue_fire_forces(s, 1, G_NORMAL,
+                              s->method == DPM_NBR_CELL || s->method == DPM_NBR_VERLET ? pos : -1);
 }
 
 

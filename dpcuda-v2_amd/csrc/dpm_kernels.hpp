@@ -187,7 +187,7 @@ int bumpy_mix(dpm_system* s, double alpha);
 int bumpy_stress(dpm_system* s);
 int bumpy_contacts(dpm_system* s);
 int bumpy_nve_run(dpm_system* s, double dt, int64_t n);
-void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate);
+void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate, int check_pos = -1);
 void bumpy_enqueue_fire_step(dpm_system* s, int pos);
 int bumpy_snapshot(dpm_system* s, bool save);
 int bumpy_download(dpm_system* s, int field, void* data, int64_t count,
