@@ -162,8 +162,10 @@ __global__ __launch_bounds__(BLOCK) void k_b_prep(BDev b) {
         }
     }
     if (!start) return;
-    double m = wave_max(d2);
-    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = m;
+    // 3 max d^2 > thr holds exactly when 3 d^2 > thr holds for some slot (3x is monotone
+    // under rounding): one ballot per wave replaces the wave maximum
+    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > b.thr2) != 0;
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
 }
 
 // Fire::step first half (src/integrator/fire.cu:14-40): [uphill back-step with the
@@ -200,8 +202,8 @@ __global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
         double dx = q.x - l.x, dy = q.y - l.y;
         d2 = dx * dx + dy * dy;
     }
-    double m = wave_max(d2);
-    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = m;
+    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > b.thr2) != 0;
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
 }
 
 // Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
@@ -214,10 +216,12 @@ __global__ void k_b_decide(BDev b) {
     S.snap_op = 0;
     S.scale_f = 0.0;
     if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
-    const double* D = bpart(b, BP_DMAX) + S.off / WAVE;
-    double m = 0.0;
-    for (int c = 0; c < S.nc; ++c) m = fmax(m, D[c]);
-    if (3.0 * m > b.thr2) S.rebuild = 1;
+    const double* D = bpart(b, BP_DMAX) + S.off / WAVE;  // per wave: some 3 d^2 > thr
+    for (int c = 0; c < S.nc; ++c)
+        if (D[c] != 0.0) {
+            S.rebuild = 1;
+            break;
+        }
 }
 
 // kernelUpdateNeighborList (base/kernels.cu:256-293) for the systems that need it:

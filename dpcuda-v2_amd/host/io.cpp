@@ -457,7 +457,11 @@ ThreadPool::ThreadPool(long num_threads) {
                     task = std::move(tasks_.front());
                     tasks_.pop_front();
                 }
-                task();
+                try {
+                    task();
+                } catch (const std::exception& e) {  // a failed write must not end the process
+                    std::cerr << "ERROR: IOManager worker: " << e.what() << std::endl;
+                }
             }
         });
 }
