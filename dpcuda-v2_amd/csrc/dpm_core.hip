@@ -133,7 +133,7 @@ __device__ __forceinline__ void row_forces(const Dev& d, const double4* P, const
 // the caller ahead of everything that waits on the control block, batches of ROW_PF
 // records gathered together, and the wave's slot bound taken from ballots instead of
 // a shuffle reduction.  Same slots, same order, same arithmetic as row_forces.
-constexpr int ROW_PF = 4;
+constexpr int ROW_PF = 2;
 
 __device__ __forceinline__ bool wave_any(bool x) {
     return __builtin_amdgcn_ballot_w64(x) != 0;
