@@ -688,8 +688,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of "
-              "it (systems shard by independent systems or spatial slabs, see DESIGN.md).",
+        print("bench.py: measures one GPU only; runs on several GPUs are not supported "
+              "(systems would shard by independent systems or by spatial slabs).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -5,7 +5,7 @@ Vertex rings and rigid-body rotations use one shared sin/cos (dpm_sincos) on bot
 so set-up, vertex and particle pair sets, forces, torques, energies, NVE and FIRE
 trajectories, stress without spin and contacts agree bit for bit.  The stress tensor of
 spinning particles goes through the library atan2/sin/cos and is held to 1e-12.
-The survey's bars (forces 1e-12, NVE 1e-9 over 100 steps, FIRE final E, P, phi 1e-8)
+The parity bars (forces 1e-12, NVE 1e-9 over 100 steps, FIRE final E, P, phi 1e-8)
 are therefore met with margin.
 """
 import numpy as np
