@@ -406,14 +406,14 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
 // calculateForces) as a chain of launches that a graph replays; each kernel reads the
 // control block, so a halted or finished loop turns the rest of the chunk into no-ops:
 //   k_bfire_move      particles: [uphill: drift(-dt/2), zero v and w] + kick(dt/2) + mix
-//                     + drift(dt); records both rigid moves and the displacement maxima
+//                     + drift(dt), and the rebuild requests of the new positions
 //   k_bfire_vertices  vertices: the rotations of those moves, in the same order
 //   k_bfire_decide    the neighbour check of the step: a list rebuild runs inside the
 //                     chunk, a cell rebuild halts the loop until the chunk's end
 //   k_build_cell / k_build_verlet, k_bumpy_vertex_build   (gated on that decision)
-//   k_bfire_check     an in-chunk rebuild that overflowed the capacity halts the loop
-//   k_bfire_forces    vertex forces, torques, energies (kernelCalcRigidBumpyForces1) and
-//                     their per-particle sums (kernelCalcRigidBumpyParticleForces1) +
+//   k_bfire_forces    an in-chunk rebuild that overflowed the capacity halts the loop;
+//                     else vertex forces, torques, energies (kernelCalcRigidBumpyForces1)
+//                     and their per-particle sums (kernelCalcRigidBumpyParticleForces1) +
 //                     kick(dt/2), one pass
 //   k_bfire_partials  chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2)
 //   k_fire_ctl        stop test of the step and Fire::step's decision for the next
@@ -424,6 +424,78 @@ __device__ __forceinline__ bool bfire_skip(const Dev& d, int gate) {
     return false;
 }
 
+// The rigid moves of Fire::step for particle i (src/integrator/fire.cu:20-40 with the
+// rigid overrides): [uphill back-step with the decreased dt, zero v and omega] + kick
+// (dt/2) + mix + drift(dt).  Returns the old centre, the centre after the back-step, the
+// two rotation angles and the displacements of the new centre from the list / cell
+// references.
+struct BMove {
+    double2 old, mid, now;
+    double dth0, dth;
+};
+__device__ __forceinline__ BMove bfire_move_one(const Dev& d, int64_t i, double dt, double alpha,
+                                                int uphill, int rotation, double& nd,
+                                                double& cd) {
+    BMove m;
+    double4 q = d.P0[i];
+    double2 v = d.vel[i], f = d.frc[i];
+    double a = d.angle[i], w = d.omega[i], t = d.torque[i];
+    m.old = make_double2(q.x, q.y);
+    m.dth0 = 0.0;
+    if (uphill) {  // updatePositions(-dt/2) with the decreased dt; setVelocitiesToZero
+        double bdt = -dt / 2.0;
+        double ddx = v.x * bdt, ddy = v.y * bdt;
+        m.dth0 = w * bdt;
+        q.x = q.x + ddx;
+        q.y = q.y + ddy;
+        a = a + m.dth0;
+        v = make_double2(0.0, 0.0);
+        w = 0.0;
+    }
+    m.mid = make_double2(q.x, q.y);
+    d.lpos[i] = m.old;
+    d.mpos[i] = m.mid;
+    d.dth0[i] = m.dth0;
+    // updateVelocities(dt/2) (kernelUpdateRigidVelocities)
+    double hdt = dt / 2.0;
+    if (rotation) w += t * hdt / d.moi[i];
+    else w = 0.0;
+    v.x += f.x * hdt / q.w;
+    v.y += f.y * hdt / q.w;
+    rigid_mix_one(v, f, w, t, alpha);
+    // updatePositions(dt) (kernelUpdateRigidPositions)
+    double ddx = v.x * dt, ddy = v.y * dt, dda = w * dt;
+    m.dth = dda;
+    d.dth[i] = dda;
+    double nx = q.x + ddx, ny = q.y + ddy;
+    m.now = make_double2(nx, ny);
+    d.P0[i] = make_double4(nx, ny, q.z, q.w);
+    d.angle[i] = a + dda;
+    d.vel[i] = v;
+    d.omega[i] = w;
+    disp_of(d, i, nx, ny, nd, cd);
+    return m;
+}
+
+// kernelTranslateAndRotateVertices1 (rigid_bumpy/kernels.cu:152-201) for one vertex: the
+// back-step rotation by dth0 about the old centre onto the back-stepped one (when
+// uphill), then the rotation by dth about that centre onto the new one.  (s0, c0) and
+// (s1, c1) are dpm_sincos of dth0 and dth.
+__device__ __forceinline__ double2 bfire_vertex_one(double2 x, double2 old, double2 mid,
+                                                    double2 now, double s0, double c0,
+                                                    double s1, double c1, int uphill) {
+    if (uphill) {
+        double rx = x.x - old.x, ry = x.y - old.y;
+        double rxr = rx * c0 - ry * s0;
+        double ryr = rx * s0 + ry * c0;
+        x = make_double2(mid.x + rxr, mid.y + ryr);
+    }
+    double rx = x.x - mid.x, ry = x.y - mid.y;
+    double rxr = rx * c1 - ry * s1;
+    double ryr = rx * s1 + ry * c1;
+    return make_double2(now.x + rxr, now.y + ryr);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_bfire_move(Dev d, NbrParams nb, int pos,
                                                       int rotation) {
     Ctl* c = d.ctl;
@@ -431,75 +503,26 @@ __global__ __launch_bounds__(BLOCK) void k_bfire_move(Dev d, NbrParams nb, int p
     const FireCtl* F = d.fire;
     if (F->done) return;
     long long e = c->epoch;
-    double dt = F->dt, alpha = F->alpha;
-    int uphill = F->uphill;
     int tile;
     int64_t i = particle_of_thread(tile);
     double nd = 0.0, cd = 0.0;
-    if (i < d.N) {
-        double4 q = d.P0[i];
-        double2 v = d.vel[i], f = d.frc[i];
-        double a = d.angle[i], w = d.omega[i], t = d.torque[i];
-        double2 old = make_double2(q.x, q.y);
-        double dth0 = 0.0;
-        if (uphill) {  // updatePositions(-dt/2) with the decreased dt; setVelocitiesToZero
-            double bdt = -dt / 2.0;
-            double ddx = v.x * bdt, ddy = v.y * bdt;
-            dth0 = w * bdt;
-            q.x = q.x + ddx;
-            q.y = q.y + ddy;
-            a = a + dth0;
-            v = make_double2(0.0, 0.0);
-            w = 0.0;
-        }
-        d.lpos[i] = old;
-        d.mpos[i] = make_double2(q.x, q.y);
-        d.dth0[i] = dth0;
-        // updateVelocities(dt/2) (kernelUpdateRigidVelocities)
-        double hdt = dt / 2.0;
-        if (rotation) w += t * hdt / d.moi[i];
-        else w = 0.0;
-        v.x += f.x * hdt / q.w;
-        v.y += f.y * hdt / q.w;
-        rigid_mix_one(v, f, w, t, alpha);
-        // updatePositions(dt) (kernelUpdateRigidPositions)
-        double ddx = v.x * dt, ddy = v.y * dt, dda = w * dt;
-        d.dth[i] = dda;
-        double nx = q.x + ddx, ny = q.y + ddy;
-        d.P0[i] = make_double4(nx, ny, q.z, q.w);
-        d.angle[i] = a + dda;
-        d.vel[i] = v;
-        d.omega[i] = w;
-        disp_of(d, i, nx, ny, nd, cd);
-    }
+    if (i < d.N) bfire_move_one(d, i, F->dt, F->alpha, F->uphill, rotation, nd, cd);
     (void)tile;
     disp_flags(d, nb, nd, cd, pos & 1, stamp(e, pos));  // maxima: k_disp_partials at the end
 }
 
-// kernelTranslateAndRotateVertices1 (rigid_bumpy/kernels.cu:152-201) for the back-step
-// (when uphill) and then for the forward drift
 __global__ void k_bfire_vertices(Dev d) {
     if (d.ctl->halt || d.fire->done) return;
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= d.Nv) return;
     int p = d.vpart[v];
-    double2 x = d.vpos[v], mid = d.mpos[p];
-    if (d.fire->uphill) {
-        double2 o = d.lpos[p];
-        double rx = x.x - o.x, ry = x.y - o.y;
-        double sn, c;
-        dpm_sincos(d.dth0[p], &sn, &c);
-        double rxr = rx * c - ry * sn;
-        double ryr = rx * sn + ry * c;
-        x = make_double2(mid.x + rxr, mid.y + ryr);
-    }
+    double s0 = 0.0, c0 = 1.0, s1, c1;
+    const int uphill = d.fire->uphill;
+    if (uphill) dpm_sincos(d.dth0[p], &s0, &c0);
+    dpm_sincos(d.dth[p], &s1, &c1);
     double4 nq = d.P0[p];
-    double rx = x.x - mid.x, ry = x.y - mid.y;
-    double sn, c;
-    dpm_sincos(d.dth[p], &sn, &c);
-    double rxr = rx * c - ry * sn;
-    double ryr = rx * sn + ry * c;
-    d.vpos[v] = make_double2(nq.x + rxr, nq.y + ryr);
+    d.vpos[v] = bfire_vertex_one(d.vpos[v], d.lpos[p], d.mpos[p], make_double2(nq.x, nq.y), s0,
+                                 c0, s1, c1, uphill);
 }
 
 // Particle::checkForNeighborUpdate of the step (particle.cu:1137-1161) from the
