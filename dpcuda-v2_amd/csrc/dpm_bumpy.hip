@@ -163,6 +163,55 @@ __global__ void k_bumpy_drift_vertices(Dev d) {
     d.vpos[v] = make_double2(n.x + rxr, n.y + ryr);
 }
 
+// kernelRigidBumpyAdamStep, rigid_bumpy/kernels.cu:1004-1093: moments of the force and
+// the torque, the bias-corrected step; the angle (and its change, which the vertex pass
+// reads) moves only with rotation on, so without it the vertices turn by the last
+// recorded change, as in the reference.  The vertices follow in k_bumpy_drift_vertices
+// (kernelTranslateAndRotateVertices1, launched after it in rigid_bumpy.cu:1473-1486).
+__global__ __launch_bounds__(BLOCK) void k_badam_update(Dev d, double alpha, double b1,
+                                                        double b2, double omb1, double omb2,
+                                                        double eps, int rotation) {
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double nd = 0.0, cd = 0.0;
+    if (i < d.N) {
+        double2 f = d.frc[i], m1 = d.m1[i], m2 = d.m2[i], ma = d.ma[i];
+        double t = d.torque[i];
+        m1.x = fma(b1, m1.x, (b1 - 1) * f.x);
+        m1.y = fma(b1, m1.y, (b1 - 1) * f.y);
+        ma.x = fma(b1, ma.x, (b1 - 1) * t);
+        m2.x = fma(b2, m2.x, (1 - b2) * f.x * f.x);
+        m2.y = fma(b2, m2.y, (1 - b2) * f.y * f.y);
+        ma.y = fma(b2, ma.y, (1 - b2) * t * t);
+        double mhx = m1.x / omb1, mhy = m1.y / omb1, mha = ma.x / omb1;
+        double vhx = m2.x / omb2, vhy = m2.y / omb2, vha = ma.y / omb2;
+        double ux = -alpha * mhx / (sqrt(vhx) + eps);
+        double uy = -alpha * mhy / (sqrt(vhy) + eps);
+        double ua = -alpha * mha / (sqrt(vha) + eps);
+        double4 q = d.P0[i];
+        d.lpos[i] = make_double2(q.x, q.y);
+        double nx = q.x + ux, ny = q.y + uy;
+        if (rotation) {
+            d.angle[i] = d.angle[i] + ua;
+            d.dth[i] = ua;
+        }
+        d.P0[i] = make_double4(nx, ny, q.z, q.w);
+        double2 ln = d.lref[i], lc = d.cref[i];
+        double dxn = nx - ln.x, dyn = ny - ln.y;
+        nd = dxn * dxn + dyn * dyn;
+        double dxc = nx - lc.x, dyc = ny - lc.y;
+        cd = dxc * dxc + dyc * dyc;
+        d.m1[i] = m1;
+        d.m2[i] = m2;
+        d.ma[i] = ma;
+    }
+    double bn = block_max(nd);
+    double bc = block_max(cd);
+    if (threadIdx.x == 0) {
+        d.pmax_nd[blockIdx.x] = bn;
+        d.pmax_cd[blockIdx.x] = bc;
+    }
+}
+
 // kernelMixRigidVelocitiesAndForces, rigid_bumpy/kernels.cu:89-121
 __device__ __forceinline__ void rigid_mix_one(double2& v, double2 f, double& w, double t,
                                               double alpha) {
@@ -844,6 +893,15 @@ int bumpy_kick(dpm_system* s, double dt) {
 
 int bumpy_drift(dpm_system* s, double dt) {
     hipLaunchKernelGGL(k_bumpy_drift_com, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, dt);
+    hipLaunchKernelGGL(k_bumpy_drift_vertices, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
+    return check_launch();
+}
+
+// RigidBumpy::updatePositionsAdam, rigid_bumpy.cu:1473-1486
+int bumpy_adam_update(dpm_system* s, double alpha, double b1, double b2, double omb1,
+                      double omb2, double eps) {
+    hipLaunchKernelGGL(k_badam_update, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, alpha, b1,
+                       b2, omb1, omb2, eps, (int)s->rotation);
     hipLaunchKernelGGL(k_bumpy_drift_vertices, pgrid(s->Nv), dim3(BLOCK), 0, s->stream, s->d);
     return check_launch();
 }

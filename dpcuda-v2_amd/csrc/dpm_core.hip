@@ -769,6 +769,7 @@ __global__ void k_reorder_gather(Dev d, Spare sp, int gated) {
         sp.m1[k] = d.m1[o];
         sp.m2[k] = d.m2[o];
     }
+    if (d.ma) sp.ma[k] = d.ma[o];
     if (d.vpos) {
         sp.angle[k] = d.angle[o];
         sp.omega[k] = d.omega[o];
@@ -794,6 +795,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_copyback(Dev d, Spare sp, int
         d.m1[k] = sp.m1[k];
         d.m2[k] = sp.m2[k];
     }
+    if (d.ma) d.ma[k] = sp.ma[k];
     if (d.vpos) {
         d.angle[k] = sp.angle[k];
         d.omega[k] = sp.omega[k];
@@ -1848,6 +1850,9 @@ int ensure_adam(dpm_system* s) {
     if (dalloc(s, &s->d.m1, s->N) || dalloc(s, &s->d.m2, s->N) || dalloc(s, &s->sp.m1, s->N) ||
         dalloc(s, &s->sp.m2, s->N))
         return -1;
+    // RigidBumpy::initAdamVariables adds the angle moments (rigid_bumpy.cu:1461-1465)
+    if (s->kind == DPM_BUMPY && (dalloc(s, &s->d.ma, s->N) || dalloc(s, &s->sp.ma, s->N)))
+        return -1;
     s->gen++;
     return 0;
 }
@@ -2416,15 +2421,12 @@ int dpm_fire_step(dpm_system* s, dpm_fire_state* st) {
 
 int dpm_adam_step(dpm_system* s, int64_t step, double alpha, double b1, double b2, double eps) {
     if (require_nbr(s, "dpm_adam_step")) return -1;
-    if (s->kind == DPM_BUMPY) {
-        set_error("dpm_adam_step: rigid bumpy Adam is not supported");
-        return -1;
-    }
     if (ensure_adam(s)) return -1;
     // Adam::minimize (src/integrator/adam.cu:11-18)
     if (dpm_zero_forces(s) || check_and_rebuild(s, nullptr) || dpm_compute_forces(s)) return -1;
     double omb1 = 1 - std::pow(b1, step + 1);
     double omb2 = 1 - std::pow(b2, step + 1);
+    if (s->kind == DPM_BUMPY) return bumpy_adam_update(s, alpha, b1, b2, omb1, omb2, eps);
     hipLaunchKernelGGL(k_adam_update, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d,
                        nbr_params(s), alpha, b1, b2, omb1, omb2, eps);
     return check_launch();
@@ -2435,6 +2437,7 @@ int dpm_adam_reset(dpm_system* s) {
     // Particle::initAdamVariables (particle.cu:319-322)
     DPM_HIP(hipMemsetAsync(s->d.m1, 0, sizeof(double2) * s->N, s->stream));
     DPM_HIP(hipMemsetAsync(s->d.m2, 0, sizeof(double2) * s->N, s->stream));
+    if (s->d.ma) DPM_HIP(hipMemsetAsync(s->d.ma, 0, sizeof(double2) * s->N, s->stream));
     return 0;
 }
 

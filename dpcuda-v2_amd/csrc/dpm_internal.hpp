@@ -84,6 +84,7 @@ struct Dev {
     Ctl* ctl;
     FireCtl* fire;
     double2 *m1, *m2;           // Adam moments (allocated on first use)
+    double2* ma;                // rigid Adam: angle moments {first, second}
     double *sxx, *sxy, *syx, *syy;
     int *contacts, *vcontacts;
     // rigid bumpy
@@ -104,7 +105,7 @@ struct Dev {
 // Reorder scratch: one spare copy of every permuted array.
 struct Spare {
     double4* P;
-    double2 *vel, *frc, *m1, *m2;
+    double2 *vel, *frc, *m1, *m2, *ma;
     double *angle, *omega, *torque, *moi, *vmass;
     double2* vpos;
     int *sidx, *nverts, *vstart, *vpart, *svidx, *old_to_new;

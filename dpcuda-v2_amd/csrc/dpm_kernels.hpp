@@ -183,6 +183,8 @@ void bumpy_enqueue_vertex_reorder(dpm_system* s, int gated);
 int bumpy_ensure_vcap(dpm_system* s, int64_t cap);
 int bumpy_kick(dpm_system* s, double dt);
 int bumpy_drift(dpm_system* s, double dt);
+int bumpy_adam_update(dpm_system* s, double alpha, double b1, double b2, double omb1,
+                      double omb2, double eps);
 int bumpy_mix(dpm_system* s, double alpha);
 int bumpy_stress(dpm_system* s);
 int bumpy_contacts(dpm_system* s);

@@ -141,8 +141,11 @@ int dpm_fire_step(dpm_system* s, dpm_fire_state* st);
 
 /* ---- Adam -------------------------------------------------------------------------------
  * Adam::minimize (src/integrator/adam.cu:11-18) + Particle::updatePositionsAdam
- * (particle.cu:329-333); minimizeAdam (src/routines/minimization.cu:3-33);
- * dpm_adam_reset: Particle::initAdamVariables (particle.cu:319-322), zero moments. */
+ * (particle.cu:329-333); rigid bumpy: RigidBumpy::updatePositionsAdam (rigid_bumpy.cu:
+ * 1473-1486, moments of force and torque, then the vertex rotation); minimizeAdam
+ * (src/routines/minimization.cu:3-33); dpm_adam_reset: Particle::initAdamVariables
+ * (particle.cu:319-322) and RigidBumpy::initAdamVariables (rigid_bumpy.cu:1461-1465),
+ * zero moments. */
 int dpm_adam_reset(dpm_system* s);
 int dpm_adam_step(dpm_system* s, int64_t step, double alpha, double beta1, double beta2,
                   double eps);

@@ -153,7 +153,7 @@ struct orc_system {
     // per particle (internal order)
     std::vector<double> x, y, vx, vy, fx, fy, r, m, pe, ke;
     std::vector<double> lnx, lny, lcx, lcy, nd2, cd2;
-    std::vector<double> m1x, m1y, m2x, m2y;
+    std::vector<double> m1x, m1y, m2x, m2y, m1a, m2a;  // m1a, m2a: rigid angle moments
     std::vector<double> sxx, sxy, syx, syy;
     std::vector<long> contacts, vcontacts;
     // neighbour structures (reference layout: row-major, -1 padded, int64)
@@ -194,7 +194,7 @@ void resize_particles(orc_system* s) {
     long N = s->N;
     for (auto* v : {&s->x, &s->y, &s->vx, &s->vy, &s->fx, &s->fy, &s->r, &s->m, &s->pe, &s->ke,
                     &s->lnx, &s->lny, &s->lcx, &s->lcy, &s->nd2, &s->cd2, &s->m1x, &s->m1y,
-                    &s->m2x, &s->m2y, &s->sxx, &s->sxy, &s->syx, &s->syy, &s->th, &s->w,
+                    &s->m2x, &s->m2y, &s->m1a, &s->m2a, &s->sxx, &s->sxy, &s->syx, &s->syy, &s->th, &s->w,
                     &s->tq, &s->moi, &s->lpx, &s->lpy, &s->dlx, &s->dly, &s->dth})
         v->assign(N, 0.0);
     s->contacts.assign(N, 0);
@@ -375,7 +375,7 @@ void update_cell_list(orc_system* s) {
     for (auto* v : {&s->x, &s->y, &s->fx, &s->fy, &s->vx, &s->vy, &s->m, &s->r})
         permute(*v, old);
     // Adam moments follow their particles (documented deviation)
-    for (auto* v : {&s->m1x, &s->m1y, &s->m2x, &s->m2y}) permute(*v, old);
+    for (auto* v : {&s->m1x, &s->m1y, &s->m2x, &s->m2y, &s->m1a, &s->m2a}) permute(*v, old);
     if (s->kind == ORC_BUMPY) {
         std::vector<long> old_to_new(N);
         for (long k = 0; k < N; ++k) old_to_new[old[k]] = k;
@@ -543,6 +543,22 @@ void kick(orc_system* s, double dt) {
     }
 }
 
+// kernelTranslateAndRotateVertices1 rigid_bumpy/kernels.cu:152-201: each vertex turns by
+// its particle's last angle change about the old centre and lands on the new one
+void move_vertices(orc_system* s) {
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < s->Nv; ++v) {
+        long p = s->vpart[v];
+        double rx = s->vxp[v] - s->lpx[p], ry = s->vyp[v] - s->lpy[p];
+        double sn, c;
+        dpm_sincos(s->dth[p], &sn, &c);
+        double rxr = rx * c - ry * sn;
+        double ryr = rx * sn + ry * c;
+        s->vxp[v] = s->x[p] + rxr;
+        s->vyp[v] = s->y[p] + ryr;
+    }
+}
+
 // kernelUpdatePositions base/kernels.cu:48-81; rigid: kernelUpdateRigidPositions
 // rigid_bumpy/kernels.cu:21-78 + kernelTranslateAndRotateVertices1 :152-201
 void drift(orc_system* s, double dt) {
@@ -565,17 +581,7 @@ void drift(orc_system* s, double dt) {
             double dxc = nx - s->lcx[i], dyc = ny - s->lcy[i];
             s->cd2[i] = dxc * dxc + dyc * dyc;
         }
-#pragma omp parallel for schedule(static)
-        for (long v = 0; v < s->Nv; ++v) {
-            long p = s->vpart[v];
-            double rx = s->vxp[v] - s->lpx[p], ry = s->vyp[v] - s->lpy[p];
-            double sn, c;
-            dpm_sincos(s->dth[p], &sn, &c);
-            double rxr = rx * c - ry * sn;
-            double ryr = rx * sn + ry * c;
-            s->vxp[v] = s->x[p] + rxr;
-            s->vyp[v] = s->y[p] + ryr;
-        }
+        move_vertices(s);
         return;
     }
 #pragma omp parallel for schedule(static)
@@ -952,6 +958,45 @@ void minimize_fire(orc_system* s, const orc_fire_params* p, orc_fire_result* out
 void adam_update(orc_system* s, long step, double alpha, double b1, double b2, double eps) {
     double omb1 = 1 - std::pow(b1, step + 1);
     double omb2 = 1 - std::pow(b2, step + 1);
+    if (s->kind == ORC_BUMPY) {
+        // RigidBumpy::updatePositionsAdam rigid_bumpy.cu:1473-1486 + kernelRigidBumpyAdamStep
+        // rigid_bumpy/kernels.cu:1004-1093; without rotation the angle and its recorded
+        // change stay as they are, and the vertex pass still turns by that change
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < s->N; ++i) {
+            double fx = s->fx[i], fy = s->fy[i], t = s->tq[i];
+            double m1x = std::fma(b1, s->m1x[i], (b1 - 1) * fx);
+            double m1y = std::fma(b1, s->m1y[i], (b1 - 1) * fy);
+            double m1a = std::fma(b1, s->m1a[i], (b1 - 1) * t);
+            double m2x = std::fma(b2, s->m2x[i], (1 - b2) * fx * fx);
+            double m2y = std::fma(b2, s->m2y[i], (1 - b2) * fy * fy);
+            double m2a = std::fma(b2, s->m2a[i], (1 - b2) * t * t);
+            double ux = -alpha * (m1x / omb1) / (std::sqrt(m2x / omb2) + eps);
+            double uy = -alpha * (m1y / omb1) / (std::sqrt(m2y / omb2) + eps);
+            double ua = -alpha * (m1a / omb1) / (std::sqrt(m2a / omb2) + eps);
+            double px = s->x[i], py = s->y[i];
+            s->lpx[i] = px;
+            s->lpy[i] = py;
+            px += ux;
+            py += uy;
+            s->dlx[i] = ux;
+            s->dly[i] = uy;
+            if (s->rotation) {
+                s->th[i] += ua;
+                s->dth[i] = ua;
+            }
+            s->x[i] = px;
+            s->y[i] = py;
+            double dxn = px - s->lnx[i], dyn = py - s->lny[i];
+            s->nd2[i] = dxn * dxn + dyn * dyn;
+            double dxc = px - s->lcx[i], dyc = py - s->lcy[i];
+            s->cd2[i] = dxc * dxc + dyc * dyc;
+            s->m1x[i] = m1x; s->m1y[i] = m1y; s->m1a[i] = m1a;
+            s->m2x[i] = m2x; s->m2y[i] = m2y; s->m2a[i] = m2a;
+        }
+        move_vertices(s);
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < s->N; ++i) {
         double fx = s->fx[i], fy = s->fy[i];
@@ -989,6 +1034,9 @@ void adam_reset(orc_system* s) {
     std::fill(s->m1y.begin(), s->m1y.end(), 0.0);
     std::fill(s->m2x.begin(), s->m2x.end(), 0.0);
     std::fill(s->m2y.begin(), s->m2y.end(), 0.0);
+    // RigidBumpy::initAdamVariables rigid_bumpy.cu:1461-1465
+    std::fill(s->m1a.begin(), s->m1a.end(), 0.0);
+    std::fill(s->m2a.begin(), s->m2a.end(), 0.0);
 }
 
 long minimize_adam(orc_system* s, double alpha, double b1, double b2, double eps, double pe_t,

@@ -3,7 +3,7 @@ CPU oracle.
 
 Vertex rings and rigid-body rotations use one shared sin/cos (dpm_sincos) on both sides,
 so set-up, vertex and particle pair sets, forces, torques, energies, NVE and FIRE
-trajectories, stress without spin and contacts agree bit for bit.  The stress tensor of
+trajectories, Adam, stress without spin and contacts agree bit for bit.  The stress tensor of
 spinning particles goes through the library atan2/sin/cos and is held to 1e-12.
 The parity bars (forces 1e-12, NVE 1e-9 over 100 steps, FIRE final E, P, phi 1e-8)
 are therefore met with margin.
@@ -183,6 +183,23 @@ def test_fire_step_trace_with_rotation():
         assert (sg.power, sg.dt, sg.alpha, sg.n_good, sg.n_bad, sg.stopped) == \
             (so.power, so.dt, so.alpha, so.n_good, so.n_bad, so.stopped), k
     same(g, o, PART)
+
+
+@pytest.mark.parametrize("rotation", [True, False])
+def test_adam_matches_oracle(rotation):
+    # rigid Adam (kernelRigidBumpyAdamStep + the vertex pass) through minimizeAdam's loop;
+    # 400 steps of up to 2e-3 per coordinate move particles across cells, so the angle
+    # moments are permuted with their particles at the cell reorders
+    say("bumpy adam", rotation)
+    st = W.bumpy_config(256, 0.85, seed=23)
+    g, o = bpair(st, rotation=rotation)
+    ng = g.minimize_adam(alpha=2e-3, max_steps=400)
+    no = o.minimize_adam(alpha=2e-3, max_steps=400)
+    assert ng == no
+    assert g.neighbor_stats()["n_cell_rebuilds"] == o.neighbor_stats()["n_cell_rebuilds"] > 0
+    same(g, o, PART)
+    for f in ("F_VERTEX_X", "F_VERTEX_Y"):
+        assert np.array_equal(g.download(getattr(P, f)), o.download(getattr(P, f))), f
 
 
 def test_snapshot_restore():

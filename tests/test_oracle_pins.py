@@ -14,6 +14,8 @@ plausible slip (a dropped term, a wrong sign or index, a transposed operand) fai
                                      O(dt^2) energy error
   FIRE (fire.cu:13-46)               state-machine known answers; two disks relax
   Adam (base/kernels.cu:550-611)     first-step known answer
+  rigid Adam (rigid_bumpy/kernels.cu: first-step known answer for centres, angles and
+    1004-1093)                       vertex rings; rotation off keeps angles and rings
   stress (disk/kernels.cu:203-251)   2-particle closed form; virial == -dU/dA
   packing fraction (disk.cu:205-207) pi*sum r^2 / A
   cell reorder (particle.cu:1292-)   stable sort by cell id
@@ -448,6 +450,57 @@ def test_bumpy_torque_balance_and_contact():
     x, y = np.array([5.0, 6.15]), np.array([5.0, 5.05])
     total = (tq + x * fy - y * fx).sum()
     assert abs(total) < 1e-13
+
+
+@pytest.mark.parametrize("rotation", [True, False])
+def test_bumpy_adam_first_step_known_answer(rotation):
+    # two overlapping rings: on the first step m_hat = -f, v_hat = f^2 for the force and
+    # the torque alike, so the centre moves by alpha f / (|f| + eps), the angle by
+    # alpha tau / (|tau| + eps) (rotation on) and every vertex keeps its place in the
+    # ring, turned by that angle about the moved centre
+    x0, y0, a0 = np.array([5.0, 6.15]), np.array([5.0, 5.05]), np.array([0.0, 0.1])
+    s, rad, nverts, rv = bumpy_small(2, rotation=rotation, x=x0, y=y0, angle=a0)
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * rv, 1.4)
+    s.neighbor_init()
+    vx0, vy0 = s.download(P.F_VERTEX_X), s.download(P.F_VERTEX_Y)
+    alpha, eps = 1e-3, 1e-8
+    s.adam_reset()
+    s.adam_step(0, alpha, 0.9, 0.999, eps)
+    fx, fy, tq = s.download(P.F_FX), s.download(P.F_FY), s.download(P.F_TORQUE)
+    assert np.all(np.abs(fx) > 1e-6) and np.all(np.abs(tq) > 1e-9)
+    x1, y1, a1 = s.download(P.F_X), s.download(P.F_Y), s.download(P.F_ANGLE)
+    np.testing.assert_allclose(x1 - x0, alpha * fx / (np.abs(fx) + eps), rtol=1e-10)
+    np.testing.assert_allclose(y1 - y0, alpha * fy / (np.abs(fy) + eps), rtol=1e-10)
+    da = alpha * tq / (np.abs(tq) + eps) if rotation else np.zeros(2)
+    np.testing.assert_allclose(a1 - a0, da, rtol=1e-10, atol=0)
+    owner = np.repeat(np.arange(2), nverts)
+    rx, ry = vx0 - x0[owner], vy0 - y0[owner]
+    c, sn = np.cos(da[owner]), np.sin(da[owner])
+    np.testing.assert_allclose(s.download(P.F_VERTEX_X), x1[owner] + rx * c - ry * sn,
+                               rtol=0, atol=1e-13)
+    np.testing.assert_allclose(s.download(P.F_VERTEX_Y), y1[owner] + rx * sn + ry * c,
+                               rtol=0, atol=1e-13)
+
+
+def test_bumpy_adam_relaxes_overlaps():
+    # Adam from overlapping rings lowers the energy; the rings stay rigid
+    x0, y0 = np.array([5.0, 6.15]), np.array([5.0, 5.05])
+    s, rad, nverts, rv = bumpy_small(2, x=x0, y=y0, angle=[0.0, 0.1])
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * rv, 1.4)
+    s.neighbor_init()
+    s.zero_forces()
+    s.compute_forces()
+    pe0 = s.reduce(P.RED_PE)
+    steps = s.minimize_adam(alpha=1e-3, max_steps=2000)
+    assert 0 < steps <= 2000
+    assert s.reduce(P.RED_PE) < 1e-3 * pe0
+    x, y = s.download(P.F_X), s.download(P.F_Y)
+    owner = np.repeat(np.arange(2), nverts)
+    d = np.hypot(s.download(P.F_VERTEX_X) - x[owner], s.download(P.F_VERTEX_Y) - y[owner])
+    ring = (rad - rv)[owner]
+    np.testing.assert_allclose(d, ring, rtol=0, atol=1e-12)
 
 
 # ---------------------------------------------------------------- shared sin/cos
