@@ -24,6 +24,7 @@
 //   k_b_move    FIRE move of systems inside a minimisation (Fire::step first half)
 //   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
 //   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
+//   k_b_build_cells  per large system: the same rows from a cell grid in LDS
 //   k_b_force   forces (+ closing kick) and the chunk partials of P, PE, max F^2
 //   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision
 //   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials
@@ -208,20 +209,178 @@ __global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
 
 // Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
 // the controller's requests that k_b_prep has applied.
-__global__ void k_b_decide(BDev b) {
-    const long long s = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (s == 0) b.flags[3] += 1;  // one thread of the step counts it
-    if (s >= b.nsys) return;
+__global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
+    // one wave per system: the lanes test the system's chunk flags together
+    const int s = blockIdx.x, lane = threadIdx.x;
+    if (s == 0 && lane == 0) b.flags[3] += 1;  // one thread of the step counts it
     BSys& S = b.sys[s];
-    S.snap_op = 0;
-    S.scale_f = 0.0;
-    if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
-    const double* D = bpart(b, BP_DMAX) + S.off / WAVE;  // per wave: some 3 d^2 > thr
-    for (int c = 0; c < S.nc; ++c)
-        if (D[c] != 0.0) {
-            S.rebuild = 1;
-            break;
+    const int mode = S.mode;
+    bool far = false;
+    if (mode == BM_PROLOGUE || mode == BM_FIRE) {
+        const double* D = bpart(b, BP_DMAX) + S.off / WAVE;  // per wave: some 3 d^2 > thr
+        for (int c = lane; c < S.nc; c += WAVE) far = far || D[c] != 0.0;
+    }
+    const bool any = __builtin_amdgcn_ballot_w64(far) != 0;
+    if (lane == 0) {
+        S.snap_op = 0;
+        S.scale_f = 0.0;
+        if (any) S.rebuild = 1;
+    }
+}
+
+// Large systems get their rows from a cell grid instead of the all-pairs scan (one
+// workgroup per system, grid and lists in LDS).  Cells are at least rc (1 + 1e-9) wide,
+// so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
+// hold the same indices in the same ascending order as the all-pairs scan.
+constexpr int BC_MIN_N = 2048;     // systems from this size up use the cell grid
+constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
+constexpr int BC_MAX_D = 96;       // cells per side at most
+constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
+constexpr int BC_BLOCK = 1024;
+
+struct BCGrid {
+    int nx, ny;
+    double csx, csy;
+};
+
+__device__ __forceinline__ bool bc_grid(const BDev& b, const BSys& S, BCGrid& g) {
+    if (S.n < BC_MIN_N || S.n > BC_MAX_N) return false;
+    const double w = b.rc * (1.0 + 1e-9);
+    g.nx = (int)fmin((double)BC_MAX_D, floor(S.Lx / w));
+    g.ny = (int)fmin((double)BC_MAX_D, floor(S.Ly / w));
+    if (g.nx < 3 || g.ny < 3) return false;
+    g.csx = S.Lx / g.nx;
+    g.csy = S.Ly / g.ny;
+    return true;
+}
+
+// the cell of one coordinate: wrapped into [0, L], clamped to the grid
+__device__ __forceinline__ int bc_coord(double x, double L, double cs, int n) {
+    double w = x - L * floor(x / L);
+    int c = (int)(w / cs);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+__global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
+    const BSys& S = b.sys[blockIdx.x];
+    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
+    BCGrid g;
+    if (!bc_grid(b, S, g)) return;
+    __shared__ int cend[BC_CELLS];  // counts, then (after the fill) each cell's end
+    __shared__ int lst[BC_MAX_N];   // particles (system-local) grouped by cell
+    __shared__ int wsum[BC_BLOCK / WAVE];
+    const int tid = threadIdx.x, n = S.n, ncell = g.nx * g.ny;
+    const long long off = S.off;
+    for (int c = tid; c < ncell; c += BC_BLOCK) cend[c] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += BC_BLOCK) {
+        double4 q = b.P[off + i];
+        int c = bc_coord(q.x, S.Lx, g.csx, g.nx) + g.nx * bc_coord(q.y, S.Ly, g.csy, g.ny);
+        atomicAdd(&cend[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: each thread owns a run of consecutive cells
+    const int per = (ncell + BC_BLOCK - 1) / BC_BLOCK;
+    const int c0 = min(tid * per, ncell), c1 = min(c0 + per, ncell);
+    int local = 0;
+    for (int c = c0; c < c1; ++c) local += cend[c];
+    const int lane = tid % WAVE, wv = tid / WAVE;
+    int incl = local;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += o;
+    }
+    if (lane == WAVE - 1) wsum[wv] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wv; ++w) before += wsum[w];
+    int run = before + incl - local;
+    for (int c = c0; c < c1; ++c) {
+        int t = cend[c];
+        cend[c] = run;
+        run += t;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += BC_BLOCK) {
+        double4 q = b.P[off + i];
+        int c = bc_coord(q.x, S.Lx, g.csx, g.nx) + g.nx * bc_coord(q.y, S.Ly, g.csy, g.ny);
+        lst[atomicAdd(&cend[c], 1)] = i;
+    }
+    __syncthreads();
+    // each cell's members in ascending order (a few per cell)
+    for (int c = tid; c < ncell; c += BC_BLOCK) {
+        const int a = c == 0 ? 0 : cend[c - 1], e = cend[c];
+        for (int u = a + 1; u < e; ++u) {
+            int v = lst[u], k = u - 1;
+            while (k >= a && lst[k] > v) {
+                lst[k + 1] = lst[k];
+                --k;
+            }
+            lst[k + 1] = v;
         }
+    }
+    __syncthreads();
+    const PairParams p = bparams(b, S);
+    constexpr int NONE = 0x7fffffff;
+    int mx = 0;
+    for (int il = tid; il < n; il += BC_BLOCK) {
+        const long long i = off + il;
+        const double4 qi = b.P[i];
+        const int cx = bc_coord(qi.x, S.Lx, g.csx, g.nx), cy = bc_coord(qi.y, S.Ly, g.csy, g.ny);
+        // the nine cells' ranges, merged in ascending particle order
+        int pa[9], pe[9], h[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            int x = cx + t % 3 - 1, y = cy + t / 3 - 1;
+            x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+            y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+            const int c = x + g.nx * y;
+            pa[t] = c == 0 ? 0 : cend[c - 1];
+            pe[t] = cend[c];
+            h[t] = pa[t] < pe[t] ? lst[pa[t]] : NONE;
+        }
+        int cnt = 0;
+        while (true) {
+            int best = h[0], bt = 0;
+#pragma unroll
+            for (int t = 1; t < 9; ++t)
+                if (h[t] < best) {
+                    best = h[t];
+                    bt = t;
+                }
+            if (best == NONE) break;
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+                if (t == bt) {
+                    pa[t]++;
+                    h[t] = pa[t] < pe[t] ? lst[pa[t]] : NONE;
+                }
+            if (best == il) continue;
+            const long long j = off + best;
+            const double4 qj = b.P[j];
+            double dx, dy;
+            if (p.fast) {
+                dx = pbc_rint(qi.x, qj.x, p.Lx, p.invLx);
+                dy = pbc_rint(qi.y, qj.y, p.Ly, p.invLy);
+            } else {
+                dx = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, 0);
+                dy = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, 0);
+            }
+            if (dx * dx + dy * dy < b.rc2) {
+                if (cnt < b.cap) b.nbr[(long long)cnt * b.nslot + i] = (int)j;
+                cnt++;
+            }
+        }
+        b.nnum[i] = cnt;
+        b.lref[i] = make_double2(qi.x, qi.y);
+        mx = max(mx, cnt);
+    }
+    mx = wave_max_i(mx);
+    if (lane == 0) {
+        atomicMax(&b.flags[1], mx);
+        if (mx > b.cap) atomicMax(&b.flags[0], 1);
+    }
 }
 
 // kernelUpdateNeighborList (base/kernels.cu:256-293) for the systems that need it:
@@ -232,12 +391,16 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
     const int2 t = b.tiles[blockIdx.x];
     const BSys& S = b.sys[t.x];
     if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL) return;
+    BCGrid g;
+    if (bc_grid(b, S, g)) return;  // k_b_build_cells builds this system
     __shared__ double2 tp[BLOCK];
     const long long i = (long long)t.y + threadIdx.x;
     const long long end = S.off + S.n;
     const bool live = i < end;
     double4 qi = live ? b.P[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double Lx = S.Lx, Ly = S.Ly;
+    // when half the box exceeds every cutoff the branch-free minimum image gives the
+    // same verdict for every pair (it differs only where |d| is half the box)
+    const PairParams p = bparams(b, S);
     int cnt = 0;
     for (long long base = S.off; base < end; base += BLOCK) {
         __syncthreads();
@@ -251,8 +414,14 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
             for (int u = 0; u < lim; ++u) {
                 long long j = base + u;
                 if (j == i) continue;
-                double dx = pbc(qi.x, tp[u].x, Lx, 1.0 / Lx, 0.5 * Lx, 0);
-                double dy = pbc(qi.y, tp[u].y, Ly, 1.0 / Ly, 0.5 * Ly, 0);
+                double dx, dy;
+                if (p.fast) {
+                    dx = pbc_rint(qi.x, tp[u].x, p.Lx, p.invLx);
+                    dy = pbc_rint(qi.y, tp[u].y, p.Ly, p.invLy);
+                } else {
+                    dx = pbc(qi.x, tp[u].x, p.Lx, p.invLx, p.halfLx, 0);
+                    dy = pbc(qi.y, tp[u].y, p.Ly, p.invLy, p.halfLy, 0);
+                }
                 if (dx * dx + dy * dy < b.rc2) {
                     if (cnt < b.cap) b.nbr[(long long)cnt * b.nslot + i] = (int)j;
                     cnt++;
@@ -690,11 +859,11 @@ static void enqueue_batch_step(dpm_batch* b) {
     const BDev& d = b->d;
     hipStream_t st = b->stream;
     dim3 slots((unsigned)((b->nslot + BLOCK - 1) / BLOCK));
-    dim3 sysb((unsigned)((b->nsys + BLOCK - 1) / BLOCK));
     hipLaunchKernelGGL(k_b_prep, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_move, slots, dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_decide, sysb, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)b->nsys), dim3(BC_BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_force, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
@@ -927,10 +1096,8 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
                            b->stream));
     DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
-    {
-        int nt = (int)b->ntiles;
-        DPM_HIP(hipMemcpyAsync(d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice, b->stream));
-    }
+    const int nt = (int)b->ntiles;  // lives until the synchronisation below
+    DPM_HIP(hipMemcpyAsync(d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice, b->stream));
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (b->exec) {  // the parameters are baked into the graph's kernel arguments
         (void)hipGraphExecDestroy(b->exec);

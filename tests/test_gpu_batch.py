@@ -18,9 +18,9 @@ def say(*a):
     print("[gpu-case]", *a, flush=True)
 
 
-def params(max_steps=1500, fmax=1e-10, strain=5e-3):
+def params(max_steps=1500, fmax=1e-10, strain=5e-3, csteps=400):
     return P.jam_params(pressure_target=1e-3, window_rel=0.2, strain_step=strain,
-                        max_compression_steps=400,
+                        max_compression_steps=csteps,
                         fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
                                            fmax_target=fmax, max_steps=max_steps))
 
@@ -63,6 +63,23 @@ def test_batched_equals_each_system_alone():
         bx, by, side = pos[k]
         assert np.array_equal(bx, x) and np.array_equal(by, y), k
         assert side == r.box_length
+
+
+def test_large_systems_cell_built_rows():
+    """Systems of 2048 particles and more get their Verlet rows from a cell grid; the rows,
+    and so every result bit, equal the all-pairs rows of the plain path.  The largest
+    system fills the grid's particle capacity (16,384)."""
+    sts = systems([(2048, 0.80, 11), (4500, 0.82, 12), (300, 0.80, 13), (16384, 0.80, 14)])
+    prm = params(max_steps=300, fmax=1e-8, strain=1e-2, csteps=40)
+    res, pos, steps = batched(sts, prm)
+    say("large batched steps", steps)
+    for k, st in enumerate(sts):
+        r, x, y = plain(st, prm)
+        say("system", k, st.n, r.status, r.compression_steps, r.fire_steps_total)
+        for f in FIELDS:
+            assert getattr(res[k], f) == getattr(r, f), (k, f, getattr(res[k], f), getattr(r, f))
+        bx, by, side = pos[k]
+        assert np.array_equal(bx, x) and np.array_equal(by, y), k
 
 
 def test_result_same_alone_and_in_any_mix():
