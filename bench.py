@@ -409,7 +409,8 @@ def jam_batch(args, P, W):
     both on the same systems.  value = jammed packings per second of the batched engine."""
     import numpy as np
     rng = np.random.default_rng(args.seed)
-    ns = np.exp(rng.uniform(np.log(64), np.log(2048), args.systems)).astype(np.int64)
+    ns = np.exp(rng.uniform(np.log(args.batch_n_lo), np.log(args.batch_n_hi),
+                            args.systems)).astype(np.int64)
     sts = [W.disk_config(int(n), args.jam_phi0, seed=args.seed + 1 + k) for k, n in enumerate(ns)]
     prm = P.jam_params(pressure_target=args.jam_pressure, window_rel=args.jam_window,
                        strain_step=args.jam_strain, max_compression_steps=args.jam_csteps,
@@ -422,6 +423,7 @@ def jam_batch(args, P, W):
     b = P.Batch([st.n for st in sts])
     b.config(1.0, 2.0, rc, thr_sq)
     b.set_option(P.OPT_CHUNK, args.batch_chunk)
+    b.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
     for k, st in enumerate(sts):
         b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
     tuned = None
@@ -447,6 +449,7 @@ def jam_batch(args, P, W):
     for k in pick:
         st = sts[int(k)]
         s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
+        s.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
         s.synchronize()
         c0 = time.perf_counter()
         r = s.jam(prm)
@@ -469,10 +472,11 @@ def jam_batch(args, P, W):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform positions; N log-uniform in [64, 2048])",
+        "data": f"synthetic (seeded uniform positions; N log-uniform in [{args.batch_n_lo}, "
+                f"{args.batch_n_hi}])",
         "config": {
             "workload": f"{len(sts)} bidisperse disk systems (1.0/1.4, 50/50), N log-uniform "
-                        f"in [64, 2048], phi0={args.jam_phi0}, compress by strain "
+                        f"in [{args.batch_n_lo}, {args.batch_n_hi}], phi0={args.jam_phi0}, compress by strain "
                         f"{args.jam_strain} with FIRE to |F|max<{args.jam_fmax} (at most "
                         f"{args.jam_fire_steps} steps each), pressure bisection to "
                         f"{args.jam_pressure} (window {args.jam_window}), Verlet rc=1.7",
@@ -646,6 +650,8 @@ def main():
     ap.add_argument("--jam-fire-steps", type=int, default=100000)
     ap.add_argument("--jam-csteps", type=int, default=100000)
     ap.add_argument("--batch-chunk", type=int, default=32)
+    ap.add_argument("--batch-n-lo", type=int, default=64)
+    ap.add_argument("--batch-n-hi", type=int, default=2048)
     ap.add_argument("--batch-max-steps", type=int, default=10**9)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--autotune", action="store_true")
