@@ -225,6 +225,14 @@ void scenario_bumpy_init() {
     minimizeFire(*p, 0.1, 1e-2, 100, 1000);
     state(*p, "bumpy.fire100");
     put("bumpy.angles100", rb->getAngles());
+    // rigid Adam (RigidBumpy::updatePositionsAdam): moments of force and torque
+    p->scaleToPackingFractionFull(0.75);
+    minimizeAdam(*p, 1e-3, 0.9, 0.999, 1e-8, 0.0, 0.0, 200, 1000);
+    state(*p, "bumpy.adam200");
+    put("bumpy.angles_adam", rb->getAngles());
+    auto va = rb->getVertexPositions();
+    put("bumpy.vx_adam", va.first);
+    put("bumpy.vy_adam", va.second);
 }
 
 // JSON text of ConfigDict values: one "@ json.<k> <dump>" line per case (no spaces in the

@@ -127,13 +127,21 @@ def test_bumpy_geometry_values():
     vx = [num(t) for t in out["bumpy.vx"]]
     vy = [num(t) for t in out["bumpy.vy"]]
     y = [num(t) for t in out["bumpy.init.y"]]
-    k = 0
-    for i in range(64):
-        nv, r = (28, 0.7) if i < 32 else (20, 0.5)
-        for j in range(nv):
-            d = math.hypot(vx[k] - x[i], vy[k] - y[i])
-            assert d == pytest.approx(r - rv, rel=1e-12)
-            k += 1
+    def rings_at(vx, vy, x, y):
+        k = 0
+        for i in range(64):
+            nv, r = (28, 0.7) if i < 32 else (20, 0.5)
+            for j in range(nv):
+                d = math.hypot(vx[k] - x[i], vy[k] - y[i])
+                assert d == pytest.approx(r - rv, rel=1e-12)
+                k += 1
+
+    rings_at(vx, vy, x, y)
+    # rigid Adam (Adam over RigidBumpy after a compression): the rings move rigidly with
+    # their centres, and the angles change
+    rings_at([num(t) for t in out["bumpy.vx_adam"]], [num(t) for t in out["bumpy.vy_adam"]],
+             [num(t) for t in out["bumpy.adam200.x"]], [num(t) for t in out["bumpy.adam200.y"]])
+    assert out["bumpy.angles_adam"] != out["bumpy.angles100"]
 
 
 def test_drivers_are_built():
