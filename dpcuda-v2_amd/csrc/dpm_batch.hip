@@ -228,8 +228,9 @@ __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     }
 }
 
-// Large systems get their rows from a cell grid instead of the all-pairs scan (one
-// workgroup per system, grid and lists in LDS).  Cells are at least rc (1 + 1e-9) wide,
+// Large systems get their rows from a cell grid instead of the all-pairs scan (up to
+// BC_SPLIT workgroups per system, each with the whole grid and lists in its LDS and the
+// rows of every parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
 // so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
 // hold the same indices in the same ascending order as the all-pairs scan.
 constexpr int BC_MIN_N = 2048;     // systems from this size up use the cell grid
@@ -237,6 +238,8 @@ constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
 constexpr int BC_MAX_D = 96;       // cells per side at most
 constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
 constexpr int BC_BLOCK = 1024;
+constexpr int BC_SPLIT = 8;        // workgroups per system at most (each bins the system
+constexpr int BC_PER = 2048;       // itself and builds the rows of BC_PER particles)
 
 struct BCGrid {
     int nx, ny;
@@ -262,10 +265,12 @@ __device__ __forceinline__ int bc_coord(double x, double L, double cs, int n) {
 }
 
 __global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
-    const BSys& S = b.sys[blockIdx.x];
+    const BSys& S = b.sys[blockIdx.x / BC_SPLIT];
     if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
     BCGrid g;
     if (!bc_grid(b, S, g)) return;
+    const int part = blockIdx.x % BC_SPLIT, parts = (S.n + BC_PER - 1) / BC_PER;
+    if (part >= parts) return;
     __shared__ int cend[BC_CELLS];  // counts, then (after the fill) each cell's end
     __shared__ int lst[BC_MAX_N];   // particles (system-local) grouped by cell
     __shared__ int wsum[BC_BLOCK / WAVE];
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
     const PairParams p = bparams(b, S);
     constexpr int NONE = 0x7fffffff;
     int mx = 0;
-    for (int il = tid; il < n; il += BC_BLOCK) {
+    for (int il = part * BC_BLOCK + tid; il < n; il += parts * BC_BLOCK) {
         const long long i = off + il;
         const double4 qi = b.P[i];
         const int cx = bc_coord(qi.x, S.Lx, g.csx, g.nx), cy = bc_coord(qi.y, S.Ly, g.csy, g.ny);
@@ -863,7 +868,8 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_move, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)b->nsys), dim3(BC_BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)(b->nsys * BC_SPLIT)), dim3(BC_BLOCK), 0,
+                       st, d);
     hipLaunchKernelGGL(k_b_force, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
