@@ -639,10 +639,17 @@ __device__ __forceinline__ int first_particle_from(const Dev& d, int64_t v) {
     return d.vstart[p] == v ? p : p + 1;
 }
 
+// threads per block: the small form gives every vertex of the tile its own lane (one
+// pass over the vertices); the large form loops
+template <int LDSV>
+constexpr int bf_threads() { return BF_TILE; }
+
 template <int LDSV, bool HOT>
-__global__ __launch_bounds__(BF_TILE) void k_bfire_forces(Dev d, PairParams p, double rv,
-                                                          int kick, int gate, int rotation,
-                                                          int check_pos) {
+__global__ __launch_bounds__(bf_threads<LDSV>()) void k_bfire_forces(Dev d, PairParams p,
+                                                                     double rv, int kick,
+                                                                     int gate, int rotation,
+                                                                     int check_pos) {
+    constexpr int NT = bf_threads<LDSV>();
     if (bfire_skip(d, gate)) return;
     if (check_pos >= 0) {  // k_bfire_check's test, made by every block (same words read)
         Ctl* c = d.ctl;
@@ -666,23 +673,60 @@ __global__ __launch_bounds__(BF_TILE) void k_bfire_forces(Dev d, PairParams p, d
     const int np = p1 - p0;
     const int64_t lo = d.vstart[p0];
     const int64_t hi = p1 < d.N ? (int64_t)d.vstart[p1] : d.Nv;
-    for (int64_t v = lo + threadIdx.x; v < hi; v += BF_TILE) {
+    // the closing kick's per-particle loads (np <= BF_TILE <= NT: one particle per lane),
+    // issued before the vertex pass so their latency overlaps it
+    const bool own = (int)threadIdx.x < np;
+    double2 vq = make_double2(0.0, 0.0);
+    double wq = 0.0, iq = 1.0, mq = 1.0;
+    if (own && kick) {
+        const int i = p0 + threadIdx.x;
+        vq = d.vel[i];
+        wq = d.omega[i];
+        iq = d.moi[i];
+        mq = d.P0[i].w;
+    }
+    for (int64_t v = lo + threadIdx.x; v < hi; v += NT) {
         const double4 cq = d.P0[d.vpart[v]];
         const int nn = min(d.vnum[v], d.vcap);
         const double2 xv = d.vpos[v];
         double ax = 0.0, ay = 0.0, e = 0.0;
-        for (int k = 0; k < nn; ++k) {
-            int u = d.vnbr[(int64_t)k * d.Nv + v];
-            double2 xu = d.vpos[u];
-            double fx, fy, ie;
-            if constexpr (HOT)
-                ie = pair_hot<true>(make_double4(xv.x, xv.y, rv, 0.0),
-                                    make_double4(xu.x, xu.y, rv, 0.0), true, p, fx, fy);
-            else
+        if constexpr (HOT) {
+            // wave-uniform row loop, two neighbours per batch with the next batch's
+            // indices loaded ahead; every ELL slot holds a valid vertex index (zero-filled
+            // at allocation, only written with indices), and a slot past the row end
+            // adds +0.0, which leaves the sums unchanged
+            const int* col = d.vnbr + v;
+            const int64_t Nv = d.Nv;
+            const int last = d.vcap - 1;
+            const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
+            int u0 = col[0], u1 = col[(int64_t)min(1, last) * Nv];
+            for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
+                const double2 x0 = d.vpos[u0], x1 = d.vpos[u1];
+                if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
+                    u0 = col[(int64_t)min(k + 2, last) * Nv];
+                    u1 = col[(int64_t)min(k + 3, last) * Nv];
+                }
+                double fx, fy, ie;
+                ie = pair_hot<true>(qv, make_double4(x0.x, x0.y, rv, 0.0), k < nn, p, fx, fy);
+                ax += fx;
+                ay += fy;
+                e += ie;
+                ie = pair_hot<true>(qv, make_double4(x1.x, x1.y, rv, 0.0), k + 1 < nn, p, fx,
+                                    fy);
+                ax += fx;
+                ay += fy;
+                e += ie;
+            }
+        } else {
+            for (int k = 0; k < nn; ++k) {
+                int u = d.vnbr[(int64_t)k * d.Nv + v];
+                double2 xu = d.vpos[u];
+                double fx, fy, ie;
                 ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
-            ax += fx;
-            ay += fy;
-            e += ie;
+                ax += fx;
+                ay += fy;
+                e += ie;
+            }
         }
         // calcTorque, include/particles/base/kernels.cuh:237-241
         double dx = xv.x - cq.x, dy = xv.y - cq.y;
@@ -695,32 +739,32 @@ __global__ __launch_bounds__(BF_TILE) void k_bfire_forces(Dev d, PairParams p, d
     }
     __syncthreads();
     // four lanes per particle, one per quantity, each summing in vertex order
-    for (int j = threadIdx.x; j < 4 * np; j += BF_TILE) {
+    for (int j = threadIdx.x; j < 4 * np; j += NT) {
         int q = j >> 2, comp = j & 3;
         int i = p0 + q;
         int64_t st = d.vstart[i] - lo;
         int nv = d.nverts[i];
         const double* src = sv[comp] + st;
         double acc = 0.0;
+#pragma unroll 8
         for (int k = 0; k < nv; ++k) acc += src[k];
         sres[comp][q] = 0.0 + acc;
     }
     __syncthreads();
     const double hdt = d.fire->dt / 2.0;
-    for (int q = threadIdx.x; q < np; q += BF_TILE) {
-        int i = p0 + q;
+    if (own) {
+        const int q = threadIdx.x, i = p0 + q;
         double fx = sres[0][q], fy = sres[1][q], t = sres[2][q];
         d.frc[i] = make_double2(fx, fy);
         d.torque[i] = t;
         d.pe[i] = sres[3][q];
         if (kick) {
-            double2 v = d.vel[i];
-            double w = d.omega[i];
-            if (rotation) w += t * hdt / d.moi[i];
+            double2 v = vq;
+            double w = wq;
+            if (rotation) w += t * hdt / iq;
             else w = 0.0;
-            double m = d.P0[i].w;
-            v.x += fx * hdt / m;
-            v.y += fy * hdt / m;
+            v.x += fx * hdt / mq;
+            v.y += fy * hdt / mq;
             d.vel[i] = v;
             d.omega[i] = w;
         }
@@ -945,8 +989,8 @@ void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate, int check_pos)
         auto k = small ? (hot ? k_bfire_forces<BF_LDS_SMALL, true>
                               : k_bfire_forces<BF_LDS_SMALL, false>)
                        : (hot ? k_bfire_forces<BF_LDS, true> : k_bfire_forces<BF_LDS, false>);
-        hipLaunchKernelGGL(k, dim3(nb), dim3(BF_TILE), 0, s->stream, s->d, p, s->rv, kick, gate,
-                           (int)s->rotation, check_pos);
+        hipLaunchKernelGGL(k, dim3(nb), dim3(small ? bf_threads<BF_LDS_SMALL>() : bf_threads<BF_LDS>()),
+                           0, s->stream, s->d, p, s->rv, kick, gate, (int)s->rotation, check_pos);
         hipLaunchKernelGGL(k_bfire_partials, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream,
                            s->d, gate);
         return;
