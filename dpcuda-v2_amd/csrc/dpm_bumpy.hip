@@ -504,7 +504,11 @@ __device__ __forceinline__ BMove bfire_move_one(const Dev& d, int64_t i, double 
     m.mid = make_double2(q.x, q.y);
     d.lpos[i] = m.old;
     d.mpos[i] = m.mid;
-    d.dth0[i] = m.dth0;
+    if (uphill) {
+        double s0, c0;
+        dpm_sincos(m.dth0, &s0, &c0);
+        d.rot0[i] = make_double2(s0, c0);
+    }
     // updateVelocities(dt/2) (kernelUpdateRigidVelocities)
     double hdt = dt / 2.0;
     if (rotation) w += t * hdt / d.moi[i];
@@ -516,6 +520,9 @@ __device__ __forceinline__ BMove bfire_move_one(const Dev& d, int64_t i, double 
     double ddx = v.x * dt, ddy = v.y * dt, dda = w * dt;
     m.dth = dda;
     d.dth[i] = dda;
+    double s1, c1;
+    dpm_sincos(dda, &s1, &c1);
+    d.rot1[i] = make_double2(s1, c1);
     double nx = q.x + ddx, ny = q.y + ddy;
     m.now = make_double2(nx, ny);
     d.P0[i] = make_double4(nx, ny, q.z, q.w);
@@ -565,13 +572,12 @@ __global__ void k_bfire_vertices(Dev d) {
     int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= d.Nv) return;
     int p = d.vpart[v];
-    double s0 = 0.0, c0 = 1.0, s1, c1;
     const int uphill = d.fire->uphill;
-    if (uphill) dpm_sincos(d.dth0[p], &s0, &c0);
-    dpm_sincos(d.dth[p], &s1, &c1);
+    // the rotations' sin/cos, formed once per particle by k_bfire_move
+    const double2 r0 = uphill ? d.rot0[p] : make_double2(0.0, 1.0), r1 = d.rot1[p];
     double4 nq = d.P0[p];
-    d.vpos[v] = bfire_vertex_one(d.vpos[v], d.lpos[p], d.mpos[p], make_double2(nq.x, nq.y), s0,
-                                 c0, s1, c1, uphill);
+    d.vpos[v] = bfire_vertex_one(d.vpos[v], d.lpos[p], d.mpos[p], make_double2(nq.x, nq.y), r0.x,
+                                 r0.y, r1.x, r1.y, uphill);
 }
 
 // Particle::checkForNeighborUpdate of the step (particle.cu:1137-1161) from the
@@ -871,7 +877,8 @@ int bumpy_alloc(dpm_system* s) {
         dalloc(s, &d.vpart, v) || dalloc(s, &d.vstart, n) || dalloc(s, &d.nverts, n) ||
         dalloc(s, &d.svidx, v) || dalloc(s, &d.vnum, v) || dalloc(s, &d.vfrc, v) ||
         dalloc(s, &d.vtq, v) || dalloc(s, &d.vpe, v) || dalloc(s, &d.lpos, n) ||
-        dalloc(s, &d.dth, n) || dalloc(s, &d.mpos, n) || dalloc(s, &d.dth0, n))
+        dalloc(s, &d.dth, n) || dalloc(s, &d.mpos, n) || dalloc(s, &d.rot0, n) ||
+        dalloc(s, &d.rot1, n))
         return -1;
     Spare& sp = s->sp;
     if (dalloc(s, &sp.angle, n) || dalloc(s, &sp.omega, n) || dalloc(s, &sp.torque, n) ||

@@ -99,7 +99,8 @@ struct Dev {
     double2* lpos;              // COM before the last drift (vertex rotation)
     double* dth;                // angle change of the last drift
     double2* mpos;              // FIRE: COM after the uphill back-step (= lpos otherwise)
-    double* dth0;               // FIRE: angle change of the uphill back-step
+    double2 *rot0, *rot1;       // FIRE: (sin, cos) of the back-step's and the step's
+                                // angle changes, formed once per particle
 };
 
 // Reorder scratch: one spare copy of every permuted array.
