@@ -19,9 +19,10 @@
 //   part      : per chunk, 9 partial sums / maxima (below)
 //
 // One batched step is a fixed sequence of launches:
-//   k_b_prep    snapshot save/restore and box scaling requested by the controller,
-//               zero velocities (minimizeFire prologue), displacement maxima
-//   k_b_move    FIRE move of systems inside a minimisation (Fire::step first half)
+//   k_b_prep_move  per slot: snapshot save/restore and box scaling requested by the
+//               controller, zero velocities (minimizeFire prologue), displacement
+//               maxima; then the FIRE move of systems inside a minimisation
+//               (Fire::step first half)
 //   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
 //   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
 //   k_b_build_cells  per large system: the same rows from a cell grid in LDS
@@ -62,7 +63,7 @@ struct BSys {
     int stopped, done, converged, uphill;
     int mode;      // BMode of the next batched step
     int rebuild;   // Verlet rebuild before the next force evaluation
-    int snap_op;   // pending for k_b_prep: 1 save, 2 restore
+    int snap_op;   // pending for b_prep_slot: 1 save, 2 restore
     int pad0;
     double scale_f;  // pending position scale (0: none)
     // jamming controller (dpm_jam)
@@ -133,9 +134,11 @@ __device__ __forceinline__ void fire_reset(const BDev& b, BSys& S) {
 }
 
 // ------------------------------------------------------------------ kernels
-__global__ __launch_bounds__(BLOCK) void k_b_prep(BDev b) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= b.nslot) return;
+// Each slot's controller work (b_prep_slot) and then its FIRE move (b_move_slot), in
+// one kernel: both touch only their own slot, so the order per slot is the order of
+// two separate passes.  Every condition that guards a ballot is uniform over a wave
+// (one system per wave).
+__device__ __forceinline__ void b_prep_slot(const BDev& b, long long i) {
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
@@ -171,9 +174,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_prep(BDev b) {
 
 // Fire::step first half (src/integrator/fire.cu:14-40): [uphill back-step with the
 // decreased dt, zero v] + kick(dt/2) + mix + drift(dt)
-__global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= b.nslot) return;
+__device__ __forceinline__ void b_move_slot(const BDev& b, long long i) {
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
@@ -207,8 +208,15 @@ __global__ __launch_bounds__(BLOCK) void k_b_move(BDev b) {
     if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_b_prep_move(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    b_prep_slot(b, i);
+    b_move_slot(b, i);
+}
+
 // Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
-// the controller's requests that k_b_prep has applied.
+// the controller's requests that b_prep_slot has applied.
 __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     // one wave per system: the lanes test the system's chunk flags together
     const int s = blockIdx.x, lane = threadIdx.x;
@@ -463,19 +471,31 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     const int* col = b.nbr + il;
     double ax = 0.0, ay = 0.0, e = 0.0;
     const bool hot = p.fast && p.fastdiv;
-    for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; ++k) {
-        const bool valid = k < nn;
-        const int j = col[(long long)min(k, b.cap - 1) * b.nslot];
-        const double4 q = b.P[valid ? j : il];
-        double fx = 0.0, fy = 0.0, ie = 0.0;
-        if (hot) {
-            ie = pair_hot<true>(qi, q, valid, p, fx, fy);
-        } else if (valid) {
-            ie = pair_law(qi.x, qi.y, qi.z, q.x, q.y, q.z, p, fx, fy);
+    // two neighbours per batch, the next batch's indices loaded ahead; every ELL slot
+    // holds a valid arena slot (zero-filled at allocation, only written with slots), and
+    // a slot past the row end adds +0.0, which leaves the sums unchanged
+    const int last = b.cap - 1;
+    int jn[2] = {col[0], col[(long long)min(1, last) * b.nslot]};
+    for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
+        const double4 q2[2] = {b.P[jn[0]], b.P[jn[1]]};
+        if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
+            jn[0] = col[(long long)min(k + 2, last) * b.nslot];
+            jn[1] = col[(long long)min(k + 3, last) * b.nslot];
         }
-        ax += fx;
-        ay += fy;
-        e += ie;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const bool valid = k + t < nn;
+            const double4 q = q2[t];
+            double fx = 0.0, fy = 0.0, ie = 0.0;
+            if (hot) {
+                ie = pair_hot<true>(qi, q, valid, p, fx, fy);
+            } else if (valid) {
+                ie = pair_law(qi.x, qi.y, qi.z, q.x, q.y, q.z, p, fx, fy);
+            }
+            ax += fx;
+            ay += fy;
+            e += ie;
+        }
     }
     double pxv = 0.0, pyv = 0.0, pe = 0.0, f2 = 0.0;
     if (live) {
@@ -720,7 +740,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
             S.status = 2;
             stop = true;
         }
-        if (restore) {  // revertToLastState: positions (k_b_prep) and box
+        if (restore) {  // revertToLastState: positions (b_prep_slot) and box
             S.snap_op = 2;
             S.Lx = S.snapLx;
             S.Ly = S.snapLy;
@@ -864,8 +884,7 @@ static void enqueue_batch_step(dpm_batch* b) {
     const BDev& d = b->d;
     hipStream_t st = b->stream;
     dim3 slots((unsigned)((b->nslot + BLOCK - 1) / BLOCK));
-    hipLaunchKernelGGL(k_b_prep, slots, dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_move, slots, dim3(BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_prep_move, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)(b->nsys * BC_SPLIT)), dim3(BC_BLOCK), 0,
