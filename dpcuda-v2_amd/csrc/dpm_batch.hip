@@ -86,8 +86,11 @@ struct BDev {
     int *nnum, *nbr;
     double* part;      // BP_COUNT arrays of nslot/64 chunk values
     int2* tiles;       // (system, first slot) of each build tile
+    int2* ctiles;      // (system, part) of each cell-grid build workgroup
+    long long nctiles; // capacity of ctiles (the cell-grid build's grid size)
     int* flags;        // [0] capacity overflow, [1] largest row, [2] systems finished,
-                       // [3] batched steps executed, [4] build tiles in use
+                       // [3] batched steps executed, [4] build tiles in use,
+                       // [5] cell-grid workgroups in use
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double e_c, n_c, ncm1, two_nc, inv2nc;
@@ -236,9 +239,9 @@ __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     }
 }
 
-// Large systems get their rows from a cell grid instead of the all-pairs scan (up to
-// BC_SPLIT workgroups per system, each with the whole grid and lists in its LDS and the
-// rows of every parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
+// Large systems get their rows from a cell grid instead of the all-pairs scan (one
+// workgroup per BC_PER particles, listed in ctiles, each with the whole grid and lists
+// in its LDS and the rows of every parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
 // so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
 // hold the same indices in the same ascending order as the all-pairs scan.
 constexpr int BC_MIN_N = 2048;     // systems from this size up use the cell grid
@@ -246,8 +249,8 @@ constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
 constexpr int BC_MAX_D = 96;       // cells per side at most
 constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
 constexpr int BC_BLOCK = 1024;
-constexpr int BC_SPLIT = 8;        // workgroups per system at most (each bins the system
-constexpr int BC_PER = 2048;       // itself and builds the rows of BC_PER particles)
+constexpr int BC_PER = 2048;       // particles per workgroup (each bins the whole system
+                                   // itself, then builds the rows of its part)
 
 struct BCGrid {
     int nx, ny;
@@ -273,12 +276,13 @@ __device__ __forceinline__ int bc_coord(double x, double L, double cs, int n) {
 }
 
 __global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
-    const BSys& S = b.sys[blockIdx.x / BC_SPLIT];
+    if ((int)blockIdx.x >= b.flags[5]) return;
+    const int2 ct = b.ctiles[blockIdx.x];
+    const BSys& S = b.sys[ct.x];
     if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
     BCGrid g;
     if (!bc_grid(b, S, g)) return;
-    const int part = blockIdx.x % BC_SPLIT, parts = (S.n + BC_PER - 1) / BC_PER;
-    if (part >= parts) return;
+    const int part = ct.y, parts = (S.n + BC_PER - 1) / BC_PER;
     __shared__ int cend[BC_CELLS];  // counts, then (after the fill) each cell's end
     __shared__ int lst[BC_MAX_N];   // particles (system-local) grouped by cell
     __shared__ int wsum[BC_BLOCK / WAVE];
@@ -773,6 +777,7 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
 // =========================================================================
 struct dpm_batch {
     int64_t nsys = 0, nslot = 0, ntiles = 0;
+    int nct = 0;  // cell-grid build workgroups of a batch (ctiles entries)
     int device = 0;
     hipStream_t stream = nullptr;
     dpm::BDev d{};
@@ -793,6 +798,14 @@ struct dpm_batch {
 };
 
 namespace dpm {
+
+// The cell-grid build workgroups of a system: one per BC_PER particles when the system
+// is in the cell-grid size range (k_b_build_cells re-checks the grid for the box of the
+// step and leaves small boxes to k_b_build).
+static void cell_tiles(std::vector<int2>& out, int s, int n) {
+    if (n < BC_MIN_N || n > BC_MAX_N) return;
+    for (int part = 0; part < (n + BC_PER - 1) / BC_PER; ++part) out.push_back(make_int2(s, part));
+}
 
 // Copies and fills are ordered on the batch's own stream (never the legacy stream, which
 // would tie them to graph captures of other threads' systems).
@@ -830,13 +843,16 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
     d.nsys = nsys;
     d.nslot = nslot;
     d.ntiles = tiles_cap;
+    // parts = ceil(n / BC_PER) <= n / 1024 for n >= BC_MIN_N, and the systems share nslot
+    d.nctiles = nslot / 1024 + 1;
     d.cap = 32;
     DPM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     if (balloc_n(b, &d.sys, nsys) || balloc_n(b, &d.sys_of, nslot) || balloc_n(b, &d.P, nslot) ||
         balloc_n(b, &d.snap, nslot) || balloc_n(b, &d.vel, nslot) || balloc_n(b, &d.frc, nslot) ||
         balloc_n(b, &d.lref, nslot) || balloc_n(b, &d.pe, nslot) || balloc_n(b, &d.nnum, nslot) ||
         balloc_n(b, &d.nbr, nslot * d.cap) || balloc_n(b, &d.part, (nslot / WAVE) * BP_COUNT) ||
-        balloc_n(b, &d.tiles, tiles_cap) || balloc_n(b, &d.flags, 8) ||
+        balloc_n(b, &d.tiles, tiles_cap) || balloc_n(b, &d.ctiles, d.nctiles) ||
+        balloc_n(b, &d.flags, 8) ||
         balloc_n(b, &d.done_list, nsys))
         return -1;
     DPM_HIP(bsync(b, hipMemsetAsync(d.sys_of, 0xff, sizeof(int) * nslot, b->stream)));  // -1: free
@@ -887,8 +903,7 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_prep_move, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)(b->nsys * BC_SPLIT)), dim3(BC_BLOCK), 0,
-                       st, d);
+    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)d.nctiles), dim3(BC_BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_force, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
@@ -966,6 +981,12 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
                                      hipMemcpyHostToDevice, b->stream)) == hipSuccess &&
              bsync(b, hipMemcpyAsync(d.tiles, tiles.data(), sizeof(int2) * tiles.size(),
                                      hipMemcpyHostToDevice, b->stream)) == hipSuccess;
+        std::vector<int2> ct;
+        for (int64_t s = 0; s < n_systems; ++s) cell_tiles(ct, (int)s, b->n[s]);
+        b->nct = (int)ct.size();
+        if (ok && b->nct)
+            ok = bsync(b, hipMemcpyAsync(d.ctiles, ct.data(), sizeof(int2) * ct.size(),
+                                         hipMemcpyHostToDevice, b->stream)) == hipSuccess;
     }
     b->hsys.assign(n_systems, BSys{});
     for (int64_t s = 0; s < n_systems; ++s) {
@@ -1121,8 +1142,8 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
     DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
                            b->stream));
     DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
-    const int nt = (int)b->ntiles;  // lives until the synchronisation below
-    DPM_HIP(hipMemcpyAsync(d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    const int nt[2] = {(int)b->ntiles, b->nct};  // live until the synchronisation below
+    DPM_HIP(hipMemcpyAsync(d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (b->exec) {  // the parameters are baked into the graph's kernel arguments
         (void)hipGraphExecDestroy(b->exec);
@@ -1345,11 +1366,17 @@ static void serve_upload_tiles(dpm_server* sv) {
         for (int64_t t = 0; t < q.n; t += BLOCK)
             sv->tiles_host.push_back(make_int2((int)s, (int)(q.off + t)));
     }
-    int nt = (int)sv->tiles_host.size();
-    if (nt)
-        (void)bsync(b, hipMemcpyAsync(b->d.tiles, sv->tiles_host.data(), sizeof(int2) * nt,
+    std::vector<int2> ct;
+    for (size_t s = 0; s < sv->sys_req.size(); ++s)
+        if (sv->sys_req[s]) cell_tiles(ct, (int)s, (int)sv->reqs.at(sv->sys_req[s]).n);
+    int nt[2] = {(int)sv->tiles_host.size(), (int)ct.size()};
+    if (nt[0])
+        (void)bsync(b, hipMemcpyAsync(b->d.tiles, sv->tiles_host.data(), sizeof(int2) * nt[0],
                                       hipMemcpyHostToDevice, b->stream));
-    (void)bsync(b, hipMemcpyAsync(b->d.flags + 4, &nt, sizeof(int), hipMemcpyHostToDevice,
+    if (nt[1])
+        (void)bsync(b, hipMemcpyAsync(b->d.ctiles, ct.data(), sizeof(int2) * nt[1],
+                                      hipMemcpyHostToDevice, b->stream));
+    (void)bsync(b, hipMemcpyAsync(b->d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice,
                                   b->stream));
 }
 
