@@ -71,6 +71,25 @@ def test_disk_requests_arriving_during_a_batch():
     sv.close()
 
 
+def test_large_disk_requests_with_cell_grid_rows():
+    # requests of 2,048+ particles get their rows from the cell grid; the 4,000-particle
+    # request waits until the first two leave, then takes their arena range (the grid
+    # workgroups are listed again at each admission)
+    prm = P.jam_params(pressure_target=1e-3, window_rel=0.5, strain_step=1e-2,
+                       max_compression_steps=30,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-8, max_steps=300))
+    sts = [W.disk_config(n, 0.80, seed=s) for n, s in ((2100, 31), (300, 32), (4000, 33))]
+    sv = P.Server(max_systems=4, arena_slots=4224, budget_bytes=1 << 30, bumpy_workers=0)
+    ids = [sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm) for st in sts]
+    run_until_done(sv, ids)
+    for k, st in enumerate(sts):
+        got = sv.result(ids[k])
+        say("large request", k, st.n, got["result"].status, got["result"].fire_steps_total)
+        check(got, alone_disk(st, prm))
+    sv.close()
+
+
 def test_admission_waits_for_space():
     # an arena of 256 slots holds one 200-particle system at a time: requests queue
     sts = [W.disk_config(n, 0.80, seed=20 + n) for n in (200, 150, 180)]
