@@ -459,6 +459,10 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
 
 // zero + calculateForces + (closing) updateVelocities(dt/2), chunk partials of
 // F.v, PE and max |F|^2 (k_fire_force's arithmetic)
+// FASTDIV (the batch's pair law allows the short sqrt/division sequences): pair_hot, with
+// the reference's minimum image for a box too small for the branch-free one; otherwise
+// pair_law.  The two instantiations keep pair_law's registers out of the usual one.
+template <bool FASTDIV>
 __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
@@ -474,7 +478,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     const int nn = live ? min(b.nnum[il], b.cap) : 0;
     const int* col = b.nbr + il;
     double ax = 0.0, ay = 0.0, e = 0.0;
-    const bool hot = p.fast && p.fastdiv;
+    const bool fast = p.fast;
     // two neighbours per batch, the next batch's indices loaded ahead; every ELL slot
     // holds a valid arena slot (zero-filled at allocation, only written with slots), and
     // a slot past the row end adds +0.0, which leaves the sums unchanged
@@ -491,8 +495,9 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
             const bool valid = k + t < nn;
             const double4 q = q2[t];
             double fx = 0.0, fy = 0.0, ie = 0.0;
-            if (hot) {
-                ie = pair_hot<true>(qi, q, valid, p, fx, fy);
+            if constexpr (FASTDIV) {
+                if (fast) ie = pair_hot<true>(qi, q, valid, p, fx, fy);
+                else ie = pair_hot<true, true>(qi, q, valid, p, fx, fy);
             } else if (valid) {
                 ie = pair_law(qi.x, qi.y, qi.z, q.x, q.y, q.z, p, fx, fy);
             }
@@ -904,7 +909,10 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)d.nctiles), dim3(BC_BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_force, slots, dim3(BLOCK), 0, st, d);
+    if (d.fastdiv)
+        hipLaunchKernelGGL(k_b_force<true>, slots, dim3(BLOCK), 0, st, d);
+    else
+        hipLaunchKernelGGL(k_b_force<false>, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_jam, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);

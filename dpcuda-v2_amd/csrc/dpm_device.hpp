@@ -139,12 +139,15 @@ __device__ __forceinline__ double pbc_rint(double a, double b, double L, double 
 // the branch-free minimum image, the contact test folded into selects and the short
 // sqrt/division sequences; bit-identical to pair_law.  `valid` false (a slot past the
 // row end) contributes +0.0, which leaves the running sums unchanged.  Needs
-// p.fast && p.fastdiv; pair_law covers the other cases.
-template <bool ENERGY>
+// p.fast && p.fastdiv (EXACT_IMAGE: p.fastdiv only, with the reference's minimum image);
+// pair_law covers the other cases.
+template <bool ENERGY, bool EXACT_IMAGE = false>
 __device__ __forceinline__ double pair_hot(double4 qi, double4 q, bool valid, const PairParams& p,
                                            double& fx, double& fy) {
-    double dx = pbc_rint(qi.x, q.x, p.Lx, p.invLx);
-    double dy = pbc_rint(qi.y, q.y, p.Ly, p.invLy);
+    double dx = EXACT_IMAGE ? pbc(qi.x, q.x, p.Lx, p.invLx, p.halfLx, 0)
+                            : pbc_rint(qi.x, q.x, p.Lx, p.invLx);
+    double dy = EXACT_IMAGE ? pbc(qi.y, q.y, p.Ly, p.invLy, p.halfLy, 0)
+                            : pbc_rint(qi.y, q.y, p.Ly, p.invLy);
     double rs = qi.z + q.z;
     double d2 = dx * dx + dy * dy;
     bool contact = valid && d2 < rs * rs;

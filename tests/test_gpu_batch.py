@@ -82,6 +82,22 @@ def test_large_systems_cell_built_rows():
         assert np.array_equal(bx, x) and np.array_equal(by, y), k
 
 
+def test_box_smaller_than_twice_the_cutoff():
+    # 6 disks: half the box is under the cutoff, so the batched force pass takes the
+    # reference's minimum image (round half away) instead of the branch-free one
+    sts = systems([(6, 0.80, 41), (64, 0.80, 42)])
+    assert sts[0].side < 2 * 1.7
+    prm = params(max_steps=800)
+    res, pos, steps = batched(sts, prm)
+    for k, st in enumerate(sts):
+        r, x, y = plain(st, prm)
+        say("system", k, st.n, st.side, r.status, r.fire_steps_total)
+        for f in FIELDS:
+            assert getattr(res[k], f) == getattr(r, f), (k, f, getattr(res[k], f), getattr(r, f))
+        bx, by, side = pos[k]
+        assert np.array_equal(bx, x) and np.array_equal(by, y), k
+
+
 def test_result_same_alone_and_in_any_mix():
     sts = systems(SPEC)
     prm = params(max_steps=800)
