@@ -25,7 +25,7 @@
 //               (Fire::step first half)
 //   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
 //   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
-//   k_b_build_cells  per large system: the same rows from a cell grid in LDS
+//   k_b_build_cells  systems of 128+ particles: the same rows from a cell grid in LDS
 //   k_b_force   forces (+ closing kick) and the chunk partials of P, PE, max F^2
 //   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision
 //   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials
@@ -239,12 +239,13 @@ __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     }
 }
 
-// Large systems get their rows from a cell grid instead of the all-pairs scan (one
-// workgroup per BC_PER particles, listed in ctiles, each with the whole grid and lists
-// in its LDS and the rows of every parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
+// Systems of BC_MIN_N particles and more get their rows from a cell grid instead of the
+// all-pairs scan (one work item per BC_PER particles, listed in ctiles; the workgroup
+// holding an item bins the whole system into its LDS and builds the rows of every
+// parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
 // so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
 // hold the same indices in the same ascending order as the all-pairs scan.
-constexpr int BC_MIN_N = 2048;     // systems from this size up use the cell grid
+constexpr int BC_MIN_N = 128;      // systems from this size up use the cell grid
 constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
 constexpr int BC_MAX_D = 96;       // cells per side at most
 constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
@@ -275,17 +276,16 @@ __device__ __forceinline__ int bc_coord(double x, double L, double cs, int n) {
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
-__global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
-    if ((int)blockIdx.x >= b.flags[5]) return;
-    const int2 ct = b.ctiles[blockIdx.x];
+// One work item (system, part): every early return is uniform over the workgroup.
+// cend: counts, then (after the fill) each cell's end; lst: the system's particles
+// (system-local) grouped by cell.
+__device__ __forceinline__ void bc_build_item(const BDev& b, int2 ct, int* cend, int* lst,
+                                              int* wsum) {
     const BSys& S = b.sys[ct.x];
     if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
     BCGrid g;
     if (!bc_grid(b, S, g)) return;
     const int part = ct.y, parts = (S.n + BC_PER - 1) / BC_PER;
-    __shared__ int cend[BC_CELLS];  // counts, then (after the fill) each cell's end
-    __shared__ int lst[BC_MAX_N];   // particles (system-local) grouped by cell
-    __shared__ int wsum[BC_BLOCK / WAVE];
     const int tid = threadIdx.x, n = S.n, ncell = g.nx * g.ny;
     const long long off = S.off;
     for (int c = tid; c < ncell; c += BC_BLOCK) cend[c] = 0;
@@ -397,6 +397,20 @@ __global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
     if (lane == 0) {
         atomicMax(&b.flags[1], mx);
         if (mx > b.cap) atomicMax(&b.flags[0], 1);
+    }
+}
+
+// A fixed grid of BC_GRID workgroups (the LDS holds one per CU) walks the work list, so
+// the graph-baked launch stays small whatever the list's capacity.
+constexpr int BC_GRID = 512;
+__global__ __launch_bounds__(BC_BLOCK) void k_b_build_cells(BDev b) {
+    __shared__ int cend[BC_CELLS];
+    __shared__ int lst[BC_MAX_N];
+    __shared__ int wsum[BC_BLOCK / WAVE];
+    const int count = b.flags[5];
+    for (int w = blockIdx.x; w < count; w += gridDim.x) {
+        bc_build_item(b, b.ctiles[w], cend, lst, wsum);
+        __syncthreads();  // the next item reuses the LDS
     }
 }
 
@@ -848,8 +862,9 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
     d.nsys = nsys;
     d.nslot = nslot;
     d.ntiles = tiles_cap;
-    // parts = ceil(n / BC_PER) <= n / 1024 for n >= BC_MIN_N, and the systems share nslot
-    d.nctiles = nslot / 1024 + 1;
+    // parts = ceil(n / BC_PER) <= n / BC_PER + n / BC_MIN_N for n >= BC_MIN_N, and the
+    // systems share the nslot slots
+    d.nctiles = nslot / BC_PER + nslot / BC_MIN_N + 1;
     d.cap = 32;
     DPM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     if (balloc_n(b, &d.sys, nsys) || balloc_n(b, &d.sys_of, nslot) || balloc_n(b, &d.P, nslot) ||
@@ -908,7 +923,8 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_prep_move, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)d.nctiles), dim3(BC_BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)std::min<long long>(d.nctiles, BC_GRID)),
+                       dim3(BC_BLOCK), 0, st, d);
     if (d.fastdiv)
         hipLaunchKernelGGL(k_b_force<true>, slots, dim3(BLOCK), 0, st, d);
     else
@@ -992,6 +1008,7 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
         std::vector<int2> ct;
         for (int64_t s = 0; s < n_systems; ++s) cell_tiles(ct, (int)s, b->n[s]);
         b->nct = (int)ct.size();
+        ok = ok && (long long)ct.size() <= d.nctiles;  // always holds (bound of batch_alloc)
         if (ok && b->nct)
             ok = bsync(b, hipMemcpyAsync(d.ctiles, ct.data(), sizeof(int2) * ct.size(),
                                          hipMemcpyHostToDevice, b->stream)) == hipSuccess;
@@ -1377,7 +1394,8 @@ static void serve_upload_tiles(dpm_server* sv) {
     std::vector<int2> ct;
     for (size_t s = 0; s < sv->sys_req.size(); ++s)
         if (sv->sys_req[s]) cell_tiles(ct, (int)s, (int)sv->reqs.at(sv->sys_req[s]).n);
-    int nt[2] = {(int)sv->tiles_host.size(), (int)ct.size()};
+    // the bound of batch_alloc: ct never exceeds d.nctiles entries
+    int nt[2] = {(int)sv->tiles_host.size(), (int)std::min<long long>(ct.size(), b->d.nctiles)};
     if (nt[0])
         (void)bsync(b, hipMemcpyAsync(b->d.tiles, sv->tiles_host.data(), sizeof(int2) * nt[0],
                                       hipMemcpyHostToDevice, b->stream));

@@ -653,8 +653,11 @@ constexpr int bf_threads() { return BF_TILE; }
 // At most 72 VGPRs (seven waves per SIMD instead of six): 4 % faster force passes at
 // N = 65,536 than the unconstrained 74-register build; a 64-register build (eight waves)
 // spills and is slower.
+template <int LDSV>
+constexpr int bf_waves() { return LDSV == BF_LDS_SMALL ? 7 : 1; }  // the large form: LDS-bound
+
 template <int LDSV, bool HOT>
-__global__ __launch_bounds__(bf_threads<LDSV>(), 7) void k_bfire_forces(Dev d, PairParams p,
+__global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_forces(Dev d, PairParams p,
                                                                      double rv, int kick,
                                                                      int gate, int rotation,
                                                                      int check_pos) {

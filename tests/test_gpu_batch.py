@@ -66,7 +66,7 @@ def test_batched_equals_each_system_alone():
 
 
 def test_large_systems_cell_built_rows():
-    """Systems of 2048 particles and more get their Verlet rows from a cell grid; the rows,
+    """Systems of 128 particles and more get their Verlet rows from a cell grid; the rows,
     and so every result bit, equal the all-pairs rows of the plain path.  The largest
     system fills the grid's particle capacity (16,384)."""
     sts = systems([(2048, 0.80, 11), (4500, 0.82, 12), (300, 0.80, 13), (16384, 0.80, 14)])

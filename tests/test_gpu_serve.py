@@ -72,7 +72,7 @@ def test_disk_requests_arriving_during_a_batch():
 
 
 def test_large_disk_requests_with_cell_grid_rows():
-    # requests of 2,048+ particles get their rows from the cell grid; the 4,000-particle
+    # requests of 128+ particles get their rows from the cell grid; the 4,000-particle
     # request waits until the first two leave, then takes their arena range (the grid
     # workgroups are listed again at each admission)
     prm = P.jam_params(pressure_target=1e-3, window_rel=0.5, strain_step=1e-2,
