@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Headline benchmark: FP64 NVE of N=262,144 bidisperse disks on one MI355X.
 
-Workload (config 2): 50/50 diameters 1.0/1.4 at packing fraction 0.85 in a periodic
+Workload: 50/50 diameters 1.0/1.4 at packing fraction 0.85 in a periodic
 square box, harmonic contacts (e_c=1, n_c=2, m=1), seeded uniform positions relaxed
 by 200 FIRE steps, Gaussian velocities at T=1e-5, dt=0.01, cell list with
 rc = 1.4 + 0.3 and rebuild when the max displacement exceeds 0.15.
@@ -170,7 +170,7 @@ def rigid_fire(args, P, W):
         "dtype": "f64",
         "data": "synthetic (seeded uniform positions and orientations)",
         "config": {
-            "workload": "config 3: FP64 FIRE of N=65536 rigid bumpy particles (diameters "
+            "workload": "FP64 FIRE of N=65536 rigid bumpy particles (diameters "
                         "1.0 / 1.4 with 20 / 28 vertices, 50/50), phi=0.80, vertex radius "
                         "from the touching rule, contact scale (2 r_v / d_min)^2, to "
                         "|F|max < 1e-12, cell list",
@@ -298,9 +298,9 @@ def nve_logged(args, P, W):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (config-2 state written as a reference-format input directory)",
+        "data": "synthetic (the N=262144 disk state written as a reference-format input directory)",
         "config": {
-            "workload": "config-2 disks (N=262144, phi=0.85, FIRE-relaxed) NVE dt_dimless "
+            "workload": "disks (N=262144, phi=0.85, FIRE-relaxed) NVE dt_dimless "
                         "0.01 through run_nve <run directory> with the standard neighbour "
                         "config (cutoff 1.5 d_max, cell list): energy.csv every 100 steps "
                         "(PE/N, KE/N, TE/N, T, phi, P, Zp), positions + velocities every "
@@ -321,7 +321,7 @@ def nve_logged(args, P, W):
 def pairs(args, P, W):
     """Per-pair analytics (dpm_pair_analytics: energies, forces, separations, ids, overlaps,
     angles, 2x2 Hessian blocks for every neighbour-list slot, in the reference's layout) on
-    the config-2 state.  value = neighbour-list slots per second; the kernel writes every
+    the N=262,144 disk state (phi 0.85).  value = neighbour-list slots per second; the kernel writes every
     slot of every field, so it is bound by HBM writes."""
     import numpy as np
     n = args.n
@@ -377,9 +377,9 @@ def pairs(args, P, W):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (config-2 state after 200 FIRE and 200 NVE steps)",
+        "data": "synthetic (N=262144 disk state after 200 FIRE and 200 NVE steps)",
         "config": {
-            "workload": "Disk::calculateForceDistancePairs on config 2: N=262144 bidisperse "
+            "workload": "Disk::calculateForceDistancePairs on N=262144 bidisperse "
                         "disks, phi=0.85, cell list rc=1.7; 18 per-slot fields",
             "n_particles": n,
             "row_length": row,
@@ -767,7 +767,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded uniform positions, FIRE-relaxed, Gaussian velocities)",
         "config": {
-            "workload": "config 2: FP64 NVE, N=262144 bidisperse disks (1.0/1.4, 50/50), "
+            "workload": "FP64 NVE, N=262144 bidisperse disks (1.0/1.4, 50/50), "
                         "phi=0.85, T=1e-5, dt=0.01, cell list rc=1.7, rebuild at max "
                         "displacement > 0.15",
             "n_particles": n,

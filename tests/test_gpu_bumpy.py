@@ -219,7 +219,7 @@ def test_snapshot_restore():
 
 def test_config3_size_properties():
     n = 65536
-    say("config-3 size", n)
+    say("rigid benchmark size", n)
     st = W.bumpy_config(n, 0.8, seed=12345)
     g, o = bpair(st)
     gi, gj = sorted_pairs(*g.vertex_pairs())

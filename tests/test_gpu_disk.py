@@ -46,7 +46,7 @@ def forces(s):
 # ------------------------------------------------------------------ fixture
 def test_golden_fixture_config1():
     z = np.load(f"{ROOT}/tests/golden/disk_n1024.npz")
-    say("golden config-1 N=1024")
+    say("golden N=1024 phi=0.80")
     st = W.DiskState(1024, float(z["side"]), z["x"], z["y"], z["rad"], z["mass"])
     spec = W.NeighborSpec(3, float(z["cutoff"]), float(z["disp_thresh_sq"]),
                           int(z["n_cells_dim"]), float(z["cell_disp_thresh_sq"]))
@@ -318,7 +318,7 @@ def test_snapshot_restore_and_scaling():
 # ------------------------------------------------------------------ full size
 def test_config2_size_properties():
     n = 262144
-    say("config-2 size", n)
+    say("benchmark size", n)
     st = W.disk_config(n, 0.85, seed=12345)
     g, o = pair(st)
     gi, gj = g.neighbor_pairs()

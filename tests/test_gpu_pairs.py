@@ -5,7 +5,7 @@ Every field is compared slot for slot in the reference's layout (rows in the int
 order, max_neighbors_allocated slots per row): bit-identical except the separation angle
 and the particle area, which use the device library's atan2 / acos and are held to 2 ulp.
 Cases: disks with cell and Verlet lists after NVE steps that reorder the particles, a
-non-harmonic exponent, the config-2 size (262,144 disks, checked against size-independent
+non-harmonic exponent, the 262,144-disk benchmark size (φ = 0.85, checked against size-independent
 properties), and rigid bumpy rings with rotation."""
 import numpy as np
 import pytest
