@@ -192,7 +192,9 @@ def rigid_fire(args, P, W):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": force_gbs / HBM_PEAK_GBS if force_gbs else None,
-            "traffic": None,
+            # calibrated FETCH_SIZE + WRITE_SIZE of k_bfire_forces (the partials kernel
+            # beside it moves under 2 MB)
+            "traffic": traffic_from_profiles(n, "k_bfire_forces_rigid_n65536_traffic.json"),
             "algorithmic_bytes_per_launch": force_bytes,
             "force_phase_us": force_us,
             "move_phase_us": move_us,
@@ -617,8 +619,8 @@ def serve(args, P, W):
     print(json.dumps(out))
 
 
-def traffic_from_profiles(n):
-    path = os.path.join(ROOT, "profiles", "k_nve_fused_n262144_traffic.json")
+def traffic_from_profiles(n, name="k_nve_fused_n262144_traffic.json"):
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:

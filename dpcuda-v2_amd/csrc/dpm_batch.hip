@@ -478,7 +478,9 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
 // pair_law.  The two instantiations keep pair_law's registers out of the usual one.
 template <bool FASTDIV>
 __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    // XCD-aware block order: a system's slots (and the neighbours they gather) stay in
+    // one XCD's L2
+    const long long i = (long long)xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)

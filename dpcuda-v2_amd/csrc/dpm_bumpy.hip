@@ -678,7 +678,8 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_
     }
     __shared__ double sv[4][LDSV];       // per vertex: fx, fy, torque, energy
     __shared__ double sres[4][BF_TILE];  // per particle: the sums of those
-    const int64_t b0 = (int64_t)blockIdx.x * BF_TILE;
+    // XCD-aware tile order: each XCD's L2 holds the vertex positions its tiles gather
+    const int64_t b0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * BF_TILE;
     const int p0 = first_particle_from(d, b0);
     const int p1 = first_particle_from(d, b0 + BF_TILE);
     if (p0 >= p1) return;
