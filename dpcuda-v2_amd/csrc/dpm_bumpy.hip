@@ -650,11 +650,12 @@ __device__ __forceinline__ int first_particle_from(const Dev& d, int64_t v) {
 template <int LDSV>
 constexpr int bf_threads() { return BF_TILE; }
 
-// At most 72 VGPRs (seven waves per SIMD instead of six): 4 % faster force passes at
-// N = 65,536 than the unconstrained 74-register build; a 64-register build (eight waves)
-// spills and is slower.
+// Eight waves per SIMD (62 VGPRs, no spills): the closing kick loads its per-particle
+// values after the vertex pass instead of holding them through it (issued first they
+// cost ten registers: six waves unconstrained, or seven with two spilled registers,
+// whose scratch stores were most of the pass's HBM writes).
 template <int LDSV>
-constexpr int bf_waves() { return LDSV == BF_LDS_SMALL ? 7 : 1; }  // the large form: LDS-bound
+constexpr int bf_waves() { return LDSV == BF_LDS_SMALL ? 8 : 1; }  // the large form: LDS-bound
 
 template <int LDSV, bool HOT>
 __global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_forces(Dev d, PairParams p,
@@ -686,18 +687,8 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_
     const int np = p1 - p0;
     const int64_t lo = d.vstart[p0];
     const int64_t hi = p1 < d.N ? (int64_t)d.vstart[p1] : d.Nv;
-    // the closing kick's per-particle loads (np <= BF_TILE <= NT: one particle per lane),
-    // issued before the vertex pass so their latency overlaps it
+    // the per-particle phase below: np <= BF_TILE <= NT, one particle per lane
     const bool own = (int)threadIdx.x < np;
-    double2 vq = make_double2(0.0, 0.0);
-    double wq = 0.0, iq = 1.0, mq = 1.0;
-    if (own && kick) {
-        const int i = p0 + threadIdx.x;
-        vq = d.vel[i];
-        wq = d.omega[i];
-        iq = d.moi[i];
-        mq = d.P0[i].w;
-    }
     for (int64_t v = lo + threadIdx.x; v < hi; v += NT) {
         const double4 cq = d.P0[d.vpart[v]];
         const int nn = min(d.vnum[v], d.vcap);
@@ -772,10 +763,11 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_
         d.torque[i] = t;
         d.pe[i] = sres[3][q];
         if (kick) {
-            double2 v = vq;
-            double w = wq;
-            if (rotation) w += t * hdt / iq;
+            double2 v = d.vel[i];
+            double w = d.omega[i];
+            if (rotation) w += t * hdt / d.moi[i];
             else w = 0.0;
+            const double mq = d.P0[i].w;
             v.x += fx * hdt / mq;
             v.y += fy * hdt / mq;
             d.vel[i] = v;
