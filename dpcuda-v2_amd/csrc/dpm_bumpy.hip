@@ -654,11 +654,12 @@ constexpr int bf_threads() { return BF_TILE; }
 // values after the vertex pass instead of holding them through it (issued first they
 // cost ten registers: six waves unconstrained, or seven with two spilled registers,
 // whose scratch stores were most of the pass's HBM writes).
-template <int LDSV>
-constexpr int bf_waves() { return LDSV == BF_LDS_SMALL ? 8 : 1; }  // the large form: LDS-bound
+// (the general pair law and the large, LDS-bound form keep the compiler's own budget)
+template <int LDSV, bool HOT>
+constexpr int bf_waves() { return LDSV == BF_LDS_SMALL && HOT ? 8 : 1; }
 
 template <int LDSV, bool HOT>
-__global__ __launch_bounds__(bf_threads<LDSV>(), bf_waves<LDSV>()) void k_bfire_forces(Dev d, PairParams p,
+__global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k_bfire_forces(Dev d, PairParams p,
                                                                      double rv, int kick,
                                                                      int gate, int rotation,
                                                                      int check_pos) {
