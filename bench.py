@@ -749,6 +749,7 @@ def main():
     f_step_s = f_ms / chunk_steps / 1e3
     f_gbs = step_bytes / f_step_s / 1e9
     r_avg_us = r_ms / max(r_n, 1) * 1e3
+    traffic = traffic_from_profiles(n)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -786,11 +787,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": f_gbs / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(n),
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": step_bytes,
             "kernel_compulsory_bytes_per_launch": own_bytes,
             "kernel_us_per_step": f_step_s * 1e6,
             "timed_chunks": f_n,
+            # the same kernel time against the kernel's own compulsory bytes and against
+            # the HBM bytes the counters measure per launch
+            "frac_on_kernel_compulsory": own_bytes / f_step_s / 1e9 / HBM_PEAK_GBS,
+            "frac_on_measured_traffic": (traffic / f_step_s / 1e9 / HBM_PEAK_GBS
+                                         if traffic else None),
         },
         "cell_rebuild_recovery": {"mean_us": r_avg_us, "count": r_n},
         "cpu_baseline": cpu,

@@ -94,7 +94,7 @@ def fire_params(alpha_init=0.1, dt=1e-2, avg_pe_target=1e-16, avg_pe_diff_target
 
 
 class Request(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("n_particles", C.c_int64),
+    _fields_ = [("kind", C.c_int32), ("mode", C.c_int32), ("n_particles", C.c_int64),
                 ("n_vertices", C.c_int64), ("x", C.c_void_p), ("y", C.c_void_p),
                 ("radius", C.c_void_p), ("mass", C.c_void_p), ("angle", C.c_void_p),
                 ("vertices_per_particle", C.c_void_p), ("vertex_radius", C.c_double),
@@ -173,6 +173,8 @@ _GPU_ONLY = {
     "batch_set_option": (_I, [_P, _I, _L]),
     "batch_jam": (_I, [_P, C.POINTER(JamParams), _L, _P, C.POINTER(_L)]),
     "batch_finish_steps": (_I, [_P, _P]),
+    "batch_minimize": (_I, [_P, C.POINTER(FireParams), _L, _P, C.POINTER(_L)]),
+    "server_release": (_I, [_P, _L]),
     "server_create": (_P, [_L, _L, _L, C.c_int32]),
     "server_destroy": (None, [_P]),
     "server_submit": (_L, [_P, C.POINTER(Request)]),
@@ -533,9 +535,21 @@ class Batch:
                 C.byref(steps))
         return list(out), steps.value
 
+    def minimize(self, params: FireParams, max_batched_steps=10**9):
+        """minimizeFire of every system at once (dpm_batch_minimize)."""
+        out = (JamResult * len(self.sizes))()
+        steps = C.c_int64()
+        self._c("batch_minimize", C.byref(params), int(max_batched_steps),
+                C.cast(out, C.c_void_p), C.byref(steps))
+        return list(out), steps.value
+
+
+REQ_JAM = 0
+REQ_MINIMIZE = 1
+
 
 class Server:
-    """Continuous batching of jamming requests on one GPU (dpm_server_*)."""
+    """Continuous batching of jamming and minimisation requests on one GPU (dpm_server_*)."""
 
     def __init__(self, max_systems=4096, arena_slots=1 << 22, budget_bytes=8 << 30,
                  bumpy_workers=2):
@@ -558,11 +572,13 @@ class Server:
             pass
 
     def submit(self, x, y, rad, mass, side, jam: JamParams, e_c=1.0, angle=None,
-               vertices_per_particle=None, vertex_radius=0.0):
+               vertices_per_particle=None, vertex_radius=0.0, mode=REQ_JAM):
+        """mode REQ_JAM: dpm_jam with `jam`; REQ_MINIMIZE: minimizeFire with jam.fire."""
         arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, rad, mass)]
         r = Request()
         bumpy = angle is not None
         r.kind = BUMPY if bumpy else DISK
+        r.mode = int(mode)
         r.n_particles = len(arrs[0])
         r.x, r.y, r.radius, r.mass = [_ptr(a) for a in arrs]
         keep = list(arrs)
@@ -593,13 +609,20 @@ class Server:
         st = C.c_int32()
         out = JamResult()
         lat = C.c_double()
-        n = self.n[rid]
+        n = self.n.get(rid, 0)  # unknown (or released) ids are reported by the library
         x, y = np.empty(n), np.empty(n)
         side = C.c_double()
         if self.be.fn["server_result"](self.h, int(rid), C.byref(st), C.byref(out), C.byref(lat),
-                                       _ptr(x), _ptr(y), C.byref(side)) < 0:
+                                       _ptr(x) if n else None, _ptr(y) if n else None,
+                                       C.byref(side)) < 0:
             raise DpmError(f"dpm_server_result: {self.be.err()}")
         if st.value != 2:
             return {"state": st.value}
         return {"state": 2, "result": out, "latency_s": lat.value, "x": x, "y": y,
                 "box_side": side.value}
+
+    def release(self, rid):
+        """Forget a finished request (dpm_server_release)."""
+        if self.be.fn["server_release"](self.h, int(rid)) < 0:
+            raise DpmError(f"dpm_server_release: {self.be.err()}")
+        self.n.pop(rid, None)

@@ -49,7 +49,8 @@ struct BJam {
     double P_low, P_high, rate, dfac, ltol, zmin;
     long long max_csteps;
 };
-enum BPhase { BJ_DECOMP = 0, BJ_COMP = 1 };
+// BJ_MIN: one minimisation and nothing else (a minimisation request)
+enum BPhase { BJ_DECOMP = 0, BJ_COMP = 1, BJ_MIN = 2 };
 enum BPart { BP_PX = 0, BP_PY, BP_PE, BP_F2, BP_SXX, BP_SYY, BP_R2, BP_DMAX, BP_Z, BP_COUNT };
 
 struct BSys {
@@ -706,7 +707,12 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
     const double pr = (sxx + syy) / 2;
     const double P_low = S.J.P_low, P_high = S.J.P_high;
     bool finished = false;
-    if (S.jphase == BJ_DECOMP) {
+    if (S.jphase == BJ_MIN) {  // minimizeFire alone: its end state is the result
+        S.pr = pr;
+        S.contacts = (long long)z;
+        S.status = S.converged ? 0 : 1;
+        finished = true;
+    } else if (S.jphase == BJ_DECOMP) {
         // for (it < 1000) { minimize; if (P < P_low) break; scaleBox(f); }
         bool leave = pr < P_low;
         int op = 1;
@@ -1123,8 +1129,14 @@ int dpm_batch_set_option(dpm_batch* b, int option, int64_t value) {
     return -1;
 }
 
-int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_steps,
-                  dpm_jam_result* out, int64_t* steps_done) {
+}  // extern "C"
+
+namespace dpm {
+
+// dpm_batch_jam, or with min_only dpm_batch_minimize: every system runs its jam (its
+// minimisation) from the state as set, in one batched launch sequence.
+static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
+                     int64_t max_batched_steps, dpm_jam_result* out, int64_t* steps_done) {
     if (!P || !out || max_batched_steps < 0) {
         set_error("dpm_batch_jam: bad arguments");
         return -1;
@@ -1161,7 +1173,7 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
         f.dt_min = 0.001 * J.dt0;
         f.mode = BM_PROLOGUE;
         f.rebuild = 1;
-        f.jphase = BJ_DECOMP;
+        f.jphase = min_only ? BJ_MIN : BJ_DECOMP;
         f.status = 1;
         f.L_low = f.L_high = f.L_last = -1;
         S = f;
@@ -1229,6 +1241,26 @@ int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_ste
         r.contacts = S.contacts;
     }
     return 0;
+}
+
+}  // namespace dpm
+
+extern "C" {
+
+int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* P, int64_t max_batched_steps,
+                  dpm_jam_result* out, int64_t* steps_done) {
+    return batch_run(b, P, false, max_batched_steps, out, steps_done);
+}
+
+int dpm_batch_minimize(dpm_batch* b, const dpm_fire_params* F, int64_t max_batched_steps,
+                       dpm_jam_result* out, int64_t* steps_done) {
+    if (!F) {
+        set_error("dpm_batch_minimize: bad arguments");
+        return -1;
+    }
+    dpm_jam_params P{};
+    P.fire = *F;
+    return batch_run(b, &P, true, max_batched_steps, out, steps_done);
 }
 
 // Autotuner: runs `trial_steps` batched steps of the jam from the configured state for
@@ -1337,6 +1369,7 @@ int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out) {
 // by a worker thread on its own stream, overlapping the batch on the device.  A
 // finished disk system leaves the arena at once, so its space and entry serve the next
 // request.  Every result is the one the request's system gives when jammed alone.
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -1345,6 +1378,7 @@ int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out) {
 struct dpm_server_req {
     int64_t id = 0;
     int kind = DPM_DISK;
+    int mode = DPM_REQ_JAM;
     int64_t n = 0, nv = 0;
     std::vector<double> x, y, r, m, ang;
     std::vector<int64_t> vcounts;
@@ -1362,18 +1396,24 @@ struct dpm_server_req {
     std::vector<double4> staged;
 };
 
+// Locking: `mu` guards the request table, the queues and every counter; the batch
+// (arena, stream, graph) is driven only by the thread calling dpm_server_progress.  Rigid
+// bumpy requests go to a fixed pool of `workers` threads through `bumpy_queue`.
 struct dpm_server {
     dpm_batch* b = nullptr;
     int64_t budget = 0, used = 0;
     int workers = 1, running_bumpy = 0;
     std::mutex mu;
+    std::condition_variable cv;                  // bumpy_queue not empty, or stopping
+    bool stopping = false;
     std::map<int64_t, dpm_server_req> reqs;
-    std::deque<int64_t> queue;
+    std::deque<int64_t> queue;                   // submitted, not admitted
+    std::deque<int64_t> bumpy_queue;             // admitted rigid requests for the pool
     std::vector<int> free_sys;
     std::map<long long, long long> free_ranges;  // first slot -> length
     std::vector<int64_t> sys_req;                // system entry -> request id (0: free)
     int64_t next_id = 1, seen_done = 0, finished = 0, disk_active = 0;
-    std::vector<std::thread> threads;
+    std::vector<std::thread> pool;
     std::vector<int2> tiles_host;
     bool fastdiv = true;
 };
@@ -1436,19 +1476,26 @@ static void give_range(dpm_server* sv, long long off, long long slots) {
     }
 }
 
-// A rigid bumpy request, jammed alone on its own dpm_system (same set-up as a caller
-// would make: cell list with cutoff 1.5 d_max, vertex cutoff 3 r_v, threshold (0.2 r_v)^2,
-// the rules of RigidBumpy::setNeighborSize, rigid_bumpy.cu:1278-1300).
-static void bumpy_worker(dpm_server* sv, int64_t id) {
-    dpm_server_req* q;
-    {
+// A rigid bumpy request, jammed (or minimised) alone on its own dpm_system (same set-up
+// as a caller would make: cell list with cutoff 1.5 d_max, vertex cutoff 3 r_v,
+// threshold (0.2 r_v)^2, the rules of RigidBumpy::setNeighborSize,
+// rigid_bumpy.cu:1278-1300).  Runs on a pool thread without the server lock; the
+// request's entry stays in place while it runs (it is neither erased nor moved: the
+// table is a std::map and release refuses running requests).
+static void bumpy_job(dpm_server* sv, dpm_server_req* q) {
+    auto finish = [&](bool ok, const char* where, dpm_jam_result r, std::vector<double>&& rx,
+                      std::vector<double>&& ry, double side) {
         std::lock_guard<std::mutex> g(sv->mu);
-        q = &sv->reqs.at(id);
-    }
-    auto fail = [&](const char* where) {
-        std::lock_guard<std::mutex> g(sv->mu);
-        q->err = std::string(where) + ": " + dpm_last_error();
-        q->state = 3;
+        if (ok) {
+            q->res = r;
+            q->rx = std::move(rx);
+            q->ry = std::move(ry);
+            q->rside = side;
+            q->state = 2;
+        } else {
+            q->err = std::string(where) + ": " + dpm_last_error();
+            q->state = 3;
+        }
         q->t_done = std::chrono::steady_clock::now();
         sv->used -= q->bytes;
         sv->running_bumpy--;
@@ -1456,7 +1503,7 @@ static void bumpy_worker(dpm_server* sv, int64_t id) {
     };
     (void)hipSetDevice(sv->b->device);
     dpm_system* s = dpm_create(DPM_BUMPY, q->n, q->nv);
-    if (!s) return fail("dpm_create");
+    if (!s) return finish(false, "dpm_create", {}, {}, {}, 0.0);
     double dmax = 0.0;
     for (double r : q->r) dmax = std::max(dmax, 2.0 * r);
     double rc = 1.5 * dmax;
@@ -1472,27 +1519,58 @@ static void bumpy_worker(dpm_server* sv, int64_t id) {
               !dpm_neighbor_config(s, DPM_NBR_CELL, rc, (0.2 * q->rv) * (0.2 * q->rv), ncd,
                                    (0.5 * cs) * (0.5 * cs)) &&
               !dpm_bumpy_neighbor_config(s, 1.5 * 2.0 * q->rv, dmax) && !dpm_neighbor_init(s);
+    // the inputs now live on the device (only this thread touches them while it runs)
+    for (auto* v : {&q->x, &q->y, &q->r, &q->m, &q->ang}) std::vector<double>().swap(*v);
+    std::vector<int64_t>().swap(q->vcounts);
     dpm_jam_result r{};
-    ok = ok && !dpm_jam(s, &q->jam, &r);
+    if (ok && q->mode == DPM_REQ_MINIMIZE) {
+        dpm_fire_result fr{};
+        double pr = 0, phi = 0, pe = 0, nc = 0;
+        ok = !dpm_minimize_fire(s, &q->jam.fire, &fr) && !dpm_reduce(s, DPM_RED_PRESSURE, &pr) &&
+             !dpm_reduce(s, DPM_RED_CONTACTS, &nc) &&
+             !dpm_reduce(s, DPM_RED_PACKING_FRACTION, &phi) && !dpm_reduce(s, DPM_RED_PE, &pe);
+        r.fire_steps_total = fr.steps;
+        r.status = fr.converged ? 0 : 1;
+        r.pressure = pr;
+        r.packing_fraction = phi;
+        r.pe = pe;
+        r.box_length = q->side;
+        r.contacts = (int64_t)nc;
+    } else if (ok) {
+        ok = !dpm_jam(s, &q->jam, &r);
+    }
     std::vector<double> rx(q->n), ry(q->n);
     double lx = 0, ly = 0;
     ok = ok && !dpm_download(s, DPM_F_X, rx.data(), q->n) &&
          !dpm_download(s, DPM_F_Y, ry.data(), q->n) && !dpm_get_box(s, &lx, &ly);
-    if (!ok) {
-        dpm_destroy(s);
-        return fail("rigid bumpy jam");
-    }
     dpm_destroy(s);
-    std::lock_guard<std::mutex> g(sv->mu);
-    q->res = r;
-    q->rx = std::move(rx);
-    q->ry = std::move(ry);
-    q->rside = lx;
-    q->state = 2;
-    q->t_done = std::chrono::steady_clock::now();
-    sv->used -= q->bytes;
-    sv->running_bumpy--;
-    sv->finished++;
+    finish(ok, q->mode == DPM_REQ_MINIMIZE ? "rigid bumpy minimisation" : "rigid bumpy jam", r,
+           std::move(rx), std::move(ry), lx);
+}
+
+// A pool thread: takes admitted rigid requests until the server stops.
+static void bumpy_pool_thread(dpm_server* sv) {
+    for (;;) {
+        dpm_server_req* q = nullptr;
+        {
+            std::unique_lock<std::mutex> g(sv->mu);
+            sv->cv.wait(g, [sv] { return sv->stopping || !sv->bumpy_queue.empty(); });
+            if (sv->bumpy_queue.empty()) return;  // stopping, nothing left to run
+            q = &sv->reqs.at(sv->bumpy_queue.front());
+            sv->bumpy_queue.pop_front();
+        }
+        try {
+            bumpy_job(sv, q);
+        } catch (const std::exception& e) {  // e.g. bad_alloc: the request fails, not the process
+            std::lock_guard<std::mutex> g(sv->mu);
+            q->err = std::string("rigid bumpy worker: ") + e.what();
+            q->state = 3;
+            q->t_done = std::chrono::steady_clock::now();
+            sv->used -= q->bytes;
+            sv->running_bumpy--;
+            sv->finished++;
+        }
+    }
 }
 
 // Disk systems that finished since the last look leave the arena.  The stream is idle.
@@ -1571,7 +1649,8 @@ static int serve_admit(dpm_server* sv) {
             q.state = 1;
             sv->used += q.bytes;
             sv->running_bumpy++;
-            sv->threads.emplace_back(bumpy_worker, sv, q.id);
+            sv->bumpy_queue.push_back(q.id);
+            sv->cv.notify_one();
             it = sv->queue.erase(it);
             continue;
         }
@@ -1607,10 +1686,13 @@ static int serve_admit(dpm_server* sv) {
         f.dt_min = 0.001 * f.J.dt0;
         f.mode = BM_PROLOGUE;
         f.rebuild = 1;
-        f.jphase = BJ_DECOMP;
+        f.jphase = q.mode == DPM_REQ_MINIMIZE ? BJ_MIN : BJ_DECOMP;
         f.status = 1;
         f.L_low = f.L_high = f.L_last = -1;
         BCOPY(b->d.sys + s, &f, sizeof(BSys), hipMemcpyHostToDevice);
+        // the request's inputs now live in the arena
+        std::vector<double4>().swap(q.staged);
+        for (auto* v : {&q.x, &q.y, &q.r, &q.m}) std::vector<double>().swap(*v);
         tiles_changed = true;
         it = sv->queue.erase(it);
     }
@@ -1621,6 +1703,29 @@ static int serve_admit(dpm_server* sv) {
 }  // namespace dpm
 
 extern "C" {
+
+// Every server entry point turns a C++ exception (allocation failure, thread creation)
+// into an error return: nothing is thrown across the C ABI.
+#define SERVER_GUARD(fail_value)                                   \
+    catch (const std::exception& e) {                             \
+        set_error(std::string("dpm_server: ") + e.what());         \
+        return fail_value;                                         \
+    }                                                              \
+    catch (...) {                                                  \
+        set_error("dpm_server: unknown exception");                \
+        return fail_value;                                         \
+    }
+
+static void server_stop(dpm_server* sv) {
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stopping = true;
+    }
+    sv->cv.notify_all();
+    for (auto& t : sv->pool)
+        if (t.joinable()) t.join();
+    sv->pool.clear();
+}
 
 dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t budget_bytes,
                               int32_t bumpy_workers) {
@@ -1634,62 +1739,84 @@ dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t 
         set_error("dpm_server_create: no HIP device");
         return nullptr;
     }
-    auto* sv = new dpm_server();
-    sv->b = new dpm_batch();
-    int64_t slots = arena_slots / WAVE * WAVE;
-    dpm_batch* b = sv->b;
-    b->nsys = max_systems;
-    b->nslot = slots;
-    b->ntiles = slots / WAVE;
-    if (batch_alloc(b, max_systems, slots, b->ntiles)) {
-        std::string e = dpm_last_error();
-        dpm_batch_destroy(b);
-        delete sv;
-        set_error("dpm_server_create: " + e);
+    dpm_server* sv = nullptr;
+    try {
+        sv = new dpm_server();
+        sv->b = new dpm_batch();
+        int64_t slots = arena_slots / WAVE * WAVE;
+        dpm_batch* b = sv->b;
+        b->nsys = max_systems;
+        b->nslot = slots;
+        b->ntiles = slots / WAVE;
+        std::string err;
+        if (batch_alloc(b, max_systems, slots, b->ntiles)) {
+            err = dpm_last_error();
+        } else {
+            std::vector<BSys> empty(max_systems);
+            for (auto& S : empty) S.mode = BM_EMPTY;
+            if (bsync(b, hipMemcpyAsync(b->d.sys, empty.data(), sizeof(BSys) * max_systems,
+                                        hipMemcpyHostToDevice, b->stream)) != hipSuccess)
+                err = "device copy failed";
+        }
+        if (err.empty()) {
+            sv->budget = budget_bytes;
+            sv->workers = bumpy_workers;
+            sv->sys_req.assign(max_systems, 0);
+            for (int64_t s = max_systems - 1; s >= 0; --s) sv->free_sys.push_back((int)s);
+            sv->free_ranges[0] = slots;
+            batch_common(b, true);
+            if (b->use_graph && ensure_batch_graph(b, b->steps_per_graph)) err = dpm_last_error();
+        }
+        if (err.empty())
+            for (int w = 0; w < bumpy_workers; ++w) sv->pool.emplace_back(bumpy_pool_thread, sv);
+        if (!err.empty()) {
+            server_stop(sv);
+            dpm_batch_destroy(sv->b);
+            delete sv;
+            set_error("dpm_server_create: " + err);
+            return nullptr;
+        }
+        return sv;
+    } catch (const std::exception& e) {
+        if (sv) {
+            server_stop(sv);
+            if (sv->b) dpm_batch_destroy(sv->b);
+            delete sv;
+        }
+        set_error(std::string("dpm_server_create: ") + e.what());
         return nullptr;
     }
-    std::vector<BSys> empty(max_systems);
-    for (auto& S : empty) S.mode = BM_EMPTY;
-    if (bsync(b, hipMemcpyAsync(b->d.sys, empty.data(), sizeof(BSys) * max_systems,
-                                hipMemcpyHostToDevice, b->stream)) != hipSuccess) {
-        dpm_batch_destroy(b);
-        delete sv;
-        set_error("dpm_server_create: device copy failed");
-        return nullptr;
-    }
-    sv->budget = budget_bytes;
-    sv->workers = bumpy_workers;
-    sv->sys_req.assign(max_systems, 0);
-    for (int64_t s = max_systems - 1; s >= 0; --s) sv->free_sys.push_back((int)s);
-    sv->free_ranges[0] = slots;
-    batch_common(b, true);
-    if (b->use_graph && ensure_batch_graph(b, b->steps_per_graph)) {
-        std::string e = dpm_last_error();
-        dpm_batch_destroy(b);
-        delete sv;
-        set_error("dpm_server_create: " + e);
-        return nullptr;
-    }
-    return sv;
 }
 
+// Waits for the rigid requests already handed to the pool; queued ones never start.
 void dpm_server_destroy(dpm_server* sv) {
     if (!sv) return;
-    for (auto& t : sv->threads)
-        if (t.joinable()) t.join();
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        // requests not yet taken by a pool thread are dropped
+        for (int64_t id : sv->bumpy_queue) {
+            auto& q = sv->reqs.at(id);
+            q.state = 3;
+            sv->running_bumpy--;
+        }
+        sv->bumpy_queue.clear();
+    }
+    server_stop(sv);
     dpm_batch_destroy(sv->b);
     delete sv;
 }
 
-int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) {
+int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) try {
     if (!r || r->n_particles <= 0 || r->n_particles >= (int64_t)1 << 24 || !r->x || !r->y ||
         !r->radius || !r->mass || !(r->box_side > 0) ||
-        (r->kind != DPM_DISK && r->kind != DPM_BUMPY)) {
+        (r->kind != DPM_DISK && r->kind != DPM_BUMPY) ||
+        (r->mode != DPM_REQ_JAM && r->mode != DPM_REQ_MINIMIZE)) {
         set_error("dpm_server_submit: bad request");
         return -1;
     }
     dpm_server_req q;
     q.kind = r->kind;
+    q.mode = r->mode;
     q.n = r->n_particles;
     q.x.assign(r->x, r->x + q.n);
     q.y.assign(r->y, r->y + q.n);
@@ -1746,15 +1873,19 @@ int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) {
     sv->queue.push_back(id);
     return id;
 }
+SERVER_GUARD(-1)
 
 int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished_total,
-                        int64_t* in_flight, int64_t* queued) {
+                        int64_t* in_flight, int64_t* queued) try {
     dpm_batch* b = sv->b;
+    int64_t disk_active = 0, running_bumpy = 0;
     {
         std::lock_guard<std::mutex> g(sv->mu);
         if (serve_collect(sv) || serve_admit(sv)) return -1;
+        disk_active = sv->disk_active;
+        running_bumpy = sv->running_bumpy;
     }
-    if (sv->disk_active > 0 && batched_steps > 0) {
+    if (disk_active > 0 && batched_steps > 0) {
         int64_t done = 0;
         while (done < batched_steps) {
             int64_t k = std::min<int64_t>(b->steps_per_graph, batched_steps - done);
@@ -1770,7 +1901,7 @@ int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished
         DPM_HIP(hipStreamSynchronize(b->stream));
         std::lock_guard<std::mutex> g(sv->mu);
         if (serve_collect(sv) || serve_admit(sv)) return -1;
-    } else if (sv->running_bumpy > 0) {
+    } else if (running_bumpy > 0) {
         std::this_thread::sleep_for(std::chrono::microseconds(200));
     }
     std::lock_guard<std::mutex> g(sv->mu);
@@ -1779,9 +1910,10 @@ int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished
     if (queued) *queued = (int64_t)sv->queue.size();
     return 0;
 }
+SERVER_GUARD(-1)
 
 int dpm_server_result(dpm_server* sv, int64_t id, int32_t* state, dpm_jam_result* out,
-                      double* latency_s, double* x, double* y, double* box_side) {
+                      double* latency_s, double* x, double* y, double* box_side) try {
     std::lock_guard<std::mutex> g(sv->mu);
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) {
@@ -1803,5 +1935,22 @@ int dpm_server_result(dpm_server* sv, int64_t id, int32_t* state, dpm_jam_result
     if (box_side) *box_side = q.rside;
     return 0;
 }
+SERVER_GUARD(-1)
+
+int dpm_server_release(dpm_server* sv, int64_t id) try {
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) {
+        set_error("dpm_server_release: unknown request");
+        return -1;
+    }
+    if (it->second.state != 2 && it->second.state != 3) {
+        set_error("dpm_server_release: request still queued or running");
+        return -1;
+    }
+    sv->reqs.erase(it);
+    return 0;
+}
+SERVER_GUARD(-1)
 
 }  // extern "C"

@@ -17,6 +17,7 @@
 // rebuild and finishes the halted step on the device, so the host only polls a
 // pinned copy of the control block per chunk.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -276,14 +277,15 @@ __global__ __launch_bounds__(BLOCK) void k_nve_fused(Dev d, PairParams p, NbrPar
 // Zero + forces + closing kick(dt/2) on P0 (the end of a run, the first half of a
 // split step, or the resumption of a halted step).
 template <bool HOT>
-__global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double hdt, int pos,
-                                                     int hmode, int gate) {
+__device__ __forceinline__ void nve_force_body(const Dev& d, const PairParams& p, double hdt,
+                                               int pos, int hmode, int gate) {
     Ctl* c = d.ctl;
+    bool closes = hmode == H_NVE_FORCE;  // the closing force of a run
     if (gate == G_NORMAL) {
-        if (c->halt) return;
+        if (c->halt || (closes && c->closed)) return;  // a run is closed once
         int need = pending_rebuild(c, c->epoch, pos);
         if (need) {
-            if (threadIdx.x == 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
                 c->halt = 1;
                 c->halt_pos = pos;
                 c->halt_mode = hmode;
@@ -293,7 +295,9 @@ __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double
         }
     } else if (gate == G_RESUME) {
         if (!c->halt || !resumable(d) || c->halt_mode == H_FIRE) return;
+        closes = c->halt_mode == H_NVE_FORCE;
     }
+    if (closes && blockIdx.x == 0 && threadIdx.x == 0) c->closed = 1;
     int tile;
     const int64_t i = particle_of_thread(tile);
     const bool live = i < d.N;
@@ -311,6 +315,27 @@ __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double
     v.x += fx * dtm;
     v.y += fy * dtm;
     d.vel[i] = v;
+}
+
+// The closing force of a run (hmode H_NVE_FORCE, gate G_NORMAL) is a launch unit of its
+// own: the last block to finish publishes the control block.
+template <bool HOT>
+__global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double hdt, int pos,
+                                                     int hmode, int gate) {
+    nve_force_body<HOT>(d, p, hdt, pos, hmode, gate);
+    if (hmode != H_NVE_FORCE || gate != G_NORMAL) return;
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's control-block writes before its count
+        last = atomicAdd(&d.ctl->tail_cnt, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;  // uniform over the block
+    __threadfence();
+    if (threadIdx.x == 0) d.ctl->tail_cnt = 0;
+    __syncthreads();
+    publish_ctl(d);
 }
 
 // Opening kick(dt/2) + drift(dt) in place on P0.  (wr_pos, epoch + wr_eoff) names
@@ -613,21 +638,25 @@ __global__ __launch_bounds__(BLOCK) void k_adam_update(Dev d, NbrParams nb, doub
     disp_epilogue(d, nb, tile, nd, cd, -1, 0);
 }
 
-// gate != nullptr: runs only for a cell rebuild (kind 2)
-__global__ void k_fill_int(int* a, int v, int64_t n, const Ctl* gate) {
-    if (gate && gate->kind != 2) return;
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
 // ------------------------------------------------------------- neighbours
 // Every rebuild kernel takes `gated`: when set it runs only if the chain chose a
 // rebuild that includes it: the cell kernels for kind 2 (cells + list), the list
-// kernels for kinds 1 (list), 2 and 3 (list from the stored build positions).
+// kernels for kinds 1 (list), 2 and 3 (list from the stored build positions).  The cell
+// kernels also bin the grid that Verlet rows are built from; there they run with every
+// list rebuild (gated == 2).
+__device__ __forceinline__ bool cell_gate(const Ctl* c, int gated) {
+    return gated == 0 || (gated == 2 ? c->kind >= 1 : c->kind == 2);
+}
 #define GATE_CELL \
-    if (gated && d.ctl->kind != 2) return;
+    if (!cell_gate(d.ctl, gated)) return;
 #define GATE_LIST \
     if (gated && d.ctl->kind < 1) return;
+
+__global__ void k_fill_int(int* a, int v, int64_t n, const Ctl* gate, int gated) {
+    if (gate && !cell_gate(gate, gated)) return;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) a[i] = v;
+}
 
 __device__ __forceinline__ int cell_coord(double pos, double cs, int n) {
     // getPBCCellIndex, include/particles/base/kernels.cuh:285-301
@@ -888,6 +917,68 @@ __global__ __launch_bounds__(BLOCK) void k_build_verlet(Dev d, NbrParams nb, int
     }
 }
 
+// kernelUpdateNeighborList's rows (base/kernels.cu:256-293: all pairs, ascending j) built
+// from a cell grid whose cells are at least the cutoff wide: every particle within the
+// cutoff of i lies in the 3 x 3 cells around i's, each cell lists its particles in
+// ascending index (stable counting sort of unmoved particles), and the nine lists are
+// merged in ascending order, so each row holds the same indices in the same order as
+// the all-pairs scan.  Particles are binned by k_cell_bin from P0 and are not reordered.
+__global__ __launch_bounds__(BLOCK) void k_build_verlet_grid(Dev d, NbrParams nb, int gated) {
+    GATE_LIST;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (threadIdx.x == 0) d.pmax_nd[blockIdx.x] = 0.0;
+    int cnt = 0;
+    if (i < d.N) {
+        const double4 qi = d.P0[i];
+        const int ncd = nb.ncd;
+        const int cid = d.cell_tmp[i];
+        const int cx = cid % ncd, cy = cid / ncd;
+        int s[9], e[9], h[9];  // cursor, end and head index of each cell's list
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int yy = wrap_cell(cy + oy, ncd);
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int c = wrap_cell(cx + ox, ncd) + yy * ncd;
+                const int t = (oy + 1) * 3 + (ox + 1);
+                s[t] = d.cell_start[c];
+                e[t] = d.cell_start[c + 1];
+                h[t] = s[t] < e[t] ? d.perm[s[t]] : INT_MAX;
+            }
+        }
+        const int cap = d.cap;
+        for (;;) {
+            int best = INT_MAX, bt = -1;
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+                if (h[t] < best) {
+                    best = h[t];
+                    bt = t;
+                }
+            if (bt < 0) break;
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+                if (t == bt) {
+                    s[t]++;
+                    h[t] = s[t] < e[t] ? d.perm[s[t]] : INT_MAX;
+                }
+            if (best == i) continue;
+            const double4 qj = d.P0[best];
+            if (within(qi.x, qi.y, qj.x, qj.y, nb, nb.rc2)) {
+                if (cnt < cap) d.nbr[(int64_t)cnt * d.N + i] = best;
+                cnt++;
+            }
+        }
+        d.nnum[i] = cnt;
+        d.lref[i] = make_double2(qi.x, qi.y);
+    }
+    int m = block_max_int(i < d.N ? cnt : 0);
+    if (threadIdx.x == 0) {
+        atomicMax(&d.ctl->max_nnum, m);
+        atomicMax(&d.ctl->max_nnum_all, m);
+    }
+}
+
 // The per-block displacement maxima of the current positions (P0) against the list and
 // cell reference positions: what disp_epilogue leaves behind, rebuilt after a loop whose
 // steps decide their rebuilds by wave ballots (disp_flags), for the checks and queries
@@ -951,34 +1042,45 @@ __global__ void k_chain_copy_cur(Dev d) {
     d.P0[i] = make_double4(a.x, a.y, b.z, b.w);
 }
 
-__global__ void k_chain_end(Dev d, int mode) {
+// One wave.  mode 0 ends a launch unit (a chunk's chain or a recovery) and publishes the
+// control block; modes 1 and 2 end a stand-alone rebuild.
+__global__ __launch_bounds__(WAVE) void k_chain_end(Dev d, int mode) {
     Ctl* c = d.ctl;
-    bool over = c->max_nnum > d.cap || (d.vnbr != nullptr && c->max_vnum > d.vcap);
-    if (mode != 0) {
-        if (c->kind >= 1 && over) c->overflow = 1;
-        return;
-    }
-    if (c->halt) {
-        if (c->overflow || over) {
-            c->overflow = 1;
+    if (threadIdx.x == 0) {
+        bool over = c->max_nnum > d.cap || (d.vnbr != nullptr && c->max_vnum > d.vcap);
+        if (mode != 0) {
+            if (c->kind >= 1 && over) c->overflow = 1;
         } else {
-            c->halt = 0;
-            c->ovf = 0;  // the whole list was rebuilt with enough room
+            if (c->halt) {
+                if (c->overflow || over) {
+                    c->overflow = 1;
+                } else {
+                    c->halt = 0;
+                    c->ovf = 0;  // the whole list was rebuilt with enough room
+                }
+            }
+            c->last_progress = c->progress;
+            c->progress = 0;
+            c->epoch += 1;
         }
     }
-    c->last_progress = c->progress;
-    c->progress = 0;
-    c->epoch += 1;
+    if (mode != 0) return;
+    __syncthreads();
+    publish_ctl(d);
 }
 
-// End of an NVE chunk: the chunk's completed steps are published and the next epoch
-// starts.  A halted step stays halted; the host finishes it (rebuild chain) when it
+// End of an NVE chunk (one wave): the chunk's completed steps are published and the next
+// epoch starts.  A halted step stays halted; the host finishes it (rebuild chain) when it
 // reads the status.
-__global__ void k_chunk_end(Dev d) {
+__global__ __launch_bounds__(WAVE) void k_chunk_end(Dev d) {
     Ctl* c = d.ctl;
-    c->last_progress = c->progress;
-    c->progress = 0;
-    c->epoch += 1;
+    if (threadIdx.x == 0) {
+        c->last_progress = c->progress;
+        c->progress = 0;
+        c->epoch += 1;
+    }
+    __syncthreads();
+    publish_ctl(d);
 }
 
 // Host fix after a capacity overflow: rebuild only the list, without counting a
@@ -999,6 +1101,10 @@ __global__ void k_ctl_reset(Dev d, int config) {
     c->done = 0;
     c->overflow = 0;
     c->kind = 0;
+    c->closed = 0;
+    c->pub_seq = 0;
+    c->tail_cnt = 0;
+    c->epoch += 1;  // requests stamped before the reset never match a later step
     if (config) {  // a new neighbour configuration restarts the capacity statistics
         c->max_nnum = 0;
         c->max_nnum_all = 0;
@@ -1316,6 +1422,11 @@ NbrParams nbr_params(const dpm_system* s) {
     nb.halfLy = 0.5 * s->box[1];
     nb.vc2 = s->rc_v * s->rc_v;
     nb.vp2 = s->pcut_v * s->pcut_v;
+    if (s->method == DPM_NBR_VERLET && s->vg_ncd >= 3) {  // the grid of the Verlet rows
+        nb.ncd = (int)s->vg_ncd;
+        nb.ncells = s->vg_ncd * s->vg_ncd;
+        nb.cs = s->box[0] / (double)s->vg_ncd;
+    }
     return nb;
 }
 
@@ -1421,13 +1532,36 @@ int ensure_ell(dpm_system* s, int64_t cap) {
 // The rebuild kernels (cell binning, scan, scatter, stable fix-up, reorder, list
 // build) in stream order; with `gated` each one consults the chain's decision.
 // The particle list build of a rebuild (after the cell chain for the cell method).
+// Counting sort of the particles into the cells of `nb` (histogram, scan, scatter, stable
+// fix-up), leaving cell_start and the cell lists in perm; cg is the cell kernels' gate.
+static void enqueue_cell_sort(dpm_system* s, const NbrParams& nb, int cg) {
+    const Dev& d = s->d;
+    hipStream_t st = s->stream;
+    long long nc = nb.ncells;
+    long long nsb = (nc + SCAN_TILE - 1) / SCAN_TILE;
+    const Ctl* g = cg ? d.ctl : nullptr;
+    hipLaunchKernelGGL(k_fill_int, pgrid(nc), dim3(BLOCK), 0, st, d.cell_cnt, 0, (int64_t)nc, g,
+                       cg);
+    hipLaunchKernelGGL(k_cell_bin, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, cg);
+    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, cg);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, st, d, nsb, cg);
+    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, cg);
+    hipLaunchKernelGGL(k_cell_scatter, pgrid(s->N), dim3(BLOCK), 0, st, d, cg);
+    hipLaunchKernelGGL(k_cell_sort, pgrid(nc), dim3(BLOCK), 0, st, d, nc, cg);
+}
+
 void enqueue_list_build(dpm_system* s, int gated) {
     NbrParams nb = nbr_params(s);
-    if (s->method == DPM_NBR_CELL)
+    if (s->method == DPM_NBR_CELL) {
         hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nb, gated);
-    else
+    } else if (s->method == DPM_NBR_VERLET && s->vg_ncd >= 3) {
+        enqueue_cell_sort(s, nb, gated ? 2 : 0);
+        hipLaunchKernelGGL(k_build_verlet_grid, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nb,
+                           gated);
+    } else {
         hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, nb,
                            gated);
+    }
 }
 
 static void enqueue_rebuild(dpm_system* s, int gated) {
@@ -1435,17 +1569,7 @@ static void enqueue_rebuild(dpm_system* s, int gated) {
     NbrParams nb = nbr_params(s);
     hipStream_t st = s->stream;
     if (s->method == DPM_NBR_CELL) {
-        long long nc = s->ncells;
-        long long nsb = (nc + SCAN_TILE - 1) / SCAN_TILE;
-        const Ctl* g = gated ? d.ctl : nullptr;
-        hipLaunchKernelGGL(k_fill_int, pgrid(nc), dim3(BLOCK), 0, st, d.cell_cnt, 0,
-                           (int64_t)nc, g);
-        hipLaunchKernelGGL(k_cell_bin, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, gated);
-        hipLaunchKernelGGL(k_scan_a, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, gated);
-        hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, st, d, nsb, gated);
-        hipLaunchKernelGGL(k_scan_c, dim3((unsigned)nsb), dim3(BLOCK), 0, st, d, nc, gated);
-        hipLaunchKernelGGL(k_cell_scatter, pgrid(s->N), dim3(BLOCK), 0, st, d, gated);
-        hipLaunchKernelGGL(k_cell_sort, pgrid(nc), dim3(BLOCK), 0, st, d, nc, gated);
+        enqueue_cell_sort(s, nb, gated ? 1 : 0);
         hipLaunchKernelGGL(k_reorder_gather, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
         if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_reorder(s, gated);
         hipLaunchKernelGGL(k_reorder_copyback, pgrid(s->N), dim3(BLOCK), 0, st, d, s->sp, gated);
@@ -1480,7 +1604,7 @@ static void enqueue_chain(dpm_system* s, int resume, double dt) {
     const Dev& d = s->d;
     hipStream_t st = s->stream;
     if (s->method != DPM_NBR_CELL && s->method != DPM_NBR_VERLET) {
-        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, 0);
+        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(WAVE), 0, st, d, 0);
         return;
     }
     Timed t(s, DPM_K_REBUILD);
@@ -1488,7 +1612,7 @@ static void enqueue_chain(dpm_system* s, int resume, double dt) {
     if (resume == 1) hipLaunchKernelGGL(k_chain_copy_cur, pgrid(s->N), dim3(BLOCK), 0, st, d);
     enqueue_rebuild(s, 1);
     enqueue_resume(s, resume, dt);
-    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, 0);
+    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(WAVE), 0, st, d, 0);
 }
 
 // Host-side fix after a capacity overflow: regrow the ELL to 2^ceil(log2 max) (the
@@ -1507,14 +1631,11 @@ static int fix_overflow(dpm_system* s, int resume, double dt) {
         NbrParams nb = nbr_params(s);
         hipStream_t st = s->stream;
         hipLaunchKernelGGL(k_overflow_begin, dim3(1), dim3(1), 0, st, d);
-        if (s->method == DPM_NBR_CELL)
-            hipLaunchKernelGGL(k_build_cell, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 1);
-        else
-            hipLaunchKernelGGL(k_build_verlet, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 1);
+        enqueue_list_build(s, 1);
         if (s->kind == DPM_BUMPY) bumpy_enqueue_vertex_build(s, 1);
         enqueue_resume(s, resume, dt);
         // halted loops finish like a chunk end; stand-alone rebuilds only check
-        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, st, d, resume ? 0 : 1);
+        hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(WAVE), 0, st, d, resume ? 0 : 1);
         if (check_launch()) return -1;
     }
     set_error("neighbour capacity did not converge");
@@ -1530,7 +1651,7 @@ int device_rebuild(dpm_system* s, int mode, int* kind_out) {
     hipLaunchKernelGGL(k_chain_decide, dim3(1), dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
                        mode);
     enqueue_rebuild(s, 1);
-    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(1), 0, s->stream, s->d, mode);
+    hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(WAVE), 0, s->stream, s->d, mode);
     if (check_launch()) return -1;
     Ctl c;
     if (scopy(s, &c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
@@ -1592,7 +1713,7 @@ static void nve_chunk(dpm_system* s, double dt, int64_t c) {
         hipLaunchKernelGGL(k_nve_kick_drift, pgrid(s->N), dim3(BLOCK), 0, st, d, nb, 0.5 * dt, dt,
                            0, 1, k + 1, (int)G_NORMAL);
     }
-    hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(1), 0, st, d);
+    hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(WAVE), 0, st, d);
 }
 
 static void fire_chunk(dpm_system* s, double, int64_t c) {
@@ -1626,6 +1747,39 @@ static void bumpy_fire_chunk(dpm_system* s, double, int64_t c) {
 
 static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 1, dt); }
 
+// The closing force + kick of an NVE run (the last step's second half); it records in
+// the control block that it ran (`closed`), or halts when that step needs a rebuild.
+static void nve_close(dpm_system* s, double dt) {
+    Timed t(s, DPM_K_FORCE);
+    PairParams p = params_of(s);
+    LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, 0.5 * dt, 0,
+               (int)H_NVE_FORCE, (int)G_NORMAL);
+}
+typedef void (*TailFn)(dpm_system*, double);
+
+// The chunk lengths a graph-launched loop uses: every length up to 32 (a run's last
+// chunk is its remainder) and powers of two up to the configured chunk.  A longer
+// remainder that is not a power of two is run as power-of-two chunks.
+constexpr int64_t EXACT_LEN_MAX = 32;
+static int64_t graph_len(int64_t c) {
+    if (c <= EXACT_LEN_MAX) return c;
+    int64_t p = 1;
+    while (2 * p <= c) p *= 2;
+    return p;
+}
+
+// Captures every chunk length a run can launch (and the recovery chain) ahead of the
+// run, so that no capture falls inside a run whose length was not seen before.
+static int ensure_all_graphs(dpm_system* s, std::map<int64_t, Graph>& cache, double dt,
+                             ChunkFn chunk_fn) {
+    for (int64_t c = 1; c <= std::min<int64_t>(s->chunk, EXACT_LEN_MAX); ++c)
+        if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
+    for (int64_t c = 2 * EXACT_LEN_MAX; c <= s->chunk; c *= 2)
+        if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
+    if (s->chunk > EXACT_LEN_MAX && ensure_graph(s, cache, s->chunk, dt, chunk_fn)) return -1;
+    return ensure_graph(s, cache, -1, dt, recovery_fill);
+}
+
 // Runs `total` steps in chunks with up to two launch units in flight; each unit ends
 // with a copy of the control block to pinned memory, from which the host counts the
 // completed steps.  FIRE chunks carry the gated rebuild chain themselves
@@ -1638,30 +1792,49 @@ static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 
 // halted at step p the next one is about 2(p + 1) steps long (a halted chunk's later
 // steps are no-ops), and each chunk that runs through doubles it again, up to the
 // configured length.  Chunk boundaries do not change results.
+//
+// `tail` (NVE: the closing force) is a unit of its own, enqueued as soon as every step
+// has been launched, before the host waits for them: the loop ends once the device
+// reports it applied (`closed`).  A tail launched behind a halted step does nothing and
+// is enqueued again after the recovery; a tail that halts itself is finished by the
+// recovery chain, which then closes the run.
 static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, ChunkFn chunk_fn,
-                      std::map<int64_t, Graph>& cache, bool stop_on_done, bool host_recovery) {
+                      std::map<int64_t, Graph>& cache, bool stop_on_done, bool host_recovery,
+                      TailFn tail = nullptr) {
+    enum { U_CHUNK = 0, U_RECOVERY = 1, U_TAIL = 2 };
     struct Unit {
-        int64_t c;       // steps planned (0 for a recovery unit)
-        int slot;
-        bool recovery;
+        int64_t c;       // steps planned (0 for a recovery or tail unit)
+        int seq;         // publication sequence number (mirror slot seq mod 4)
+        int kind;
         bool covered;    // launched before a recovery that finishes its halted step
     };
+    const bool graphs = s->use_graph && s->timing != 1;
+    if (graphs && tail && ensure_all_graphs(s, cache, dt, chunk_fn)) return -1;
     std::deque<Unit> q;
     int64_t done = 0, planned = 0;
-    int next_slot = 0;
+    int next_slot = 0;  // publication sequence of the next unit (k_ctl_reset starts it at 0)
     bool need_recovery = false;
+    bool closed = tail == nullptr, tail_queued = false;
     int64_t cur = s->chunk;
-    while (done < total || need_recovery) {
-        while (q.size() < 2 && (need_recovery || planned < total)) {
-            Unit u{0, 0, false, false};
+    auto recovering = [&q] {
+        for (auto& x : q)
+            if (x.kind == U_RECOVERY) return true;
+        return false;
+    };
+    while (done < total || need_recovery || !closed) {
+        // the tail goes behind the last step, never behind a recovery that may close
+        while (q.size() < 2 && (need_recovery || planned < total ||
+                                (!closed && !tail_queued && !recovering()))) {
+            Unit u{0, 0, U_CHUNK, false};
             hipEvent_t ua = nullptr, ub = nullptr;
-            if (s->timing == 2) {
+            const bool timed = s->timing == 2 && (need_recovery || planned < total);
+            if (timed) {
                 ua = pool_event(s);
                 ub = pool_event(s);
                 DPM_HIP(hipEventRecord(ua, s->stream));
             }
             if (need_recovery) {
-                if (s->use_graph && s->timing != 1) {
+                if (graphs) {
                     if (ensure_graph(s, cache, -1, dt, recovery_fill)) return -1;
                     DPM_HIP(hipGraphLaunch(cache[-1].exec, s->stream));
                 } else {
@@ -1669,12 +1842,13 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
                     if (check_launch()) return -1;
                 }
                 for (auto& x : q) x.covered = true;
-                u.recovery = true;
+                u.kind = U_RECOVERY;
                 planned += 1;
                 need_recovery = false;
-            } else {
+            } else if (planned < total) {
                 int64_t c = std::min<int64_t>(cur, total - planned);
-                if (s->use_graph && s->timing != 1 && c == cur) {
+                if (graphs && (tail || c == cur)) {
+                    c = graph_len(c);
                     if (ensure_graph(s, cache, c, dt, chunk_fn)) return -1;
                     DPM_HIP(hipGraphLaunch(cache[c].exec, s->stream));
                 } else {
@@ -1683,43 +1857,65 @@ static int run_chunks(dpm_system* s, int64_t total, double dt, int resume, Chunk
                 }
                 u.c = c;
                 planned += c;
+            } else {
+                tail(s, dt);
+                if (check_launch()) return -1;
+                u.kind = U_TAIL;
+                tail_queued = true;
             }
-            if (s->timing == 2) {
+            if (timed) {
                 DPM_HIP(hipEventRecord(ub, s->stream));
-                s->ev_pending.push_back({u.recovery ? DPM_K_REBUILD : DPM_K_FORCE, {ua, ub}});
+                s->ev_pending.push_back(
+                    {u.kind == U_RECOVERY ? DPM_K_REBUILD : DPM_K_FORCE, {ua, ub}});
             }
-            u.slot = next_slot;
-            next_slot = (next_slot + 1) % 4;
-            DPM_HIP(hipMemcpyAsync(&s->hctl[u.slot], s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
-                                   s->stream));
-            DPM_HIP(hipEventRecord(s->poll_ev[u.slot], s->stream));
+            // the unit's last kernel publishes the control block into mirror slot seq mod 4
+            u.seq = next_slot++;
+            DPM_HIP(hipEventRecord(s->poll_ev[u.seq & 3], s->stream));
             q.push_back(u);
         }
         if (q.empty()) break;
         Unit f = q.front();
         q.pop_front();
-        DPM_HIP(hipEventSynchronize(s->poll_ev[f.slot]));
+        DPM_HIP(hipEventSynchronize(s->poll_ev[f.seq & 3]));
         if (s->timing == 1) collect_timing(s);
         else if (s->timing == 2) collect_finished(s);
-        Ctl hc = s->hctl[f.slot];
-        done += hc.last_progress;
-        planned -= (f.recovery ? 1 : f.c) - hc.last_progress;
+        Ctl hc;
+        std::memcpy(&hc, (const void*)&s->hctl[f.seq & 3], sizeof(Ctl));
+        if (hc.pub_seq != f.seq) {  // the copy carries the sequence number it was written for
+            set_error("launch loop: status of unit " + std::to_string(f.seq) +
+                      " not published (mirror holds " + std::to_string(hc.pub_seq) + ")");
+            (void)hipStreamSynchronize(s->stream);
+            return -1;
+        }
+        if (f.kind == U_TAIL) {
+            tail_queued = false;
+            closed = hc.closed != 0;
+            // a tail that halted itself: the recovery applies the closing force
+            if (!closed && hc.halt && !f.covered) need_recovery = true;
+        } else {
+            done += hc.last_progress;
+            planned -= (f.kind == U_RECOVERY ? 1 : f.c) - hc.last_progress;
+            if (tail && f.kind == U_RECOVERY) closed = hc.closed != 0;
+        }
         if (hc.overflow) {
             // the halted step is unfinished and every later unit in flight was a no-op
             DPM_HIP(hipStreamSynchronize(s->stream));
             if (fix_overflow(s, resume, dt)) return -1;
             Ctl after;
             if (scopy(s, &after, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
+            next_slot = after.pub_seq;  // the fix published its own unit end
             done += after.last_progress;
             q.clear();
             planned = done;
             need_recovery = false;
+            tail_queued = false;
+            if (tail) closed = after.closed != 0;
             if (stop_on_done && after.done) break;
             continue;
         }
-        if (host_recovery && hc.halt && !f.recovery && !f.covered) need_recovery = true;
+        if (host_recovery && hc.halt && f.kind == U_CHUNK && !f.covered) need_recovery = true;
         if (stop_on_done && hc.done) break;
-        if (!f.recovery && f.c > 0) {
+        if (f.kind == U_CHUNK && f.c > 0) {
             if (hc.last_progress < f.c)
                 cur = std::min<int64_t>(s->chunk, pow2_at_least(2 * (hc.last_progress + 1)));
             else
@@ -1748,21 +1944,8 @@ int nve_run(dpm_system* s, double dt, int64_t n) {
                            0.5 * dt, dt, 0, 0, -1, (int)G_NORMAL);
     }
     if (check_launch()) return -1;
-    if (n > 1 && run_chunks(s, n - 1, dt, 1, nve_chunk, s->nve_graphs, false, true)) return -1;
-    {  // closing force + kick of the last step
-        Timed t(s, DPM_K_FORCE);
-        PairParams p = params_of(s);
-        LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, st, d, p, 0.5 * dt, 0,
-                   (int)H_NVE_FORCE, (int)G_NORMAL);
-    }
-    enqueue_chain(s, 1, dt);
-    if (check_launch()) return -1;
-    Ctl c;
-    if (scopy(s, &c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost)) return -1;
-    if (c.overflow && fix_overflow(s, 1, dt)) return -1;
-    DPM_HIP(hipStreamSynchronize(st));
-    if (s->timing) collect_timing(s);
-    return 0;
+    // n - 1 fused steps, then the closing force + kick of the last step
+    return run_chunks(s, n - 1, dt, 1, nve_chunk, s->nve_graphs, false, true, nve_close);
 }
 
 int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* out) {
@@ -1821,6 +2004,44 @@ int minimize_fire(dpm_system* s, const dpm_fire_params* prm, dpm_fire_result* ou
 
 void reset_ctl(dpm_system* s, int config) {
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s->stream, s->d, config);
+}
+
+// Cell arrays for at least nc cells (the cell method's grid, or the Verlet rows' grid).
+int ensure_cells(dpm_system* s, int64_t nc) {
+    if (nc <= s->ncells_alloc) return 0;
+    DPM_HIP(hipStreamSynchronize(s->stream));  // a queued rebuild may still use the old ones
+    dfree(s, s->d.cell_start);
+    dfree(s, s->d.cell_cnt);
+    dfree(s, s->d.cell_cur);
+    dfree(s, s->d.scan_bsum);
+    s->d.cell_start = s->d.cell_cnt = s->d.cell_cur = s->d.scan_bsum = nullptr;
+    int64_t nsb = (nc + SCAN_TILE - 1) / SCAN_TILE;
+    if (dalloc(s, &s->d.cell_start, nc + 1) || dalloc(s, &s->d.cell_cnt, nc) ||
+        dalloc(s, &s->d.cell_cur, nc) || dalloc(s, &s->d.scan_bsum, nsb + 1))
+        return -1;
+    s->ncells_alloc = nc;
+    s->gen++;
+    return 0;
+}
+
+// The grid the Verlet method builds its rows from (k_build_verlet_grid): cells at least
+// rc (1 + 1e-9) wide, at most about 2 per particle, three or more a side, in a square box;
+// systems under VGRID_MIN_N particles (where one all-pairs pass costs less than the grid's
+// launches) and other boxes use the all-pairs scan.  Either way the rows are identical.
+constexpr int64_t VGRID_MIN_N = 1024;
+int update_verlet_grid(dpm_system* s) {
+    int64_t ncd = 0;
+    if (s->method == DPM_NBR_VERLET && s->N >= VGRID_MIN_N && s->box[0] == s->box[1] &&
+        s->rc > 0) {
+        double c = std::floor(s->box[0] / (s->rc * (1.0 + 1e-9)));
+        c = std::min(c, std::floor(std::sqrt(2.0 * (double)s->N)));
+        if (c >= 3) ncd = (int64_t)c;
+    }
+    if (ncd == s->vg_ncd) return 0;
+    if (ncd && ensure_cells(s, ncd * ncd)) return -1;
+    s->vg_ncd = ncd;
+    s->gen++;
+    return 0;
 }
 
 }  // namespace dpm
@@ -1962,7 +2183,10 @@ dpm_system* dpm_create(int kind, int64_t n, int64_t nv) {
         c.epoch = 1;
         ok = ok && scopy(s, d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice) == 0;
     }
-    ok = ok && hipHostMalloc((void**)&s->hctl, 4 * sizeof(Ctl), hipHostMallocDefault) == hipSuccess;
+    // fine-grained (coherent) pinned memory: kernels write the status into it directly
+    ok = ok && hipHostMalloc((void**)&s->hctl, 4 * sizeof(Ctl),
+                             hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess;
+    ok = ok && hipHostGetDevicePointer((void**)&s->d.hpub, s->hctl, 0) == hipSuccess;
     for (int k = 0; ok && k < 4; ++k)
         ok = hipEventCreateWithFlags(&s->poll_ev[k], hipEventDisableTiming) == hipSuccess;
     if (ok && kind == DPM_BUMPY) ok = bumpy_alloc(s) == 0;
@@ -2006,7 +2230,7 @@ int dpm_set_box(dpm_system* s, double lx, double ly) {
     s->box[1] = ly;
     if (s->method == DPM_NBR_CELL) s->cell_size = lx / s->ncd;  // Particle::updateCellSize
     s->gen++;
-    return 0;
+    return update_verlet_grid(s);
 }
 
 int dpm_get_box(dpm_system* s, double* lx, double* ly) {
@@ -2146,19 +2370,10 @@ int dpm_neighbor_config(dpm_system* s, int method, double cutoff, double thr_sq,
         s->ncells = ncd * ncd;
         s->cell_thr_sq = cell_thr_sq;
         s->cell_size = s->box[0] / ncd;
-        if (s->ncells > s->ncells_alloc) {
-            dfree(s, s->d.cell_start);
-            dfree(s, s->d.cell_cnt);
-            dfree(s, s->d.cell_cur);
-            dfree(s, s->d.scan_bsum);
-            int64_t nsb = (s->ncells + SCAN_TILE - 1) / SCAN_TILE;
-            if (dalloc(s, &s->d.cell_start, s->ncells + 1) ||
-                dalloc(s, &s->d.cell_cnt, s->ncells) || dalloc(s, &s->d.cell_cur, s->ncells) ||
-                dalloc(s, &s->d.scan_bsum, nsb + 1))
-                return -1;
-            s->ncells_alloc = s->ncells;
-        }
+        if (ensure_cells(s, s->ncells)) return -1;
     }
+    s->vg_ncd = 0;
+    if (update_verlet_grid(s)) return -1;
     int64_t cap = 16;
     if (method == DPM_NBR_ALL) {
         // Particle::initAllToAllList (particle.cu:1186-1200)

@@ -27,8 +27,8 @@ void set_error(const std::string& m);
 enum HaltMode { H_NVE_FUSED = 0, H_NVE_FORCE = 1, H_FIRE = 2 };
 
 // Device control block of one system.  Hot kernels only ever write it with plain
-// stores (no atomics, no fences); the host reads it through a pinned copy taken at
-// the end of each chunk.
+// stores (no atomics, no fences); the kernel that ends a launch unit copies it into a
+// pinned host mirror, where the host reads it.
 //
 // Rebuild requests are "stamped": the kernel that moves particles writes, into
 // flag[slot] (list rebuild) and cflag[slot] (cell + list rebuild), the stamp of the
@@ -52,7 +52,10 @@ struct Ctl {
     int max_vnum, max_vnum_all;
     int n_rebuilds, n_cell_rebuilds;
     int bad_pos;              // a non-finite position was binned
-    int pad[2];
+    int closed;               // the closing force of an NVE run has been applied
+    int pub_seq;              // launch units published to the host mirror since the reset
+    int tail_cnt;             // blocks of the closing force that have finished
+    int pad[1];
     double scal[16];          // small reduction results
 };
 
@@ -82,6 +85,7 @@ struct Dev {
     double *pmax_nd, *pmax_cd;  // per-block max displacement partials
     double* part;               // canonical-sum chunk partials
     Ctl* ctl;
+    Ctl* hpub;                  // device address of the pinned host mirror (4 slots)
     FireCtl* fire;
     double2 *m1, *m2;           // Adam moments (allocated on first use)
     double2* ma;                // rigid Adam: angle moments {first, second}
@@ -148,6 +152,8 @@ struct dpm_system {
     int method = DPM_NBR_NONE;
     double rc = -1, thr_sq = -1, cell_thr_sq = -1, cell_size = -1;
     int64_t ncd = -1, ncells = -1;
+    int64_t vg_ncd = 0;         // Verlet method: cells per side of the grid its rows are
+                                // built from (0: the all-pairs scan)
     int64_t phys_cap = 0;
     int64_t vphys_cap = 0;
     int64_t nv_max = 0;         // most vertices of any particle
@@ -171,7 +177,8 @@ struct dpm_system {
     int timing = 0;             // 0 off, 1 per launch, 2 per chunk
     uint64_t gen = 0;           // bumped whenever baked kernel arguments change
     std::map<int64_t, dpm::Graph> nve_graphs, fire_graphs;
-    dpm::Ctl* hctl = nullptr;   // pinned host mirrors of the control block (4 slots)
+    dpm::Ctl* hctl = nullptr;   // pinned host mirror of the control block (4 slots),
+                                // written by the kernel that ends each launch unit
     double ktime_ms[3] = {0, 0, 0};
     int64_t klaunch[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev_pool;

@@ -49,6 +49,22 @@ __device__ __forceinline__ long long stamp(long long epoch, int pos) {
     return epoch * STAMP_STRIDE + pos + 1;
 }
 
+// The end of a launch unit: copies the control block into slot (pub_seq mod 4) of the
+// pinned host mirror and advances pub_seq; the host reads the slot once the event behind
+// the unit has completed.  Called by every thread of one block, after the block's last
+// write to the control block.
+__device__ __forceinline__ void publish_ctl(const Dev& d) {
+    Ctl* c = d.ctl;
+    const int k = c->pub_seq;
+    constexpr int NW = (int)(sizeof(Ctl) / sizeof(int));
+    int* dst = (int*)(d.hpub + (k & 3));
+    const int* src = (const int*)c;
+    for (int w = threadIdx.x; w < NW; w += blockDim.x) dst[w] = src[w];
+    __threadfence_system();
+    __syncthreads();  // every lane has read the sequence number and copied its words
+    if (threadIdx.x == 0) c->pub_seq = k + 1;
+}
+
 __device__ __forceinline__ bool resumable(const Dev& d) {
     const Ctl* c = d.ctl;
     return c->max_nnum <= d.cap && (d.vnbr == nullptr || c->max_vnum <= d.vcap);
@@ -164,6 +180,8 @@ int check_and_rebuild(dpm_system* s, int* kind_out);
 int refresh_rmax(dpm_system* s);
 void enqueue_list_build(dpm_system* s, int gated);
 void reset_ctl(dpm_system* s, int config);
+int ensure_cells(dpm_system* s, int64_t nc);
+int update_verlet_grid(dpm_system* s);
 void collect_timing(dpm_system* s);
 
 // Brackets one launch with HIP events on the system stream when timing is on.

@@ -158,3 +158,46 @@ def make_bumpy_system(sys_cls, st: BumpyState, backend=None, e_c: float = 1.0,
     s.bumpy_neighbor_config(1.5 * 2.0 * st.rv, 1.4)
     s.neighbor_init()
     return s
+
+
+# ---- batched and served jamming workloads -------------------------------------------
+def batched_sizes(n_systems: int, seed: int, n_lo: int = 64, n_hi: int = 2048) -> np.ndarray:
+    """Particles per system, log-uniform in [n_lo, n_hi] from the seed."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), n_systems)).astype(np.int64)
+
+
+def batched_systems(n_systems: int, seed: int, n_lo: int = 64, n_hi: int = 2048,
+                    phi0: float = 0.80) -> list:
+    """Independent bidisperse disk systems of uneven N at packing fraction phi0, each
+    with its own seeded uniform positions."""
+    ns = batched_sizes(n_systems, seed, n_lo, n_hi)
+    return [disk_config(int(n), phi0, seed=seed + 1 + k) for k, n in enumerate(ns)]
+
+
+@dataclass
+class ServeRequest:
+    arrival: float    # seconds after the start
+    bumpy: bool
+    state: object     # DiskState or BumpyState
+    pressure: float   # target pressure
+
+
+def serve_requests(count: int, rate: float, seed: int, bumpy_fraction: float = 0.3,
+                   disk_n=(128, 16384), bumpy_n=(128, 4096), pressure=(1e-7, 1e-3),
+                   phi0: float = 0.80) -> list:
+    """Poisson arrivals at `rate` per second: a fraction `bumpy_fraction` rigid bumpy
+    systems (20/28 vertices, N log-uniform in bumpy_n, at phi0 - 0.1), the rest disks (N
+    log-uniform in disk_n, at phi0); target pressure log-uniform in `pressure`."""
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for k in range(count):
+        t += rng.exponential(1.0 / rate)
+        bumpy = bool(rng.uniform() < bumpy_fraction)
+        pt = float(np.exp(rng.uniform(np.log(pressure[0]), np.log(pressure[1]))))
+        lo, hi = bumpy_n if bumpy else disk_n
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        st = (bumpy_config(n, phi0 - 0.1, seed=seed + 1000 + k) if bumpy
+              else disk_config(n, phi0, seed=seed + 1000 + k))
+        out.append(ServeRequest(t, bumpy, st, pt))
+    return out

@@ -248,6 +248,14 @@ int dpm_batch_config(dpm_batch* b, double e_c, double n_c, double cutoff, double
 int dpm_batch_set_option(dpm_batch* b, int option, int64_t value);
 int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* p, int64_t max_batched_steps,
                   dpm_jam_result* out, int64_t* steps_done);
+/* minimizeFire (src/routines/minimization.cu:75-112) of every system at once, from the state
+ * as set: each system's result equals dpm_minimize_fire on that system alone (Verlet method,
+ * this batch's cutoff and threshold) followed by dpm_reduce of the pressure, contacts,
+ * packing fraction and energy.  out[s]: fire_steps_total = the FIRE steps, status 0
+ * converged / 1 not converged (-1 still running after max_batched_steps),
+ * compression_steps 0, box_length unchanged. */
+int dpm_batch_minimize(dpm_batch* b, const dpm_fire_params* p, int64_t max_batched_steps,
+                       dpm_jam_result* out, int64_t* steps_done);
 /* the batched step (1-based) in which each system finished its last dpm_batch_jam, -1 if
  * it has not */
 int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out);
@@ -261,19 +269,26 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* p, int64_t trial_step
                        double* best_ms_per_step);
 
 /* ---- serving loop (no reference counterpart) ---------------------------------------------
- * Continuous batching of jamming requests.  dpm_server_submit queues a request (arrays are
- * copied); dpm_server_progress admits queued requests while the device budget allows (disk
- * systems join the running batch of the batched engine, whose arena holds arena_slots
- * particle slots and max_systems systems; rigid bumpy systems run on worker threads, each
- * on its own stream), advances the batch by batched_steps and collects finished requests.
+ * Continuous batching of jamming and minimisation requests.  dpm_server_submit queues a
+ * request (arrays are copied); dpm_server_progress admits queued requests while the device
+ * budget allows (disk systems join the running batch of the batched engine, whose arena
+ * holds arena_slots particle slots and max_systems systems; rigid bumpy systems go to a
+ * fixed pool of bumpy_workers threads, each jamming one system at a time on its own
+ * stream), advances the batch by batched_steps and collects finished requests.
  * Disk requests share the server's pair law (e_c = 1, n_c = 2, Verlet cutoff 1.7, rebuild at
- * 3 max d^2 > 3 (0.15)^2); each result equals dpm_jam on the request's system alone.
+ * 3 max d^2 > 3 (0.15)^2).  A DPM_REQ_JAM request's result equals dpm_jam on the request's
+ * system alone; a DPM_REQ_MINIMIZE request runs only minimizeFire with jam.fire and its
+ * result is that of dpm_batch_minimize (status 0 converged, 1 not).
  * dpm_server_result: state 0 queued, 1 running, 2 done (out, latency from submission to
- * completion, final positions and box side are filled), 3 failed. */
+ * completion, final positions and box side are filled), 3 failed.  dpm_server_release
+ * forgets a done or failed request (its result memory is freed).  dpm_server_destroy waits
+ * for the rigid requests the pool is running.  Entry points of one server may be called from
+ * several threads; dpm_server_progress calls must not overlap. */
+enum { DPM_REQ_JAM = 0, DPM_REQ_MINIMIZE = 1 };
 typedef struct dpm_server dpm_server;
 typedef struct {
     int32_t kind;             /* DPM_DISK or DPM_BUMPY */
-    int32_t pad;
+    int32_t mode;             /* DPM_REQ_JAM or DPM_REQ_MINIMIZE */
     int64_t n_particles, n_vertices;
     const double *x, *y, *radius, *mass;
     const double* angle;                  /* rigid bumpy: orientation per particle */
@@ -290,6 +305,7 @@ int dpm_server_progress(dpm_server* s, int64_t batched_steps, int64_t* finished_
                         int64_t* in_flight, int64_t* queued);
 int dpm_server_result(dpm_server* s, int64_t id, int32_t* state, dpm_jam_result* out,
                       double* latency_s, double* x, double* y, double* box_side);
+int dpm_server_release(dpm_server* s, int64_t id);
 
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).

@@ -39,6 +39,11 @@
 #include <string>
 #include <vector>
 
+// Loops over fewer elements than this run on one thread: every parallel loop here is
+// element-wise (or a maximum), so the thread count never changes a result, and small
+// systems (the batched and jamming tests) do not pay a parallel region per kernel.
+constexpr long OMP_MIN_N = 2048;
+
 namespace {
 
 thread_local std::string g_err;
@@ -61,7 +66,7 @@ template <class F>
 double canonical_sum_f(long n, F f) {
     long nc = (n + CH - 1) / CH;
     std::vector<double> P(nc);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (nc >= OMP_MIN_N)
     for (long c = 0; c < nc; ++c) {
         double v[CH];
         for (int t = 0; t < CH; ++t) {
@@ -82,7 +87,7 @@ double canonical_sum_f(long n, F f) {
 template <class F>
 double max_f(long n, F f) {
     double m = 0.0;
-#pragma omp parallel for reduction(max : m)
+#pragma omp parallel for reduction(max : m) if (n >= OMP_MIN_N)
     for (long i = 0; i < n; ++i) m = std::max(m, f(i));
     return m;
 }
@@ -214,7 +219,7 @@ void resize_particles(orc_system* s) {
 void verlet_kernel(orc_system* s) {
     long N = s->N;
     double c2 = s->rc * s->rc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N >= OMP_MIN_N)
     for (long i = 0; i < N; ++i) {
         long added = 0;
         for (long j = 0; j < N; ++j) {
@@ -235,7 +240,7 @@ void verlet_kernel(orc_system* s) {
 void cell_nbr_kernel(orc_system* s) {
     long N = s->N, n = s->ncd;
     double c2 = s->rc * s->rc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N >= OMP_MIN_N)
     for (long i = 0; i < N; ++i) {
         long added = 0;
         long cid = s->cell_index[i];
@@ -268,7 +273,7 @@ void cell_nbr_kernel(orc_system* s) {
 // kernelUpdateVertexNeighborList: src/particles/rigid_bumpy/kernels.cu:870-922
 void vertex_nbr_kernel(orc_system* s) {
     double c2 = s->rc_v * s->rc_v, p2 = s->pcut_v * s->pcut_v;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->Nv >= OMP_MIN_N)
     for (long v = 0; v < s->Nv; ++v) {
         long added = 0;
         double px = s->vxp[v], py = s->vyp[v];
@@ -450,7 +455,7 @@ void zero_forces(orc_system* s) {
 
 // kernelCalcDiskForces: src/particles/disk/kernels.cu:11-57
 void disk_forces(orc_system* s) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         long nn = s->nnum[i];
         if (nn == 0) continue;
@@ -474,7 +479,7 @@ void disk_forces(orc_system* s) {
 // kernelCalcRigidBumpyForces1 + kernelCalcRigidBumpyParticleForces1:
 // src/particles/rigid_bumpy/kernels.cu:366-442
 void bumpy_forces(orc_system* s) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->Nv >= OMP_MIN_N)
     for (long v = 0; v < s->Nv; ++v) {
         long nn = s->vnum[v];
         if (nn == 0) continue;
@@ -498,7 +503,7 @@ void bumpy_forces(orc_system* s) {
         s->vtq[v] += t;
         s->vpe[v] += e;
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long p = 0; p < s->N; ++p) {
         double ax = 0.0, ay = 0.0, at = 0.0, e = 0.0;
         for (long k = 0; k < s->nverts[p]; ++k) {
@@ -524,7 +529,7 @@ void compute_forces(orc_system* s) {
 // rigid_bumpy/kernels.cu:123-150
 void kick(orc_system* s, double dt) {
     if (s->kind == ORC_BUMPY) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
         for (long i = 0; i < s->N; ++i) {
             double w = s->w[i];
             if (s->rotation) w += s->tq[i] * dt / s->moi[i];
@@ -535,7 +540,7 @@ void kick(orc_system* s, double dt) {
         }
         return;
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         double dtm = dt / s->m[i];
         s->vx[i] += s->fx[i] * dtm;
@@ -546,7 +551,7 @@ void kick(orc_system* s, double dt) {
 // kernelTranslateAndRotateVertices1 rigid_bumpy/kernels.cu:152-201: each vertex turns by
 // its particle's last angle change about the old centre and lands on the new one
 void move_vertices(orc_system* s) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->Nv >= OMP_MIN_N)
     for (long v = 0; v < s->Nv; ++v) {
         long p = s->vpart[v];
         double rx = s->vxp[v] - s->lpx[p], ry = s->vyp[v] - s->lpy[p];
@@ -563,7 +568,7 @@ void move_vertices(orc_system* s) {
 // rigid_bumpy/kernels.cu:21-78 + kernelTranslateAndRotateVertices1 :152-201
 void drift(orc_system* s, double dt) {
     if (s->kind == ORC_BUMPY) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
         for (long i = 0; i < s->N; ++i) {
             double ox = s->x[i], oy = s->y[i], a = s->th[i];
             s->lpx[i] = ox;
@@ -584,7 +589,7 @@ void drift(orc_system* s, double dt) {
         move_vertices(s);
         return;
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         double px = s->x[i], py = s->y[i];
         px += s->vx[i] * dt;
@@ -601,7 +606,7 @@ void drift(orc_system* s, double dt) {
 // kernelMixVelocitiesAndForces base/kernels.cu:91-112;
 // kernelMixRigidVelocitiesAndForces rigid_bumpy/kernels.cu:89-121
 void mix(orc_system* s, double alpha) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         double fx = s->fx[i], fy = s->fy[i], vx = s->vx[i], vy = s->vy[i];
         double fn = std::sqrt(fx * fx + fy * fy);
@@ -629,7 +634,7 @@ void zero_velocities(orc_system* s) {
 
 // kinetic energy: base/kernels.cu:163-180; rigid_bumpy/kernels.cu:254-269
 void kinetic(orc_system* s) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         double v2 = s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i];
         if (s->kind == ORC_BUMPY && s->rotation)
@@ -662,7 +667,7 @@ double fmax_sq(orc_system* s) {
 void stress(orc_system* s) {
     double area = s->box[0] * s->box[1];
     if (s->kind == ORC_DISK) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
         for (long i = 0; i < s->N; ++i) {
             double px = s->x[i], py = s->y[i], mm = s->m[i], rr = s->r[i], vx = s->vx[i],
                    vy = s->vy[i];
@@ -687,7 +692,7 @@ void stress(orc_system* s) {
         }
         return;
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long p = 0; p < s->N; ++p) {
         double axx = 0.0, axy = 0.0, ayx = 0.0, ayy = 0.0;
         double px = s->x[p], py = s->y[p], vx = s->vx[p], vy = s->vy[p], w = s->w[p];
@@ -739,7 +744,7 @@ double pressure(orc_system* s) {
 // kernelCountRigidBumpyContacts rigid_bumpy/kernels.cu:924-997
 void count_contacts(orc_system* s) {
     if (s->kind == ORC_DISK) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
         for (long i = 0; i < s->N; ++i) {
             long c = 0;
             for (long k = 0; k < s->nnum[i]; ++k) {
@@ -755,7 +760,7 @@ void count_contacts(orc_system* s) {
         return;
     }
     double vrs = 2 * s->rv;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long p = 0; p < s->N; ++p) {
         long nc = 0, nvc = 0;
         for (long k = 0; k < s->nnum[p]; ++k) {
@@ -962,7 +967,7 @@ void adam_update(orc_system* s, long step, double alpha, double b1, double b2, d
         // RigidBumpy::updatePositionsAdam rigid_bumpy.cu:1473-1486 + kernelRigidBumpyAdamStep
         // rigid_bumpy/kernels.cu:1004-1093; without rotation the angle and its recorded
         // change stay as they are, and the vertex pass still turns by that change
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
         for (long i = 0; i < s->N; ++i) {
             double fx = s->fx[i], fy = s->fy[i], t = s->tq[i];
             double m1x = std::fma(b1, s->m1x[i], (b1 - 1) * fx);
@@ -997,7 +1002,7 @@ void adam_update(orc_system* s, long step, double alpha, double b1, double b2, d
         move_vertices(s);
         return;
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->N >= OMP_MIN_N)
     for (long i = 0; i < s->N; ++i) {
         double fx = s->fx[i], fy = s->fy[i];
         double m1x = s->m1x[i], m1y = s->m1y[i], m2x = s->m2x[i], m2y = s->m2y[i];

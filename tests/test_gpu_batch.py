@@ -1,10 +1,11 @@
 """GPU tests of the batched jamming engine (dpm_batch_*): every system's result is
 bit-identical to the same system jammed alone by dpm_jam (Verlet method, same cutoff and
-threshold), whatever the other systems in the batch are."""
+threshold), whatever the other systems in the batch are, and to the oracle's jam of that
+system (orc_jam, the CPU restatement of scripts/runArthurJamWithFire.cpp:64-106)."""
 import numpy as np
 import pytest
 
-from _helpers import pkg, workloads
+from _helpers import oracle, pkg, workloads
 
 P = pkg()
 W = workloads()
@@ -29,11 +30,16 @@ def systems(spec):
     return [W.disk_config(n, phi, seed=seed) for n, phi, seed in spec]
 
 
-def plain(st, prm):
-    s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
+def plain(st, prm, backend=None):
+    s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2), backend=backend)
     r = s.jam(prm)
     x, y = s.download(P.F_X), s.download(P.F_Y)
     return r, x, y
+
+
+def same_result(got, ref, k):
+    for f in FIELDS:
+        assert getattr(got, f) == getattr(ref, f), (k, f, getattr(got, f), getattr(ref, f))
 
 
 def batched(sts, prm, max_batched_steps=10**8):
@@ -57,12 +63,71 @@ def test_batched_equals_each_system_alone():
     say("batched steps", steps)
     for k, st in enumerate(sts):
         r, x, y = plain(st, prm)
+        ro, ox, oy = plain(st, prm, backend=oracle())
         say("system", k, st.n, r.status, r.compression_steps, r.fire_steps_total)
-        for f in FIELDS:
-            assert getattr(res[k], f) == getattr(r, f), (k, f, getattr(res[k], f), getattr(r, f))
+        same_result(res[k], r, k)
+        same_result(res[k], ro, k)
         bx, by, side = pos[k]
         assert np.array_equal(bx, x) and np.array_equal(by, y), k
-        assert side == r.box_length
+        assert np.array_equal(bx, ox) and np.array_equal(by, oy), k
+        assert side == r.box_length == ro.box_length
+
+
+def test_batched_minimization_equals_minimize_fire_alone():
+    """dpm_batch_minimize: minimizeFire of every system at once; each system's steps,
+    energy, pressure, contacts, packing fraction and positions equal dpm_minimize_fire on
+    that system alone and the oracle's, bit for bit."""
+    sts = systems(SPEC + [(2048, 0.84, 6)])
+    fire = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=1e-9,
+                         max_steps=4000)
+    b = P.Batch([st.n for st in sts])
+    b.config(1.0, 2.0, 1.7, 3.0 * 0.15 * 0.15)
+    for k, st in enumerate(sts):
+        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    res, steps = b.minimize(fire)
+    say("batched minimisation steps", steps)
+    for k, st in enumerate(sts):
+        for be in (None, oracle()):
+            s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2), backend=be)
+            fr = s.minimize_fire(fire)
+            assert res[k].fire_steps_total == fr.steps, (k, be)
+            assert res[k].status == (0 if fr.converged else 1), k
+            assert res[k].pe == s.reduce(P.RED_PE), k
+            assert res[k].pressure == s.reduce(P.RED_PRESSURE), k
+            assert res[k].contacts == s.reduce(P.RED_CONTACTS), k
+            assert res[k].packing_fraction == s.reduce(P.RED_PACKING_FRACTION), k
+            assert res[k].box_length == st.side and res[k].compression_steps == 0
+            bx, by, _ = b.get_system(k)
+            assert np.array_equal(bx, s.download(P.F_X)), k
+            assert np.array_equal(by, s.download(P.F_Y)), k
+    b.close()
+
+
+def test_config4_batched_jamming():
+    """The batched-jamming configuration: 4,096 seeded systems, N log-uniform in
+    [64, 2048], phi0 = 0.80, jammed in one batch with the relaxed protocol the benchmark
+    states (strain 1e-2, FIRE to |F|max < 1e-6 in at most 3,000 steps, target pressure
+    1e-2 within a window of 0.5).  Every system must end jammed (status 0: pressure inside
+    the window with contacts), and the 16 smallest systems equal the oracle's jam bit for
+    bit."""
+    sts = W.batched_systems(4096, 12345)
+    prm = P.jam_params(pressure_target=1e-2, window_rel=0.5, strain_step=1e-2,
+                       max_compression_steps=100000,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-6, max_steps=3000))
+    res, pos, steps = batched(sts, prm)
+    status = np.array([r.status for r in res])
+    say("config 4: batched steps", steps, "statuses", np.unique(status, return_counts=True))
+    assert np.all(status == 0)
+    for r in res:
+        assert 1e-2 <= r.pressure <= 1.5e-2 and r.contacts > 0
+    small = np.argsort([st.n for st in sts], kind="stable")[:16]
+    for k in small:
+        ro, ox, oy = plain(sts[k], prm, backend=oracle())
+        say("config 4 oracle sample", k, sts[k].n, ro.compression_steps, ro.fire_steps_total)
+        same_result(res[k], ro, int(k))
+        bx, by, side = pos[k]
+        assert np.array_equal(bx, ox) and np.array_equal(by, oy), k
 
 
 def test_large_systems_cell_built_rows():

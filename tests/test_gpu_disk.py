@@ -70,7 +70,8 @@ def test_golden_fixture_config1():
 
 # ------------------------------------------------------------------ neighbours + forces
 @pytest.mark.parametrize("n,phi,method", [(1024, 0.80, 3), (4096, 0.85, 3), (600, 0.8, 2),
-                                          (200, 0.9, 1), (3000, 0.3, 3)])
+                                          (200, 0.9, 1), (3000, 0.3, 3), (2500, 0.82, 2),
+                                          (1024, 0.3, 2)])
 def test_pairs_and_forces_bit_exact(n, phi, method):
     say("pairs+forces", n, phi, method)
     st = W.disk_config(n, phi, seed=n + 7)
@@ -171,6 +172,72 @@ def test_nve_chunking_and_graphs_do_not_change_results(steps):
         sg = g.neighbor_stats()
         assert (sg["n_rebuilds"], sg["n_cell_rebuilds"]) == (so["n_rebuilds"],
                                                              so["n_cell_rebuilds"])
+
+
+def test_short_runs_and_closing_halts_match_oracle():
+    """Runs of every short length back to back, at a temperature where the list and the
+    cells are rebuilt every few steps, so that the last step of many runs (the closing
+    force, a unit of its own) needs a rebuild itself; graphs on and off."""
+    st = W.disk_config(1500, 0.8, seed=45)
+    spec = W.neighbor_spec(st.side)
+    vx, vy = W.gaussian_velocities(st.n, 0.5, 5)
+    lengths = [1, 1, 2, 3, 1, 5, 19, 20, 33, 1, 1, 7, 2, 31, 32, 1, 64, 17, 1, 1]
+    o = W.make_disk_system(P.System, st, spec, backend=oracle())
+    o.upload(P.F_VX, vx)
+    o.upload(P.F_VY, vy)
+    forces(o)
+    for n in lengths:
+        o.nve_run(0.01, n)
+    so = o.neighbor_stats()
+    assert so["n_cell_rebuilds"] > 5
+    for graph, chunk in ((1, 32), (0, 32), (1, 8)):
+        say("short runs", graph, chunk)
+        g = W.make_disk_system(P.System, st, spec)
+        g.set_option(P.OPT_GRAPH, graph)
+        g.set_option(P.OPT_CHUNK, chunk)
+        g.upload(P.F_VX, vx)
+        g.upload(P.F_VY, vy)
+        forces(g)
+        for n in lengths:
+            g.nve_run(0.01, n)
+        assert_same_state(g, o)
+        sg = g.neighbor_stats()
+        assert (sg["n_rebuilds"], sg["n_cell_rebuilds"]) == (so["n_rebuilds"],
+                                                             so["n_cell_rebuilds"])
+
+
+def test_verlet_grid_rows_through_nve_and_jam():
+    """The Verlet method builds the rows of systems of 1,024+ particles from a cell grid
+    (the nine cells' lists merged in ascending index): the rows, and so every bit of an
+    NVE run with rebuilds and of a jam whose box changes at every compression step, equal
+    the oracle's all-pairs Verlet rows."""
+    st = W.disk_config(2000, 0.80, seed=46)
+    g, o = pair(st, method=2)
+    vx, vy = W.gaussian_velocities(st.n, 0.05, 6)
+    for s in (g, o):
+        s.upload(P.F_VX, vx)
+        s.upload(P.F_VY, vy)
+        forces(s)
+        s.nve_run(0.01, 200)
+    sg, so = g.neighbor_stats(), o.neighbor_stats()
+    assert sg["n_rebuilds"] == so["n_rebuilds"] > 2
+    assert_same_state(g, o)
+    gi, gj = g.neighbor_pairs()
+    oi, oj = o.neighbor_pairs()
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    prm = P.jam_params(pressure_target=1e-3, window_rel=0.5, strain_step=1e-2,
+                       max_compression_steps=30,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-8, max_steps=300))
+    for s in (g, o):
+        s.upload(P.F_VX, np.zeros(st.n))
+        s.upload(P.F_VY, np.zeros(st.n))
+    rg, ro = g.jam(prm), o.jam(prm)
+    say("verlet grid jam", rg.status, rg.compression_steps, rg.fire_steps_total)
+    for f in ("status", "compression_steps", "fire_steps_total", "pressure",
+              "packing_fraction", "pe", "box_length", "contacts"):
+        assert getattr(rg, f) == getattr(ro, f), f
+    assert_same_state(g, o, fields=("F_X", "F_Y"))
 
 
 def _converging_ring(n_ring=20, speed=1.0, radius=0.28):
