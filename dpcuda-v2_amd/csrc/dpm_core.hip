@@ -317,26 +317,15 @@ __device__ __forceinline__ void nve_force_body(const Dev& d, const PairParams& p
     d.vel[i] = v;
 }
 
-// The closing force of a run (hmode H_NVE_FORCE, gate G_NORMAL) is a launch unit of its
-// own: the last block to finish publishes the control block.
 template <bool HOT>
 __global__ __launch_bounds__(BLOCK) void k_nve_force(Dev d, PairParams p, double hdt, int pos,
                                                      int hmode, int gate) {
     nve_force_body<HOT>(d, p, hdt, pos, hmode, gate);
-    if (hmode != H_NVE_FORCE || gate != G_NORMAL) return;
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();  // this block's control-block writes before its count
-        last = atomicAdd(&d.ctl->tail_cnt, 1) == (int)gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;  // uniform over the block
-    __threadfence();
-    if (threadIdx.x == 0) d.ctl->tail_cnt = 0;
-    __syncthreads();
-    publish_ctl(d);
 }
+
+// End of a launch unit that changes no loop state (the closing force of a run): publishes
+// the control block (one wave).
+__global__ __launch_bounds__(WAVE) void k_unit_end(Dev d) { publish_ctl(d); }
 
 // Opening kick(dt/2) + drift(dt) in place on P0.  (wr_pos, epoch + wr_eoff) names
 // the step that must check the rebuild request; progress >= 0 is recorded (a resumed
@@ -1103,7 +1092,6 @@ __global__ void k_ctl_reset(Dev d, int config) {
     c->kind = 0;
     c->closed = 0;
     c->pub_seq = 0;
-    c->tail_cnt = 0;
     c->epoch += 1;  // requests stamped before the reset never match a later step
     if (config) {  // a new neighbour configuration restarts the capacity statistics
         c->max_nnum = 0;
@@ -1750,10 +1738,13 @@ static void recovery_fill(dpm_system* s, double dt, int64_t) { enqueue_chain(s, 
 // The closing force + kick of an NVE run (the last step's second half); it records in
 // the control block that it ran (`closed`), or halts when that step needs a rebuild.
 static void nve_close(dpm_system* s, double dt) {
-    Timed t(s, DPM_K_FORCE);
-    PairParams p = params_of(s);
-    LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, 0.5 * dt, 0,
-               (int)H_NVE_FORCE, (int)G_NORMAL);
+    {
+        Timed t(s, DPM_K_FORCE);
+        PairParams p = params_of(s);
+        LAUNCH_HOT(k_nve_force, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, 0.5 * dt, 0,
+                   (int)H_NVE_FORCE, (int)G_NORMAL);
+    }
+    hipLaunchKernelGGL(k_unit_end, dim3(1), dim3(WAVE), 0, s->stream, s->d);
 }
 typedef void (*TailFn)(dpm_system*, double);
 

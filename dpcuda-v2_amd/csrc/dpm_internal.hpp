@@ -54,8 +54,6 @@ struct Ctl {
     int bad_pos;              // a non-finite position was binned
     int closed;               // the closing force of an NVE run has been applied
     int pub_seq;              // launch units published to the host mirror since the reset
-    int tail_cnt;             // blocks of the closing force that have finished
-    int pad[1];
     double scal[16];          // small reduction results
 };
 
