@@ -175,6 +175,7 @@ _GPU_ONLY = {
     "batch_finish_steps": (_I, [_P, _P]),
     "batch_minimize": (_I, [_P, C.POINTER(FireParams), _L, _P, C.POINTER(_L)]),
     "server_release": (_I, [_P, _L]),
+    "disk_forces_noalloc": (_I, [_P, _P, _P, _L, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     "server_create": (_P, [_L, _L, _L, C.c_int32]),
     "server_destroy": (None, [_P]),
     "server_submit": (_L, [_P, C.POINTER(Request)]),

@@ -307,6 +307,24 @@ int dpm_server_result(dpm_server* s, int64_t id, int32_t* state, dpm_jam_result*
                       double* latency_s, double* x, double* y, double* box_side);
 int dpm_server_release(dpm_server* s, int64_t id);
 
+/* ---- allocation-free forces (no counterpart in the reference) ------------------------------
+ * For callers that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Disk::calculateForces (disk.cu:218-221)
+ * and totalPotentialEnergy (particle.cu:1078-1080) of n disks held in the caller's device
+ * arrays x, y, radius (n doubles each): fx, fy, pe (n doubles each) receive the forces and
+ * per-particle energies and pe_total (one double, device memory) the total -- the values a
+ * disk system with the Verlet method and this cutoff returns from dpm_neighbor_init +
+ * dpm_zero_forces + dpm_compute_forces + dpm_reduce(DPM_RED_PE), bit for bit.  Kernels are
+ * enqueued on `stream` (a hipStream_t; null for the null stream) and the call returns
+ * without waiting.  The working memory is a 20 MiB array of the library, resident on each
+ * device it runs on until the process ends; the call allocates nothing.  Limits: n <=
+ * 1,048,576; a box under three cutoffs a side takes n <= 4,096.  Calls on one device share
+ * that array: calls that overlap in time on one device (other streams, other threads)
+ * race and must be serialised by the caller. */
+int dpm_disk_forces_noalloc(const double* x, const double* y, const double* radius, int64_t n,
+                            double lx, double ly, double e_c, double n_c, double cutoff,
+                            double* fx, double* fy, double* pe, double* pe_total, void* stream);
+
 /* ---- execution options and measurement (no reference counterpart) -------------------------
  * DPM_OPT_GRAPH: capture the NVE/FIRE chunk in a hipGraph (default 1).
  * DPM_OPT_CHUNK: steps per host poll of the device status (default 32).
