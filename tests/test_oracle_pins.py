@@ -21,6 +21,16 @@ plausible slip (a dropped term, a wrong sign or index, a transposed operand) fai
   cell reorder (particle.cu:1292-)   stable sort by cell id
   rigid geometry (rigid_bumpy)       ring radius, chord, I = m (R - r_v)^2, rotation
                                      round trip, torque balance
+  contacts (disk/kernels.cu:177-201, all-pairs brute force (strict <, exact touch not
+    rigid_bumpy/kernels.cu:924-997)  counted); rigid: brute force over vertex pairs of
+                                     the reference's counting rule
+  rigid stress (rigid_bumpy/kernels. two facing vertices: P = d f / 2A; moving spinning
+    cu:705-789)                      ring: M V V + I w^2 / 2; virial == -dU/dA with the
+                                     vertices scaled affinely
+  rigid pair Hessian (rigid_bumpy/   every H_ij and H_ii entry (theta ones included) by
+    kernels.cu:482-702)              central differences of the ring-pair energy
+  jamming controller (runArthurJam-  hand-derived trace on a square lattice whose
+    WithFire.cpp:64-106)             pressure P(L) is closed form
 """
 import math
 
@@ -538,3 +548,261 @@ def test_shared_sincos_against_libm(lim, max_ulp):
     sp, cp = _sincos(x)
     sn, cn = _sincos(-x)
     assert np.array_equal(sn, -sp) and np.array_equal(cn, cp)
+
+
+# ---------------------------------------------------------------- contacts
+def _min_image(d, L):
+    # pbcDistance (kernels.cuh:65-72): d - L * round(d / L), round half away from zero
+    t = d / L
+    return d - L * np.trunc(t + np.copysign(0.5, t))
+
+
+def test_disk_contacts_brute_force():
+    """countContacts (disk/kernels.cu:177-201): per particle, the listed neighbours with
+    centre distance strictly below r_i + r_j -- checked against an all-pairs count with
+    the minimum image, including a pair placed exactly at contact (not counted)."""
+    st = W.disk_config(400, 0.88, seed=21)
+    x, y = st.x.copy(), st.y.copy()
+    x[1], y[1] = x[0] + (st.rad[0] + st.rad[1]), y[0]  # exactly touching: not a contact
+    s = orc_system(st.n, st.side, x, y, st.rad, method=3)
+    s.count_contacts()
+    got = s.download(P.F_CONTACTS)
+    dx = _min_image(x[:, None] - x[None, :], st.side)
+    dy = _min_image(y[:, None] - y[None, :], st.side)
+    touch = np.sqrt(dx * dx + dy * dy) < st.rad[:, None] + st.rad[None, :]
+    np.fill_diagonal(touch, False)
+    expect = touch.sum(axis=1)
+    assert np.array_equal(got, expect)
+    assert expect.sum() > st.n  # a dense packing: many contacts
+    assert not touch[0, 1] or np.hypot(dx[0, 1], dy[0, 1]) < st.rad[0] + st.rad[1]
+    assert s.reduce(P.RED_CONTACTS) == float(expect.sum())
+
+
+def _bumpy_random(n, phi, seed, e_c=1.0):
+    st = W.bumpy_config(n, phi, seed=seed)
+    s = P.System(1, st.n, st.nv, backend=oracle())
+    s.set_box(st.side, st.side)
+    s.set_energy(e_c, 2.0)
+    s.set_state(st.x, st.y, st.rad, st.mass)
+    s.upload(P.F_ANGLE, st.angle)
+    s.bumpy_setup(st.nverts, st.rv, True)
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * st.rv, 1.4)
+    s.neighbor_init()
+    return s, st
+
+
+def test_bumpy_contacts_brute_force():
+    """countContacts for rigid bumpy particles (rigid_bumpy/kernels.cu:924-997): particle
+    q counts as a contact of p when some vertex of p is closer than 2 r_v to a vertex of
+    q; the vertex-contact count adds, for each such q, the number of q's vertices touching
+    the first vertex of p (in ring order) that touches q.  Brute force over all vertex
+    pairs of the downloaded rings."""
+    s, st = _bumpy_random(40, 0.86, seed=5)
+    s.count_contacts()
+    nc, nvc = s.download(P.F_CONTACTS), s.download(P.F_VERTEX_CONTACTS)
+    vx, vy = s.download(P.F_VERTEX_X), s.download(P.F_VERTEX_Y)
+    start = np.concatenate([[0], np.cumsum(st.nverts)[:-1]])
+    exp_c, exp_v = np.zeros(st.n, int), np.zeros(st.n, int)
+    for p in range(st.n):
+        vp = slice(start[p], start[p] + st.nverts[p])
+        for q in range(st.n):
+            if q == p:
+                continue
+            vq = slice(start[q], start[q] + st.nverts[q])
+            dx = _min_image(vx[vp][:, None] - vx[vq][None, :], st.side)
+            dy = _min_image(vy[vp][:, None] - vy[vq][None, :], st.side)
+            t = np.sqrt(dx * dx + dy * dy) < 2 * st.rv
+            rows = np.flatnonzero(t.any(axis=1))
+            if rows.size:
+                exp_c[p] += 1
+                exp_v[p] += int(t[rows[0]].sum())
+    assert np.array_equal(nc, exp_c) and np.array_equal(nvc, exp_v)
+    assert exp_c.sum() > st.n and np.any(exp_v > exp_c)  # several touching vertices occur
+
+
+# ---------------------------------------------------------------- rigid stress
+def test_bumpy_pressure_two_rings_closed_form():
+    """kernelCalcRigidBumpyStressTensor (rigid_bumpy/kernels.cu:705-789) with one touching
+    vertex pair on the line of centres: sigma summed over particles is d f / A (each side
+    adds d f / 2), so P = d f / (2 A) with f = e_c (1 - d / 2 r_v) / (2 r_v)."""
+    rv = W.vertex_radius()
+    L = 20.0
+    R0, R1 = 0.7 - rv, 0.5 - rv
+    dist = 0.13  # < 2 r_v = 0.1353: only the two facing vertices touch
+    x0, x1 = 5.0, 5.0 + R0 + dist + R1
+    # ring 0 (28 vertices) has a vertex at angle 0; ring 1 (20 vertices) at angle pi
+    s, rad, nverts, rv = bumpy_small(2, x=[x0, x1], y=[7.0, 7.0], angle=[0.0, math.pi],
+                                     side=L)
+    s.set_energy(0.8, 2.0)
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * rv, 1.4)
+    s.neighbor_init()
+    vx = s.download(P.F_VERTEX_X)
+    d_actual = vx[28] - vx[0]  # vertex 0 of each ring sits at its angle: facing each other
+    s.compute_stress()
+    sig = 2 * rv
+    f = 0.8 * (1 - d_actual / sig) / sig
+    assert d_actual == pytest.approx(dist, abs=1e-12)
+    p = s.reduce(P.RED_PRESSURE)
+    assert p == pytest.approx(d_actual * f / (2 * L * L), rel=1e-12)
+    assert s.reduce(P.RED_STRESS_YY) == pytest.approx(0.0, abs=1e-25)
+    s.count_contacts()
+    assert list(s.download(P.F_VERTEX_CONTACTS)) == [1, 1]
+
+
+def test_bumpy_kinetic_stress_of_a_moving_spinning_ring():
+    """The kinetic part of the rigid stress sums m_v v_v v_v over the vertices with
+    v_v = V + omega z x r_v: for a regular ring, sum m_v r_v = 0 and sum m_v r_y^2 =
+    sum m_v r_x^2 = I/2, so sigma_xx A = M Vx^2 + I omega^2 / 2, sigma_xy A = M Vx Vy."""
+    s, rad, nverts, rv = bumpy_small(2, x=[3.0, 12.0], y=[3.0, 12.0], angle=[0.4, 1.3])
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * rv, 1.4)
+    s.neighbor_init()
+    V = np.array([[0.3, -0.2], [-0.1, 0.25]])
+    w = np.array([1.7, -0.6])
+    s.upload(P.F_VX, V[:, 0])
+    s.upload(P.F_VY, V[:, 1])
+    s.upload(P.F_OMEGA, w)
+    s.compute_stress()
+    A = 20.0 * 20.0
+    I = s.download(P.F_MOI)
+    sxx, sxy, syy = (s.download(getattr(P, f)) for f in ("F_STRESS_XX", "F_STRESS_XY",
+                                                          "F_STRESS_YY"))
+    np.testing.assert_allclose(sxx * A, V[:, 0] ** 2 + I * w * w / 2, rtol=1e-12)
+    np.testing.assert_allclose(syy * A, V[:, 1] ** 2 + I * w * w / 2, rtol=1e-12)
+    np.testing.assert_allclose(sxy * A, V[:, 0] * V[:, 1], rtol=1e-11, atol=1e-15)
+
+
+def _vertex_energy(vx, vy, owner, L, sig, e_c):
+    """Sum over vertex pairs of different particles of the harmonic pair energy
+    (e_c / 2) (1 - d / sig)^2 (each pair once)."""
+    dx = _min_image(vx[:, None] - vx[None, :], L)
+    dy = _min_image(vy[:, None] - vy[None, :], L)
+    d = np.sqrt(dx * dx + dy * dy)
+    other = owner[:, None] != owner[None, :]
+    ov = np.where(other & (d < sig), 1 - d / sig, 0.0)
+    return 0.5 * (0.5 * e_c * ov * ov).sum()
+
+
+def test_bumpy_virial_pressure_equals_minus_dU_dA():
+    """The rigid pressure's virial sums d_uv . f_uv over touching vertex pairs, which is
+    -dU/dA when every vertex position and the box scale together (affine in the vertices);
+    U and the derivative come from the downloaded vertex positions alone."""
+    s, st = _bumpy_random(30, 0.86, seed=9, e_c=0.7)
+    s.compute_stress()
+    p = s.reduce(P.RED_PRESSURE)
+    vx, vy = s.download(P.F_VERTEX_X), s.download(P.F_VERTEX_Y)
+    owner = np.repeat(np.arange(st.n), st.nverts)
+    L, sig = st.side, 2 * st.rv
+    s.zero_forces()
+    s.compute_forces()
+    u0 = _vertex_energy(vx, vy, owner, L, sig, 0.7)
+    assert u0 == pytest.approx(s.reduce(P.RED_PE), rel=1e-12) and u0 > 0
+    eps = 1e-6
+    up = _vertex_energy(vx * (1 + eps), vy * (1 + eps), owner, L * (1 + eps), sig, 0.7)
+    um = _vertex_energy(vx * (1 - eps), vy * (1 - eps), owner, L * (1 - eps), sig, 0.7)
+    dA = (L * (1 + eps)) ** 2 - (L * (1 - eps)) ** 2
+    assert p == pytest.approx(-(up - um) / dA, rel=1e-6)
+
+
+# ---------------------------------------------------------------- rigid pair Hessian
+def test_bumpy_pair_hessian_blocks_by_finite_differences():
+    """kernelCalcRigidBumpyForceDistancePairs (rigid_bumpy/kernels.cu:482-702): the 3x3
+    blocks H_ij and H_ii of a pair row are the second derivatives of the pair's full
+    vertex energy with respect to (x_i, y_i, theta_i) x (x_j, y_j, theta_j) and
+    (x_i, y_i, theta_i) x (x_i, y_i, theta_i), where vertex u of ring i sits at
+    X_i + R_i (cos(theta_i + 2 pi u / n_i), sin(...)).  Central differences of that energy
+    (computed here from the ring geometry) pin every entry, theta ones included."""
+    rv = W.vertex_radius()
+    x, y, th = np.array([5.0, 6.13]), np.array([5.0, 5.21]), np.array([0.23, 1.01])
+    s, rad, nverts, rv = bumpy_small(2, x=x, y=y, angle=th, side=20.0)
+    s.set_energy(0.9, 2.0)
+    s.neighbor_config(P.NBR_ALL)
+    s.bumpy_neighbor_config(1.5 * 2 * rv, 1.4)
+    s.neighbor_init()
+    pa = s.pair_analytics()
+    R = rad - rv
+    sig = 2 * rv
+
+    def energy(q):  # q = (x_i, y_i, th_i, x_j, y_j, th_j) of rings 0 and 1
+        pts = []
+        for k in range(2):
+            n = nverts[k]
+            a = q[3 * k + 2] + 2 * math.pi * np.arange(n) / n
+            pts.append((q[3 * k] + R[k] * np.cos(a), q[3 * k + 1] + R[k] * np.sin(a)))
+        dx = pts[0][0][:, None] - pts[1][0][None, :]
+        dy = pts[0][1][:, None] - pts[1][1][None, :]
+        d = np.sqrt(dx * dx + dy * dy)
+        ov = np.where(d < sig, 1 - d / sig, 0.0)
+        return (0.5 * 0.9 * ov * ov).sum(), d
+
+    q0 = np.array([x[0], y[0], th[0], x[1], y[1], th[1]])
+    e0, d0 = energy(q0)
+    assert e0 > 0 and np.sum(d0 < sig) >= 2
+    assert np.min(np.abs(d0 - sig)) > 1e-3  # the touching set is stable under the steps
+    h = 1e-5
+
+    def d2(a, b):
+        def e(sa, sb):
+            q = q0.copy()
+            q[a] += sa * h
+            q[b] += sb * h
+            return energy(q)[0]
+        return (e(1, 1) - e(1, -1) - e(-1, 1) + e(-1, -1)) / (4 * h * h)
+
+    ids_i, ids_j = pa["pair_ids_i"], pa["pair_ids_j"]
+    row, slot = [(r, c) for r in range(2) for c in range(ids_i.shape[1])
+                 if ids_i[r, c] == 0 and ids_j[r, c] == 1][0]
+    names = "xyt"
+    for a in range(3):
+        for b in range(3):
+            hij = pa[f"hessian_pairs_{names[a]}{names[b]}"][row, slot]
+            hii = pa[f"hessian_ii_{names[a]}{names[b]}"][row, slot]
+            assert hij == pytest.approx(d2(a, 3 + b), rel=2e-5, abs=1e-6), ("ij", a, b)
+            assert hii == pytest.approx(d2(a, b), rel=2e-5, abs=1e-6), ("ii", a, b)
+    # the pair potential is half the pair energy (each side stores its half)
+    assert pa["potential_pairs"][row, slot] == pytest.approx(e0 / 2, rel=1e-12)
+
+
+# ---------------------------------------------------------------- jamming controller
+def test_jam_controller_known_answer_on_a_square_lattice():
+    """The pressure bisection of scripts/runArthurJamWithFire.cpp:64-106 on a 4 x 4 square
+    lattice of unit disks (sigma = 1): every force vanishes by symmetry, so each FIRE
+    minimisation leaves the lattice as it is, and with spacing a = L / 4 the pressure is
+    P(a) = (1 - a) / a (four contacts each; zero for a >= 1).  With strain 0.05 from
+    a = 1.1 and the window [0.05, 0.06] the trace is, by hand (L = 4 a):
+      1  L 4.4                P = 0        low  -> L * 0.95
+      2  L 4.18               P = 0        low  -> L * 0.95
+      3  L 3.971              P = 0.0073   low  -> L * 0.95
+      4  L 3.77245            P = 0.0603   high -> restore 3.971, L = mid(3.77245, 3.971)
+      5  L 3.871725           P = 0.0331   low  -> L = mid(3.77245, 3.871725), drop L_high
+      6  L 3.8220875          P = 0.0466   low  -> L * 0.95
+      7  L 3.631083125        P = 0.1016   high -> restore 3.8220875, L = mid
+      8  L 3.7265853125       P = 0.0734   high -> restore 3.8220875, L = mid
+      9  L 3.77433640625      P = 0.0598   inside the window with 64 contacts: done
+    so 8 compression steps, status 0, and the final state's closed-form P, PE, phi."""
+    m, a0 = 4, 1.1
+    n = m * m
+    L0 = m * a0
+    g = (np.arange(m) + 0.5) * a0
+    x, y = np.meshgrid(g, g)
+    s = orc_system(n, L0, x.ravel(), y.ravel(), np.full(n, 0.5), method=3, rc=1.5)
+    prm = P.jam_params(pressure_target=0.05, window_rel=0.2, strain_step=0.05,
+                       max_compression_steps=100, decompress_factor=1.01,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-12, max_steps=100))
+    r = s.jam(prm)
+    L3, L4 = 3.971, 3.971 * 0.95
+    L5 = (L4 + L3) / 2
+    L6 = (L4 + L5) / 2
+    L7 = L6 * 0.95
+    L8 = (L7 + L6) / 2
+    L9 = (L8 + L6) / 2
+    a = L9 / m
+    assert (r.status, r.compression_steps, r.contacts) == (0, 8, 4 * n)
+    assert r.box_length == pytest.approx(L9, rel=1e-12)
+    assert r.pressure == pytest.approx((1 - a) / a, rel=1e-9)
+    assert 0.05 <= r.pressure <= 0.06
+    assert r.pe == pytest.approx(2 * n * 0.5 * (1 - a) ** 2, rel=1e-9)
+    assert r.packing_fraction == pytest.approx(math.pi * 0.25 / (a * a), rel=1e-12)
