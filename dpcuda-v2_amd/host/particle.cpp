@@ -553,7 +553,7 @@ void RigidBumpy::initializeFromConfig(ConfigDict& config, bool minimize) {
     double new_ec = ec * std::pow(getGeometryScale(), 2);
     setEnergyScale(new_ec, "c");
     setExponent(nc, "c");
-    config_["e_c"] = new_ec;
+    this->config["e_c"] = new_ec;
     scaleToPackingFractionFull(phi);
     setupNeighbors(config);
     if (minimize) minimizeFire(*this, 1e-16, 1e-16);

@@ -46,8 +46,10 @@ class Particle {
     long n_cells_dim = -1, n_cells = -1;
     long max_neighbors_allocated = 4;
 
-    void setConfig(ConfigDict& config) { config_ = config; }
-    ConfigDict getConfig() { return config_; }
+    // the particle's configuration (particle.h:35; drivers read it, e.g. config["particle_type"])
+    ConfigDict config;
+    void setConfig(ConfigDict& c) { config = c; }
+    ConfigDict getConfig() { return config; }
     virtual void initializeFromConfig(ConfigDict& config, bool minimize = false) = 0;
     virtual std::vector<std::string> getFundamentalValues();
 
@@ -190,7 +192,6 @@ class Particle {
 
     dpm_system* sys_ = nullptr;
     int kind_ = DPM_DISK;
-    ConfigDict config_;
     std::vector<double> box_ = {1.0, 1.0};
 };
 

@@ -1,6 +1,8 @@
 // routines.cpp -- minimisation and jamming routines over the C ABI (see routines.hpp).
 #include "routines.hpp"
 
+#include "io.hpp"
+
 #include <cmath>
 #include <iostream>
 #include <stdexcept>
@@ -92,9 +94,12 @@ void minimizeAdam(Particle& particle, double avg_pe_target, double avg_pe_diff_t
 }
 
 // jam_adam (compression.cu:3-59)
-void jam_adam(Particle& particle, Adam& adam, long num_compression_steps, long num_adam_steps,
-              double avg_pe_target, double avg_pe_diff_target, double packing_fraction_increment,
-              double min_packing_fraction_increment, double mult, long start_step) {
+// The compression routines log through `io` (when given) where the reference logs: each
+// compression step before the box changes, and once more, forced, at the end.
+static void jam_adam_impl(Particle& particle, Adam& adam, IOManager* io, long num_compression_steps,
+                          long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
+                          double packing_fraction_increment,
+                          double min_packing_fraction_increment, double mult, long start_step) {
     check(dpm_adam_reset(particle.handle()));
     double packing_fraction = particle.getPackingFraction();
     long compression_step = start_step;
@@ -120,16 +125,19 @@ void jam_adam(Particle& particle, Adam& adam, long num_compression_steps, long n
                 consecutive_compressions = 0;
             }
         }
+        if (io) io->log(compression_step);
         particle.scaleToPackingFractionFull(packing_fraction + packing_fraction_increment * sign);
         packing_fraction = particle.getPackingFraction();
         compression_step++;
     }
+    if (io) io->log(compression_step, true);
 }
 
 // compress_adam (compression.cu:61-93)
-void compress_adam(Particle& particle, Adam& adam, long num_compression_steps, long num_adam_steps,
-                   double avg_pe_target, double avg_pe_diff_target,
-                   double packing_fraction_increment, double max_pe_target) {
+static void compress_adam_impl(Particle& particle, Adam& adam, IOManager* io,
+                               long num_compression_steps, long num_adam_steps,
+                               double avg_pe_target, double avg_pe_diff_target,
+                               double packing_fraction_increment, double max_pe_target) {
     check(dpm_adam_reset(particle.handle()));
     double packing_fraction = particle.getPackingFraction();
     long compression_step = 0;
@@ -141,33 +149,39 @@ void compress_adam(Particle& particle, Adam& adam, long num_compression_steps, l
             std::cout << "Adam failed to converge" << std::endl;
             break;
         }
+        if (io) io->log(compression_step);
         particle.scaleToPackingFractionFull(packing_fraction + packing_fraction_increment);
         packing_fraction = particle.getPackingFraction();
         compression_step++;
     }
+    if (io) io->log(compression_step, true);
 }
 
 // decompress_adam (compression.cu:95-126)
-void decompress_adam(Particle& particle, Adam& adam, long num_compression_steps,
-                     long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
-                     double packing_fraction_increment, double min_pe_target) {
+static void decompress_adam_impl(Particle& particle, Adam& adam, IOManager* io,
+                                 long num_compression_steps, long num_adam_steps,
+                                 double avg_pe_target, double avg_pe_diff_target,
+                                 double packing_fraction_increment, double min_pe_target) {
     check(dpm_adam_reset(particle.handle()));
     double packing_fraction = particle.getPackingFraction();
     long compression_step = 0;
     while (compression_step < num_compression_steps) {
         double avg_pe =
             adam_relax(particle, adam, num_adam_steps, avg_pe_target, avg_pe_diff_target, nullptr);
+        if (io) io->log(compression_step);
         particle.scaleToPackingFractionFull(packing_fraction - packing_fraction_increment);
         packing_fraction = particle.getPackingFraction();
         compression_step++;
         if (avg_pe < min_pe_target) break;
     }
+    if (io) io->log(compression_step, true);
 }
 
 // compress_to_phi_adam (compression.cu:128-159)
-void compress_to_phi_adam(Particle& particle, Adam& adam, long num_compression_steps,
-                          long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
-                          double delta_phi_target, long start_step) {
+static void compress_to_phi_adam_impl(Particle& particle, Adam& adam, IOManager* io,
+                                      long num_compression_steps, long num_adam_steps,
+                                      double avg_pe_target, double avg_pe_diff_target,
+                                      double delta_phi_target, long start_step) {
     check(dpm_adam_reset(particle.handle()));
     double packing_fraction = particle.getPackingFraction();
     double phi_target = packing_fraction + delta_phi_target;
@@ -177,10 +191,73 @@ void compress_to_phi_adam(Particle& particle, Adam& adam, long num_compression_s
     while (compression_step < num_compression_steps && std::abs(packing_fraction - phi_target) > inc) {
         adam_relax(particle, adam, num_adam_steps, avg_pe_target, avg_pe_diff_target, nullptr);
         double next = packing_fraction + (phi_target > packing_fraction ? inc : -inc);
+        if (io) io->log(compression_step);
         particle.scaleToPackingFractionFull(next);
         packing_fraction = particle.getPackingFraction();
         compression_step++;
     }
+    if (io) io->log(compression_step, true);
+}
+
+// ---- the reference's signatures (include/routines/compression.h:8-14), logging through
+// io_manager, and the same routines without logging
+void jam_adam(Particle& particle, Adam& adam, IOManager& io_manager, long num_compression_steps,
+              long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
+              double packing_fraction_increment, double min_packing_fraction_increment,
+              double mult, long start_step) {
+    jam_adam_impl(particle, adam, &io_manager, num_compression_steps, num_adam_steps,
+                  avg_pe_target, avg_pe_diff_target, packing_fraction_increment,
+                  min_packing_fraction_increment, mult, start_step);
+}
+void jam_adam(Particle& particle, Adam& adam, long num_compression_steps, long num_adam_steps,
+              double avg_pe_target, double avg_pe_diff_target, double packing_fraction_increment,
+              double min_packing_fraction_increment, double mult, long start_step) {
+    jam_adam_impl(particle, adam, nullptr, num_compression_steps, num_adam_steps, avg_pe_target,
+                  avg_pe_diff_target, packing_fraction_increment, min_packing_fraction_increment,
+                  mult, start_step);
+}
+void compress_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                   long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                   double avg_pe_diff_target, double packing_fraction_increment,
+                   double max_pe_target) {
+    compress_adam_impl(particle, adam, &io_manager, num_compression_steps, num_adam_steps,
+                       avg_pe_target, avg_pe_diff_target, packing_fraction_increment,
+                       max_pe_target);
+}
+void compress_adam(Particle& particle, Adam& adam, long num_compression_steps, long num_adam_steps,
+                   double avg_pe_target, double avg_pe_diff_target,
+                   double packing_fraction_increment, double max_pe_target) {
+    compress_adam_impl(particle, adam, nullptr, num_compression_steps, num_adam_steps,
+                       avg_pe_target, avg_pe_diff_target, packing_fraction_increment,
+                       max_pe_target);
+}
+void decompress_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                     long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                     double avg_pe_diff_target, double packing_fraction_increment,
+                     double min_pe_target) {
+    decompress_adam_impl(particle, adam, &io_manager, num_compression_steps, num_adam_steps,
+                         avg_pe_target, avg_pe_diff_target, packing_fraction_increment,
+                         min_pe_target);
+}
+void decompress_adam(Particle& particle, Adam& adam, long num_compression_steps,
+                     long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
+                     double packing_fraction_increment, double min_pe_target) {
+    decompress_adam_impl(particle, adam, nullptr, num_compression_steps, num_adam_steps,
+                         avg_pe_target, avg_pe_diff_target, packing_fraction_increment,
+                         min_pe_target);
+}
+void compress_to_phi_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                          long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                          double avg_pe_diff_target, double delta_phi_target, long start_step) {
+    compress_to_phi_adam_impl(particle, adam, &io_manager, num_compression_steps,
+                              num_adam_steps, avg_pe_target, avg_pe_diff_target,
+                              delta_phi_target, start_step);
+}
+void compress_to_phi_adam(Particle& particle, Adam& adam, long num_compression_steps,
+                          long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
+                          double delta_phi_target, long start_step) {
+    compress_to_phi_adam_impl(particle, adam, nullptr, num_compression_steps, num_adam_steps,
+                              avg_pe_target, avg_pe_diff_target, delta_phi_target, start_step);
 }
 
 JamResult jamWithFire(Particle& particle, double pressure_low, double pressure_high,

@@ -1,11 +1,13 @@
 // routines.hpp -- minimisation and jamming routines (include/routines/minimization.h,
 // include/routines/compression.h, scripts/runArthurJamWithFire.cpp) over the C ABI.
 //
-// The reference's routines take an IOManager for logging; logging is outside this
-// library, so the compression routines here take no IOManager argument.
+// The compression routines come with the reference's signatures (an IOManager to log
+// through) and without the IOManager (no logging).
 #pragma once
 #include "integrator.hpp"
 #include "particle.hpp"
+
+class IOManager;
 
 // minimizeFire overloads (src/routines/minimization.cu:75-148).  The loop runs on the
 // device (dpm_minimize_fire): Fire::step and the PE/N stop test per step, velocities
@@ -32,7 +34,25 @@ void minimizeAdam(Particle& particle);
 void minimizeAdam(Particle& particle, long log_every_n);
 void minimizeAdam(Particle& particle, double avg_pe_target, double avg_pe_diff_target);
 
-// Adam compression routines (src/routines/compression.cu:3-159)
+// Adam compression routines (src/routines/compression.cu:3-159), as declared by
+// include/routines/compression.h:8-14
+void jam_adam(Particle& particle, Adam& adam, IOManager& io_manager, long num_compression_steps,
+              long num_adam_steps, double avg_pe_target, double avg_pe_diff_target,
+              double packing_fraction_increment, double min_packing_fraction_increment,
+              double mult, long start_step = 0);
+void compress_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                   long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                   double avg_pe_diff_target, double packing_fraction_increment,
+                   double max_pe_target);
+void decompress_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                     long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                     double avg_pe_diff_target, double packing_fraction_increment,
+                     double min_pe_target);
+void compress_to_phi_adam(Particle& particle, Adam& adam, IOManager& io_manager,
+                          long num_compression_steps, long num_adam_steps, double avg_pe_target,
+                          double avg_pe_diff_target, double delta_phi_target,
+                          long start_step = 0);
+// the same without logging
 void jam_adam(Particle& particle, Adam& adam, long num_compression_steps, long num_adam_steps,
               double avg_pe_target, double avg_pe_diff_target, double packing_fraction_increment,
               double min_packing_fraction_increment, double mult, long start_step = 0);

@@ -376,7 +376,7 @@ long Disk::load(std::filesystem::path root_path, std::string source, long frame)
     need("positions");
     tryLoadData(frame_path, restart_path, init_path, source, "velocities");
     need("box_size");
-    config["packing_fraction"] = getPackingFraction();
+    this->config["packing_fraction"] = getPackingFraction();
     setupNeighbors(config);
     return frame_number;
 }
@@ -512,7 +512,7 @@ long RigidBumpy::load(std::filesystem::path root_path, std::string source, long 
     applyStaged();
     setEnergyScale(e, "c");
     setExponent(nc, "c");
-    config_["packing_fraction"] = getPackingFraction();
+    this->config["packing_fraction"] = getPackingFraction();
     setupNeighbors(config);
     return frame_number;
 }

@@ -145,7 +145,7 @@ def test_bumpy_geometry_values():
 
 
 def test_drivers_are_built():
-    for d in ("run_nve", "run_fire", "run_jam", "run_jam_with_fire"):
+    for d in ("run_nve", "create"):
         assert os.access(os.path.join(DRIVERS, d), os.X_OK), d
 
 
@@ -177,10 +177,6 @@ def test_host_api_over_hip_matches_oracle(scenario):
 @pytest.mark.parametrize("argv", [
     ["run_nve", "n_particles=1024", "num_steps=300", "temperature=1e-4", "loop=device"],
     ["run_nve", "n_particles=512", "num_steps=100", "temperature=1e-4", "loop=host"],
-    ["run_fire", "n_particles=512", "fire_num_steps=3000"],
-    ["run_jam", "n_particles=64", "num_compression_steps=60", "compression_increment=2e-3"],
-    ["run_jam_with_fire", "n_particles=64", "compression_rate=0.99",
-     "pressure_tolerance_low=1e-4", "pressure_tolerance_high=1.5e-4"],
 ])
 def test_drivers_run(argv):
     print("[driver]", " ".join(argv), flush=True)
@@ -236,9 +232,10 @@ def _drive(name, run_dir, run_config, timeout=300):
 
 @pytest.mark.gpu
 def test_reference_form_drivers(tmp_path):
-    """create -> run_nve (device loop between logs and the per-step host loop give the same
-    files) -> run_fire -> run_jam -> run_jam_with_fire, each started as
-    `driver <run directory>` and writing the reference's directory layout."""
+    """create -> run_nve over the HIP library, each started as `driver <run directory>` and
+    writing the reference's directory layout; the NVE steps between two logs run as one
+    device loop or one host call per step, with the same files.  (The reference's own
+    drivers, unchanged, run over the host library in tests/test_reference_drivers.py.)"""
     base = str(tmp_path)
     src = os.path.join(base, "src")
     _prepare(src, None, _disk_particle_config(256, 0.8, 7))
@@ -260,31 +257,3 @@ def test_reference_form_drivers(tmp_path):
     rows = open(os.path.join(outs["device"], "system", "energy.csv")).read().split("\n")[1:-1]
     assert [float(r.split(",")[0]) for r in rows] == list(range(0, 600, 50)) + [600]
     assert open(os.path.join(outs["device"], "system", "restart", "step.dat")).read() == "600"
-    fire_out = os.path.join(base, "fire")
-    _prepare(fire_out, None)
-    _drive("run_fire", os.path.join(base, "run_fire"),
-           {"input_dir": src, "output_dir": fire_out, "minimize": False, "fire_alpha_init": 0.1,
-            "fire_dt": 0.01, "fire_num_steps": 2000, "fire_log_every_n": 1000})
-    assert os.path.exists(os.path.join(fire_out, "system", "restart", "force_pairs.dat"))
-    jam_out = os.path.join(base, "jam")
-    _prepare(jam_out, None)
-    _drive("run_jam", os.path.join(base, "run_jam"),
-           {"input_dir": src, "output_dir": jam_out, "minimize": False,
-            "num_compression_steps": 40, "compression_increment": 2e-3,
-            "avg_pe_tolerance": 1e-16, "avg_pe_diff_tolerance": 1e-16, "phi_tolerance": 1e-6,
-            "delta_phi_stability": 1e-4}, timeout=600)
-    assert os.path.exists(os.path.join(jam_out, "system", "restart", "hessian_ii_yy.dat"))
-    small = os.path.join(base, "small")
-    _prepare(small, None, _disk_particle_config(64, 0.8, 3))
-    _drive("create", small, {"input_dir": "", "output_dir": small, "minimize": False})
-    jf_out = os.path.join(base, "jam_fire")
-    _prepare(jf_out, None)
-    stdout = _drive("run_jam_with_fire", os.path.join(base, "run_jam_fire"),
-                    {"input_dir": small, "output_dir": jf_out, "minimize": False,
-                     "num_compression_steps": 300, "compression_rate": 0.99,
-                     "pressure_tolerance_low": 1e-4, "pressure_tolerance_high": 1.5e-4,
-                     "length_tolerance": 1e-12}, timeout=600)
-    assert "Time taken:" in stdout
-    assert os.path.exists(os.path.join(jf_out, "system", "last_state", "positions.dat"))
-    rows = open(os.path.join(jf_out, "system", "energy.csv")).read().split("\n")[1:-1]
-    assert len(rows) >= 2
