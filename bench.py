@@ -405,20 +405,78 @@ def pairs(args, P, W):
 
 
 # ---------------------------------------------------------------- batched jamming
+def _jam_prm(P, args, pressure=None):
+    return P.jam_params(pressure_target=args.jam_pressure if pressure is None else pressure,
+                        window_rel=args.jam_window, strain_step=args.jam_strain,
+                        max_compression_steps=args.jam_csteps,
+                        fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                           fmax_target=args.jam_fmax,
+                                           max_steps=args.jam_fire_steps))
+
+
+def batch_force_roofline(P, sts, steps):
+    """The batched force pass (k_b_force) timed with HIP events over `steps` batched
+    steps of a batch holding `sts` from their initial state (every system is inside a
+    minimisation then, so every particle slot is worked).  Algorithmic bytes per live slot:
+    ownership 4 + position record 32 + row length 4 + 4 k (row) + velocity 16 read, force
+    16 + energy 8 + velocity 16 written = 96 + 4 k (neighbour records not counted again)."""
+    import numpy as np
+    b = P.Batch([st.n for st in sts])
+    b.config(1.0, 2.0, 1.7, 3.0 * 0.15 * 0.15)
+    for k, st in enumerate(sts):
+        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    b.set_option(P.OPT_TIMING, 1)
+    prm = P.jam_params(pressure_target=1e-2, window_rel=0.5, strain_step=1e-2,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-12, max_steps=10 * steps))
+    b.jam(prm, steps)
+    ms, launches = b.kernel_time()
+    entries = b.row_entries()
+    b.close()
+    n = int(np.sum([st.n for st in sts]))
+    k = entries / n
+    per = ms / max(launches, 1) * 1e-3
+    byts = n * (96.0 + 4.0 * k)
+    gbs = byts / per / 1e9
+    return {"bound": "hbm", "kernel": "k_b_force (batched force pass)", "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": byts,
+            "kernel_us_per_launch": per * 1e6, "timed_launches": launches,
+            "row_mean": k, "particles": n}
+
+
+def oracle_jam_sample(P, W, jobs, budget_s):
+    """The CPU oracle jamming (system, parameters) jobs of the same workload one after
+    another until the time budget is spent (the plain reference algorithm on the host)."""
+    oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(oracle_so):
+        return None
+    be = P.Backend(oracle_so, "orc_", gpu=False)
+    done, t = 0, 0.0
+    for st, prm in jobs:
+        s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2), backend=be)
+        c0 = time.perf_counter()
+        s.jam(prm)
+        t += time.perf_counter() - c0
+        done += 1
+        if t > budget_s:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": done / t, "unit": "packings/s", "cores": threads, "kind": "port",
+            "sample": f"{done} of the systems jammed one after another by the C++ oracle "
+                      f"(OpenMP; systems under 2048 particles run on one thread), {t:.1f} s"}
+
+
 def jam_batch(args, P, W):
     """Jamming of many independent disk systems of uneven N: the batched engine against
-    the plain one-system-at-a-time GPU path (dpm_jam, same Verlet lists, same result bits),
-    both on the same systems.  value = jammed packings per second of the batched engine."""
+    the plain one-system-at-a-time GPU path (dpm_jam, Verlet rows: all-pairs under 1,024
+    particles, from a cell grid above; the same result bits), both on the same systems.
+    value = systems jammed (status 0) per second by the batched engine."""
     import numpy as np
-    rng = np.random.default_rng(args.seed)
-    ns = np.exp(rng.uniform(np.log(args.batch_n_lo), np.log(args.batch_n_hi),
-                            args.systems)).astype(np.int64)
-    sts = [W.disk_config(int(n), args.jam_phi0, seed=args.seed + 1 + k) for k, n in enumerate(ns)]
-    prm = P.jam_params(pressure_target=args.jam_pressure, window_rel=args.jam_window,
-                       strain_step=args.jam_strain, max_compression_steps=args.jam_csteps,
-                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
-                                          fmax_target=args.jam_fmax,
-                                          max_steps=args.jam_fire_steps))
+    sts = W.batched_systems(args.systems, args.seed, args.batch_n_lo, args.batch_n_hi,
+                            args.jam_phi0)
+    ns = np.array([st.n for st in sts])
+    prm = _jam_prm(P, args)
     rc, thr_sq = 1.7, 3.0 * 0.15 * 0.15
     print(f"[bench] batched jamming: {len(sts)} systems, {int(ns.sum())} particles",
           file=sys.stderr, flush=True)
@@ -438,12 +496,15 @@ def jam_batch(args, P, W):
     res, steps = b.jam(prm, args.batch_max_steps)
     elapsed = time.perf_counter() - t0
     fin = b.finish_steps()
-    done = [k for k in range(len(sts)) if res[k].status >= 0]
+    b.close()
+    status = np.array([r.status for r in res])
+    jammed = int(np.sum(status == 0))
     step_s = elapsed / max(steps, 1)
     lat = np.sort(fin[fin > 0] * step_s)
     p50 = float(np.percentile(lat, 50)) if len(lat) else None
     p99 = float(np.percentile(lat, 99)) if len(lat) else None
     # plain path on a sample of the same systems, one at a time
+    rng = np.random.default_rng(args.seed + 1)
     k_sample = min(args.plain_sample, len(sts))
     pick = rng.choice(len(sts), size=k_sample, replace=False)
     print(f"[bench] plain path on {k_sample} of the systems", file=sys.stderr, flush=True)
@@ -457,11 +518,19 @@ def jam_batch(args, P, W):
         r = s.jam(prm)
         s.synchronize()
         t_plain.append(time.perf_counter() - c0)
-        for f in ("status", "fire_steps_total", "pressure", "pe", "packing_fraction"):
+        for f in ("status", "compression_steps", "fire_steps_total", "pressure", "pe",
+                  "packing_fraction", "contacts", "box_length"):
             same = same and getattr(r, f) == getattr(res[int(k)], f)
         s.close()
     plain_jams_s = 1.0 / (sum(t_plain) / len(t_plain))
-    value = len(done) / elapsed
+    value = jammed / elapsed
+    print("[bench] force-pass roofline (event-timed batched steps)", file=sys.stderr,
+          flush=True)
+    roof = batch_force_roofline(P, sts, args.roofline_steps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = oracle_jam_sample(P, W, [(sts[int(k)], prm) for k in pick], args.cpu_budget)
     out = {
         "metric": "batched jammed packings/s vs the plain one-system-at-a-time GPU path",
         "value": value,
@@ -478,13 +547,16 @@ def jam_batch(args, P, W):
                 f"{args.batch_n_hi}])",
         "config": {
             "workload": f"{len(sts)} bidisperse disk systems (1.0/1.4, 50/50), N log-uniform "
-                        f"in [{args.batch_n_lo}, {args.batch_n_hi}], phi0={args.jam_phi0}, compress by strain "
-                        f"{args.jam_strain} with FIRE to |F|max<{args.jam_fmax} (at most "
-                        f"{args.jam_fire_steps} steps each), pressure bisection to "
-                        f"{args.jam_pressure} (window {args.jam_window}), Verlet rc=1.7",
+                        f"in [{args.batch_n_lo}, {args.batch_n_hi}], phi0={args.jam_phi0}, "
+                        f"compress by strain {args.jam_strain} with FIRE to |F|max<"
+                        f"{args.jam_fmax} (at most {args.jam_fire_steps} steps each), pressure "
+                        f"bisection to {args.jam_pressure} (window {args.jam_window}), Verlet "
+                        f"rc=1.7",
             "systems": len(sts),
             "particles": int(ns.sum()),
-            "jammed": len(done),
+            "jammed_status0": jammed,
+            "status_counts": {str(int(k)): int(v) for k, v in
+                              zip(*np.unique(status, return_counts=True))},
             "elapsed_s": elapsed,
             "latency_p50_s": p50,
             "latency_p99_s": p99,
@@ -495,7 +567,8 @@ def jam_batch(args, P, W):
             "sample_results_bit_identical": same,
             "autotune": tuned,
         },
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -504,34 +577,19 @@ def jam_batch(args, P, W):
 def serve(args, P, W):
     """Poisson arrivals of jamming requests served by continuous batching (dpm_server)
     against the plain sequential GPU path (one dpm_jam at a time, first come first served)
-    on the same arrival sequence.  value = packings/s of the server over the run."""
+    on the same arrival sequence.  value = requests jammed (status 0) per second of the
+    server over the run."""
     import numpy as np
-    rng = np.random.default_rng(args.seed)
-    reqs = []
-    t = 0.0
-    for k in range(args.requests):
-        t += rng.exponential(1.0 / args.rate)
-        bumpy = rng.uniform() < args.bumpy_fraction
-        pt = float(np.exp(rng.uniform(np.log(args.p_lo), np.log(args.p_hi))))
-        if bumpy:
-            n = int(np.exp(rng.uniform(np.log(args.bumpy_n_lo), np.log(args.bumpy_n_hi))))
-            st = W.bumpy_config(n, args.jam_phi0 - 0.1, seed=args.seed + 1000 + k)
-        else:
-            n = int(np.exp(rng.uniform(np.log(args.disk_n_lo), np.log(args.disk_n_hi))))
-            st = W.disk_config(n, args.jam_phi0, seed=args.seed + 1000 + k)
-        prm = P.jam_params(pressure_target=pt, window_rel=args.jam_window,
-                           strain_step=args.jam_strain, max_compression_steps=args.jam_csteps,
-                           fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
-                                              fmax_target=args.jam_fmax,
-                                              max_steps=args.jam_fire_steps))
-        reqs.append((t, bumpy, st, prm))
-    n_bumpy = sum(1 for r in reqs if r[1])
+    reqs = W.serve_requests(args.requests, args.rate, args.seed, args.bumpy_fraction,
+                            (args.disk_n_lo, args.disk_n_hi), (args.bumpy_n_lo, args.bumpy_n_hi),
+                            (args.p_lo, args.p_hi), args.jam_phi0)
+    n_bumpy = sum(1 for r in reqs if r.bumpy)
     print(f"[bench] serving {len(reqs)} requests ({n_bumpy} rigid bumpy) at rate {args.rate}/s",
           file=sys.stderr, flush=True)
 
-    def submit(sv, r):
-        _, bumpy, st, prm = r
-        if bumpy:
+    def submit(sv, q):
+        st, prm = q.state, _jam_prm(P, args, q.pressure)
+        if q.bumpy:
             return sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm,
                              e_c=W.bumpy_contact_scale(st), angle=st.angle,
                              vertices_per_particle=st.nverts, vertex_radius=st.rv)
@@ -545,23 +603,27 @@ def serve(args, P, W):
     done = 0
     while done < len(reqs):
         now = time.perf_counter() - t0
-        while nxt < len(reqs) and reqs[nxt][0] <= now:
+        while nxt < len(reqs) and reqs[nxt].arrival <= now:
             ids[nxt] = submit(sv, reqs[nxt])
             nxt += 1
         done, inflight, queued = sv.progress(args.batch_chunk)
         if inflight == 0 and queued == 0 and nxt < len(reqs):
-            time.sleep(max(0.0, min(0.01, reqs[nxt][0] - (time.perf_counter() - t0))))
+            time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (time.perf_counter() - t0))))
     elapsed = time.perf_counter() - t0
     lat = np.array([sv.result(i)["latency_s"] for i in ids])
     served = [sv.result(i)["result"] for i in ids]
+    for i in ids:
+        sv.release(i)
     sv.close()
+    status = np.array([r.status for r in served])
     # the plain path: the same arrivals, one request at a time
     print("[bench] plain sequential path", file=sys.stderr, flush=True)
     k_plain = min(args.plain_sample, len(reqs))
     free_at, lat_plain, busy, same = 0.0, [], 0.0, True
     for k in range(k_plain):
-        arr, bumpy, st, prm = reqs[k]
-        if bumpy:
+        q = reqs[k]
+        st, prm = q.state, _jam_prm(P, args, q.pressure)
+        if q.bumpy:
             s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st))
         else:
             s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
@@ -571,15 +633,25 @@ def serve(args, P, W):
         s.synchronize()
         dt = time.perf_counter() - c0
         s.close()
-        start = max(arr, free_at)
+        start = max(q.arrival, free_at)
         free_at = start + dt
         busy += dt
-        lat_plain.append(free_at - arr)
-        for f in ("status", "fire_steps_total", "pressure", "pe", "packing_fraction"):
+        lat_plain.append(free_at - q.arrival)
+        for f in ("status", "compression_steps", "fire_steps_total", "pressure", "pe",
+                  "packing_fraction", "contacts", "box_length"):
             same = same and getattr(r, f) == getattr(served[k], f)
     plain_rate = k_plain / busy
-    value = len(reqs) / elapsed
+    value = int(np.sum(status == 0)) / elapsed
     lp = np.array(lat_plain)
+    disks = [q.state for q in reqs if not q.bumpy]
+    roof = batch_force_roofline(P, disks, args.roofline_steps) if disks else None
+    cpu = None
+    if not args.no_cpu_baseline and disks:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = oracle_jam_sample(P, W, [(q.state, _jam_prm(P, args, q.pressure))
+                                       for q in reqs if not q.bumpy], args.cpu_budget)
+        if cpu:
+            cpu["sample"] = "the disk requests in arrival order: " + cpu["sample"]
     out = {
         "metric": "served jammed packings/s and latency vs the plain sequential GPU path",
         "value": value,
@@ -604,6 +676,9 @@ def serve(args, P, W):
                         f"{args.jam_fire_steps} steps each)",
             "requests": len(reqs),
             "rigid_bumpy_requests": n_bumpy,
+            "jammed_status0": int(np.sum(status == 0)),
+            "status_counts": {str(int(k)): int(v) for k, v in
+                              zip(*np.unique(status, return_counts=True))},
             "elapsed_s": elapsed,
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
@@ -614,7 +689,8 @@ def serve(args, P, W):
             "speedup_vs_plain": value / plain_rate,
             "sample_results_bit_identical": same,
         },
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -655,7 +731,9 @@ def main():
     ap.add_argument("--batch-n-lo", type=int, default=64)
     ap.add_argument("--batch-n-hi", type=int, default=2048)
     ap.add_argument("--batch-max-steps", type=int, default=10**9)
-    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--roofline-steps", type=int, default=100)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=100)
     ap.add_argument("--requests", type=int, default=400)

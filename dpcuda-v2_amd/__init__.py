@@ -174,6 +174,8 @@ _GPU_ONLY = {
     "batch_jam": (_I, [_P, C.POINTER(JamParams), _L, _P, C.POINTER(_L)]),
     "batch_finish_steps": (_I, [_P, _P]),
     "batch_minimize": (_I, [_P, C.POINTER(FireParams), _L, _P, C.POINTER(_L)]),
+    "batch_kernel_time": (_I, [_P, C.POINTER(_D), C.POINTER(_L)]),
+    "batch_row_entries": (_I, [_P, C.POINTER(_L)]),
     "server_release": (_I, [_P, _L]),
     "disk_forces_noalloc": (_I, [_P, _P, _P, _L, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     "server_create": (_P, [_L, _L, _L, C.c_int32]),
@@ -523,6 +525,17 @@ class Batch:
         self._c("batch_autotune", C.byref(params), int(trial_steps), _ptr(c), len(c), _ptr(k),
                 len(k), C.byref(bc), C.byref(bk), C.byref(ms))
         return bc.value, bk.value, ms.value
+
+    def kernel_time(self):
+        """(milliseconds, launches) of the timed force passes (DPM_OPT_TIMING)."""
+        ms, n = C.c_double(), C.c_int64()
+        self._c("batch_kernel_time", C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def row_entries(self):
+        t = C.c_int64()
+        self._c("batch_row_entries", C.byref(t))
+        return t.value
 
     def finish_steps(self):
         out = np.empty(len(self.sizes), dtype=np.int64)

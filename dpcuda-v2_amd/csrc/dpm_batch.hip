@@ -822,6 +822,12 @@ struct dpm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t graph_steps = 0;
+    // DPM_OPT_TIMING: HIP events around every k_b_force launch (eager launches)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    std::vector<hipEvent_t> ev_free, ev_all;
+    double force_ms = 0.0;
+    int64_t force_launches = 0;
 };
 
 namespace dpm {
@@ -924,6 +930,34 @@ static BJam jam_constants(const dpm_jam_params* P) {
     return J;
 }
 
+static hipEvent_t batch_event(dpm_batch* b) {
+    if (b->ev_free.empty()) {
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        b->ev_all.push_back(e);
+        return e;
+    }
+    hipEvent_t e = b->ev_free.back();
+    b->ev_free.pop_back();
+    return e;
+}
+
+// Adds the elapsed time of every recorded k_b_force launch (waits for the stream).
+static void batch_collect_timing(dpm_batch* b) {
+    if (b->ev_pending.empty()) return;
+    (void)hipStreamSynchronize(b->stream);
+    for (auto& pr : b->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            b->force_ms += ms;
+            b->force_launches += 1;
+        }
+        b->ev_free.push_back(pr.first);
+        b->ev_free.push_back(pr.second);
+    }
+    b->ev_pending.clear();
+}
+
 static void enqueue_batch_step(dpm_batch* b) {
     const BDev& d = b->d;
     hipStream_t st = b->stream;
@@ -933,10 +967,20 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)std::min<long long>(d.nctiles, BC_GRID)),
                        dim3(BC_BLOCK), 0, st, d);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (b->timing) {
+        ea = batch_event(b);
+        eb = batch_event(b);
+        (void)hipEventRecord(ea, st);
+    }
     if (d.fastdiv)
         hipLaunchKernelGGL(k_b_force<true>, slots, dim3(BLOCK), 0, st, d);
     else
         hipLaunchKernelGGL(k_b_force<false>, slots, dim3(BLOCK), 0, st, d);
+    if (b->timing) {
+        (void)hipEventRecord(eb, st);
+        b->ev_pending.push_back({ea, eb});
+    }
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_jam, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
@@ -1048,6 +1092,7 @@ void dpm_batch_destroy(dpm_batch* b) {
     if (b->hflags) (void)hipHostFree(b->hflags);
     for (auto e : b->poll)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : b->ev_all) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1125,8 +1170,38 @@ int dpm_batch_set_option(dpm_batch* b, int option, int64_t value) {
         b->steps_per_graph = value;
         return 0;
     }
+    if (option == DPM_OPT_TIMING) {
+        batch_collect_timing(b);
+        b->timing = value != 0;
+        b->force_ms = 0.0;
+        b->force_launches = 0;
+        return 0;
+    }
     set_error("dpm_batch_set_option: unknown option");
     return -1;
+}
+
+int dpm_batch_kernel_time(dpm_batch* b, double* force_ms, int64_t* launches) {
+    batch_collect_timing(b);
+    if (force_ms) *force_ms = b->force_ms;
+    if (launches) *launches = b->force_launches;
+    return 0;
+}
+
+int dpm_batch_row_entries(dpm_batch* b, int64_t* total) {
+    if (!total) {
+        set_error("dpm_batch_row_entries: null output");
+        return -1;
+    }
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    std::vector<int> nn(b->nslot), own(b->nslot);
+    BCOPY(nn.data(), b->d.nnum, sizeof(int) * b->nslot, hipMemcpyDeviceToHost);
+    BCOPY(own.data(), b->d.sys_of, sizeof(int) * b->nslot, hipMemcpyDeviceToHost);
+    int64_t t = 0;
+    for (int64_t k = 0; k < b->nslot; ++k)
+        if (own[k] >= 0) t += std::min(nn[k], b->d.cap);
+    *total = t;
+    return 0;
 }
 
 }  // extern "C"
@@ -1197,7 +1272,7 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
     while (!all_done) {
         while (inflight < 2 && launched < max_batched_steps) {
             int64_t k = std::min<int64_t>(b->steps_per_graph, max_batched_steps - launched);
-            if (b->use_graph && k == b->steps_per_graph) {
+            if (b->use_graph && !b->timing && k == b->steps_per_graph) {
                 if (ensure_batch_graph(b, k)) return -1;
                 DPM_HIP(hipGraphLaunch(b->exec, b->stream));
             } else {
@@ -1226,6 +1301,7 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
         if (f[2] >= b->nsys) all_done = true;
     }
     DPM_HIP(hipStreamSynchronize(b->stream));
+    batch_collect_timing(b);
     if (steps_done) *steps_done = finished_steps;
     BCOPY(b->hsys.data(), d.sys, sizeof(BSys) * b->nsys, hipMemcpyDeviceToHost);
     for (int64_t s = 0; s < b->nsys; ++s) {

@@ -3,11 +3,15 @@
 
     python3 prof_summary.py trace DIR [--top K]
     python3 prof_summary.py pmc DIR [--kernel SUBSTRING]
+    python3 prof_summary.py gbs DIR --bytes SUBSTRING=BYTES [...] [--min-us T]
 
 ``trace``: per kernel name, dispatch count, mean / total / min / max duration (from
 ``rocprofv3 --kernel-trace --stats``).  ``pmc``: per kernel and counter, dispatch
 count, mean and median of the counter value (from ``rocprofv3 --pmc ...``); with
-FETCH_SIZE / WRITE_SIZE the values are KiB per dispatch.
+FETCH_SIZE / WRITE_SIZE the values are KiB per dispatch.  ``gbs``: for each kernel whose
+name contains SUBSTRING, the dispatches that ran longer than T microseconds (launches of a
+gated kernel that only skip are shorter) and the algorithmic GB/s of BYTES per dispatch over
+their mean duration.
 """
 from __future__ import annotations
 
@@ -57,14 +61,44 @@ def pmc(root, kernel):
     return "\n".join(out)
 
 
+def gbs(root, specs, min_us):
+    rows = {}
+    for db in _dbs(root):
+        con = sqlite3.connect(db)
+        for name, dur in con.execute("select name, duration from kernels"):
+            rows.setdefault(name, []).append(dur)
+        con.close()
+    out = [f"{'kernel':52s} {'bytes':>12s} {'n_run':>6s} {'mean_us':>9s} {'GB/s':>8s} "
+           f"{'of 8 TB/s':>9s}"]
+    for spec in specs:
+        sub, b = spec.split("=")
+        nbytes = float(b)
+        for name, d in sorted(rows.items()):
+            if sub not in name:
+                continue
+            run = [x for x in d if x > min_us * 1e3]
+            if not run:
+                continue
+            mean = sum(run) / len(run)
+            g = nbytes / mean  # bytes per ns = GB/s
+            out.append(f"{name[:52]:52s} {nbytes:12.4g} {len(run):6d} {mean / 1e3:9.3f} "
+                       f"{g:8.1f} {g / 8000:9.3f}")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("trace", "pmc"))
+    ap.add_argument("mode", choices=("trace", "pmc", "gbs"))
     ap.add_argument("dir")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--kernel", default="")
+    ap.add_argument("--bytes", nargs="*", default=[])
+    ap.add_argument("--min-us", type=float, default=3.0)
     a = ap.parse_args()
-    print(trace(a.dir, a.top) if a.mode == "trace" else pmc(a.dir, a.kernel))
+    if a.mode == "gbs":
+        print(gbs(a.dir, a.bytes, a.min_us))
+    else:
+        print(trace(a.dir, a.top) if a.mode == "trace" else pmc(a.dir, a.kernel))
 
 
 if __name__ == "__main__":

@@ -236,8 +236,8 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* n_vertices_per_particle, doubl
  * dpm_batch_config: pair law (e_c, n_c), Verlet cutoff and rebuild threshold (3 max d^2 >
  * disp_thresh_sq), common to all systems.  dpm_batch_jam: out[] has n_systems entries;
  * status -1 marks a system still running after max_batched_steps; steps_done returns the
- * batched steps executed.  dpm_batch_set_option takes DPM_OPT_GRAPH and DPM_OPT_CHUNK
- * (batched steps per graph launch). */
+ * batched steps executed.  dpm_batch_set_option takes DPM_OPT_GRAPH, DPM_OPT_CHUNK
+ * (batched steps per graph launch) and DPM_OPT_TIMING. */
 typedef struct dpm_batch dpm_batch;
 dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles);
 void dpm_batch_destroy(dpm_batch* b);
@@ -259,6 +259,12 @@ int dpm_batch_minimize(dpm_batch* b, const dpm_fire_params* p, int64_t max_batch
 /* the batched step (1-based) in which each system finished its last dpm_batch_jam, -1 if
  * it has not */
 int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out);
+/* Measurement: with dpm_batch_set_option(b, DPM_OPT_TIMING, 1) every batched step is
+ * launched eagerly with HIP events around its force pass; dpm_batch_kernel_time returns
+ * the summed milliseconds and the number of timed passes since the option was set.
+ * dpm_batch_row_entries: the stored neighbour-row entries of every system's particles. */
+int dpm_batch_kernel_time(dpm_batch* b, double* force_ms, int64_t* launches);
+int dpm_batch_row_entries(dpm_batch* b, int64_t* total);
 /* Autotuner: times trial_steps batched steps of dpm_batch_jam for every pair of a Verlet
  * cutoff (rebuild threshold: half its skin) and a number of steps per graph launch,
  * restoring the configured state after each trial, and keeps the fastest pair.  Neither
