@@ -596,7 +596,7 @@ def serve(args, P, W):
         return sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm)
 
     sv = P.Server(max_systems=args.serve_systems, arena_slots=args.serve_slots,
-                  budget_bytes=args.serve_budget, bumpy_workers=args.serve_workers)
+                  budget_bytes=args.serve_budget, vertex_slots=args.serve_vertex_slots)
     ids = [None] * len(reqs)
     t0 = time.perf_counter()
     nxt = 0
@@ -624,7 +624,8 @@ def serve(args, P, W):
         q = reqs[k]
         st, prm = q.state, _jam_prm(P, args, q.pressure)
         if q.bumpy:
-            s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st))
+            s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st),
+                                    method=P.NBR_VERLET)
         else:
             s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
         s.synchronize()
@@ -748,7 +749,7 @@ def main():
     ap.add_argument("--serve-systems", type=int, default=1024)
     ap.add_argument("--serve-slots", type=int, default=1 << 18)
     ap.add_argument("--serve-budget", type=int, default=16 << 30)
-    ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-vertex-slots", type=int, default=1 << 22)
     ap.add_argument("--workload", choices=("nve-disk", "fire-rigid", "jam-batch", "serve",
                                            "pairs", "nve-logged"),
                     default="nve-disk",

@@ -166,6 +166,9 @@ _GPU_ONLY = {
     "reset_kernel_time": (_I, [_P]),
     "synchronize": (_I, [_P]),
     "batch_create": (_P, [_L, _P]),
+    "batch_create_mixed": (_P, [_L, _P, _P]),
+    "batch_set_rigid_system": (_I, [_P, _L, _P, _P, _P, _P, _P, _P, _D, _D, C.c_int32, _D]),
+    "batch_get_rigid_system": (_I, [_P, _L, _P, _P, _P, _P, _P, C.POINTER(_D)]),
     "batch_destroy": (None, [_P]),
     "batch_set_system": (_I, [_P, _L, _P, _P, _P, _P, _D]),
     "batch_get_system": (_I, [_P, _L, _P, _P, C.POINTER(_D)]),
@@ -178,7 +181,7 @@ _GPU_ONLY = {
     "batch_row_entries": (_I, [_P, C.POINTER(_L)]),
     "server_release": (_I, [_P, _L]),
     "disk_forces_noalloc": (_I, [_P, _P, _P, _L, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
-    "server_create": (_P, [_L, _L, _L, C.c_int32]),
+    "server_create": (_P, [_L, _L, _L, _L]),
     "server_destroy": (None, [_P]),
     "server_submit": (_L, [_P, C.POINTER(Request)]),
     "server_progress": (_I, [_P, _L, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
@@ -469,13 +472,20 @@ class System:
 
 
 class Batch:
-    """Many independent disk systems jammed together on one GPU (dpm_batch_*)."""
+    """Many independent systems jammed together on one GPU (dpm_batch_*): disks, and with
+    `vertices` (vertices per system, 0 for a disk system) rigid bumpy systems beside them."""
 
-    def __init__(self, sizes):
+    def __init__(self, sizes, vertices=None):
         self.be = load()
         self.sizes = [int(n) for n in sizes]
         a = np.ascontiguousarray(self.sizes, dtype=np.int64)
-        self.h = self.be.fn["batch_create"](len(self.sizes), _ptr(a))
+        if vertices is None:
+            self.vertices = [0] * len(self.sizes)
+            self.h = self.be.fn["batch_create"](len(self.sizes), _ptr(a))
+        else:
+            self.vertices = [int(v) for v in vertices]
+            v = np.ascontiguousarray(self.vertices, dtype=np.int64)
+            self.h = self.be.fn["batch_create_mixed"](len(self.sizes), _ptr(a), _ptr(v))
         if not self.h:
             raise DpmError(f"dpm_batch_create: {self.be.err()}")
 
@@ -510,6 +520,27 @@ class Batch:
         side = C.c_double()
         self._c("batch_get_system", int(k), _ptr(x), _ptr(y), C.byref(side))
         return x, y, side.value
+
+    def set_rigid_system(self, k, x, y, rad, mass, angle, vertices_per_particle, vertex_radius,
+                         side, e_c=1.0, rotation=True):
+        if not 0 <= k < len(self.sizes):
+            raise DpmError(f"set_rigid_system: no system {k}")
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, rad, mass, angle)]
+        nv = np.ascontiguousarray(vertices_per_particle, dtype=np.int64)
+        if any(len(v) != self.sizes[k] for v in arrs) or len(nv) != self.sizes[k]:
+            raise DpmError("set_rigid_system: array lengths differ from the system size")
+        self._c("batch_set_rigid_system", int(k), *[_ptr(v) for v in arrs], _ptr(nv),
+                float(vertex_radius), float(e_c), int(bool(rotation)), float(side))
+
+    def get_rigid_system(self, k):
+        """(x, y, angle, vertex_x, vertex_y, box side) of rigid system k."""
+        n, nv = self.sizes[k], self.vertices[k]
+        x, y, a = np.empty(n), np.empty(n), np.empty(n)
+        vx, vy = np.empty(nv), np.empty(nv)
+        side = C.c_double()
+        self._c("batch_get_rigid_system", int(k), _ptr(x), _ptr(y), _ptr(a), _ptr(vx), _ptr(vy),
+                C.byref(side))
+        return x, y, a, vx, vy, side.value
 
     def config(self, e_c=1.0, n_c=2.0, cutoff=1.7, disp_thresh_sq=3 * 0.15 ** 2):
         self._c("batch_config", float(e_c), float(n_c), float(cutoff), float(disp_thresh_sq))
@@ -566,10 +597,10 @@ class Server:
     """Continuous batching of jamming and minimisation requests on one GPU (dpm_server_*)."""
 
     def __init__(self, max_systems=4096, arena_slots=1 << 22, budget_bytes=8 << 30,
-                 bumpy_workers=2):
+                 vertex_slots=1 << 22):
         self.be = load()
         self.h = self.be.fn["server_create"](int(max_systems), int(arena_slots),
-                                             int(budget_bytes), int(bumpy_workers))
+                                             int(budget_bytes), int(vertex_slots))
         if not self.h:
             raise DpmError(f"dpm_server_create: {self.be.err()}")
         self.n = {}

@@ -1,14 +1,19 @@
-// dpm_batch.hip -- batched jamming engine: many independent disk systems of uneven N
-// in one device arena, all advanced by the same per-phase kernels, which a hipGraph
-// replays; each system carries its own FIRE state, jamming controller and mode, so a
-// system that has finished drops out (its waves exit at once) while the others go on.
+// dpm_batch.hip -- batched jamming engine: many independent systems of uneven N (disks
+// and rigid bumpy particles side by side) in one device arena, all advanced by the same
+// per-phase kernels, which a hipGraph replays; each system carries its own FIRE state,
+// jamming controller and mode, so a system that has finished drops out (its waves exit
+// at once) while the others go on.
 //
 // Per system the arithmetic is that of one dpm_system with the Verlet method running
 // dpm_jam (the pressure bisection of scripts/runArthurJamWithFire.cpp:36-106 around
 // minimizeFire, src/routines/minimization.cu:75-112): the same pair law and row order
 // (ascending index), the same FIRE update, the same canonical 64-element reduction
-// tree and the same controller steps, so every system's result is bit-identical to
-// that system jammed alone (alone or in any mix).
+// tree, the same controller steps and the same list rebuilds (a full rebuild after a
+// restore; the prologue's test on the displacements at the end of the previous
+// minimisation), so every system's result is bit-identical to that system jammed alone
+// (alone or in any mix).  A rigid bumpy system follows the rigid path of dpm_system
+// (vertex rows from the particle rows, vertex forces summed per particle in vertex
+// order, the rigid FIRE moves).
 //
 // Arena layout: system s owns slots [off_s, off_s + n_s) with off_s a multiple of 64,
 // so a wavefront never spans two systems and a wave's 64 lanes are one canonical
@@ -16,20 +21,36 @@
 //   P, snap   : double4 {x, y, radius, mass}      vel, frc, lref : double2
 //   pe        : double                            nnum, sys_of  : int32
 //   nbr       : int32 ELL, column-major nbr[k*S + slot] (S = arena slots)
-//   part      : per chunk, 9 partial sums / maxima (below)
+//   part      : per chunk, 10 partial sums / maxima (below)
+// A rigid system also owns vertex slots [voff_s, voff_s + nvs_s) (voff_s a multiple of
+// 64; a padding vertex has vpart = -1), its particles' vertices contiguous in particle
+// order (vst = first vertex slot, nvt = count):
+//   vpos, vsnap, vfrc : double2     vtq, vpe : double     vpart, vsys_of, vnum : int32
+//   vnbr      : int32 ELL, column-major vnbr[k*V + vslot] (V = vertex slots)
+//   ang, asnap, omg, tq, moi : double per particle slot; lpos, mpos, rot0, rot1 : the
+//   centres and rotations of the step's moves, read by the vertex pass
 //
 // One batched step is a fixed sequence of launches:
 //   k_b_prep_move  per slot: snapshot save/restore and box scaling requested by the
-//               controller, zero velocities (minimizeFire prologue), displacement
-//               maxima; then the FIRE move of systems inside a minimisation
-//               (Fire::step first half)
+//               controller, zero velocities (minimizeFire prologue); then the FIRE move
+//               of systems inside a minimisation (Fire::step first half) and its
+//               displacement flags
 //   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
 //   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
 //   k_b_build_cells  systems of 128+ particles: the same rows from a cell grid in LDS
 //   k_b_force   forces (+ closing kick) and the chunk partials of P, PE, max F^2
 //   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision
-//   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials
+//   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials and the
+//               displacement flags the next prologue tests
 //   k_b_jam     per system: the jamming controller (next minimisation, or done)
+// and for rigid systems (launched when the arena has vertex slots):
+//   k_rb_vprep  per vertex: the controller's snapshot / scaling on the vertices (before
+//               k_b_prep_move moves the centres they follow)
+//   k_rb_vmove  per vertex: the rotations of the step's rigid moves
+//   k_rb_vbuild per vertex: vertex rows from the particle rows (after the builds)
+//   k_rb_vforce per vertex: force, torque and energy from the vertex row
+//   k_rb_reduce per particle: the sums over the particle's vertices in vertex order, the
+//               closing kick and the chunk partials
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -37,10 +58,17 @@
 #include <vector>
 
 #include "dpm_kernels.hpp"
+#include "dpm_rigid.hpp"
 
 namespace dpm {
 
-enum BMode { BM_PROLOGUE = 0, BM_FIRE = 1, BM_EVAL = 2, BM_DONE = 3, BM_EMPTY = 4 };
+// BM_RESTORE: the step that restores the snapshot and rebuilds the lists at the restored
+// positions (dpm_snapshot_restore's full rebuild) before the box scaling and the
+// minimisation that follow it
+enum BMode {
+    BM_PROLOGUE = 0, BM_FIRE = 1, BM_EVAL = 2, BM_DONE = 3, BM_EMPTY = 4, BM_RESTORE = 5
+};
+enum BKind { BK_DISK = 0, BK_RIGID = 1 };
 
 // dpm_jam_params of one system (minimizeFire's and the controller's constants)
 struct BJam {
@@ -51,7 +79,9 @@ struct BJam {
 };
 // BJ_MIN: one minimisation and nothing else (a minimisation request)
 enum BPhase { BJ_DECOMP = 0, BJ_COMP = 1, BJ_MIN = 2 };
-enum BPart { BP_PX = 0, BP_PY, BP_PE, BP_F2, BP_SXX, BP_SYY, BP_R2, BP_DMAX, BP_Z, BP_COUNT };
+enum BPart {
+    BP_PX = 0, BP_PY, BP_PE, BP_F2, BP_SXX, BP_SYY, BP_R2, BP_DMAX, BP_Z, BP_TW, BP_COUNT
+};
 
 struct BSys {
     long long off;  // first arena slot (multiple of 64)
@@ -74,6 +104,15 @@ struct BSys {
     double pe_final, phi_final;
     long long done_step;  // batched step in which the system finished
     BJam J;
+    // per-system constants: list cutoff, Verlet threshold, the half-box limit of the
+    // branch-free minimum image, contact energy; rigid systems also their vertex slots,
+    // vertex radius and vertex cutoffs (vc2: vertex-vertex, vp2: vertex-particle)
+    int kind, rotation;
+    int nv, nvs;
+    long long voff;
+    double rc, rc2, thr2, lim, e_c;
+    double rv, vc2, vp2;
+    double pend_f;  // BM_RESTORE: the box scaling applied after the restore's rebuild
 };
 
 struct BDev {
@@ -94,9 +133,16 @@ struct BDev {
                        // [5] cell-grid workgroups in use
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
-    double e_c, n_c, ncm1, two_nc, inv2nc;
+    double n_c, ncm1, two_nc, inv2nc;
     int fastdiv, pad;
-    double rc, rc2, thr2;
+    // rigid bumpy arena (nvslot = 0: none)
+    long long nvslot;
+    int vcap, pad2;
+    int *vsys_of, *vpart, *vnum, *vnbr, *vst, *nvt;
+    double2 *vpos, *vsnap, *vfrc;
+    double *vtq, *vpe;
+    double *ang, *asnap, *omg, *tq, *moi;
+    double2 *lpos, *mpos, *rot0, *rot1;
 };
 
 __device__ __forceinline__ double* bpart(const BDev& b, int which) {
@@ -111,13 +157,12 @@ __device__ __forceinline__ PairParams bparams(const BDev& b, const BSys& S) {
     p.invLy = 1.0 / S.Ly;
     p.halfLx = 0.5 * S.Lx;
     p.halfLy = 0.5 * S.Ly;
-    p.e_c = b.e_c;
+    p.e_c = S.e_c;
     p.n_c = b.n_c;
     p.ncm1 = b.ncm1;
     p.two_nc = b.two_nc;
     p.inv2nc = b.inv2nc;
-    double lim = fmax(2.0 * S.rmax, b.rc);
-    p.fast = 0.5 * fmin(S.Lx, S.Ly) > lim * (1.0 + 1e-9) ? 1 : 0;
+    p.fast = 0.5 * fmin(S.Lx, S.Ly) > S.lim * (1.0 + 1e-9) ? 1 : 0;
     p.fastdiv = b.fastdiv;
     return p;
 }
@@ -138,50 +183,75 @@ __device__ __forceinline__ void fire_reset(const BDev& b, BSys& S) {
 }
 
 // ------------------------------------------------------------------ kernels
+// The controller's pending operations on one particle slot, in the controller's order:
+// revertToLastState (2) or setLastState (1), then scaleBox's position scaling
+// (kernelScalePositions); op 3 (leaving a full decompression loop) saves after the
+// scaling.  A rigid particle's angle goes with its position.
+__device__ __forceinline__ double4 b_slot_ops(const BDev& b, const BSys& S, long long i,
+                                              bool rigid) {
+    double4 q = b.P[i];
+    double a = rigid ? b.ang[i] : 0.0;
+    if (S.snap_op == 2) {
+        q = b.snap[i];
+        if (rigid) a = b.asnap[i];
+    }
+    if (S.snap_op == 1) {
+        b.snap[i] = q;
+        if (rigid) b.asnap[i] = a;
+    }
+    if (S.scale_f != 0.0) {
+        q.x *= S.scale_f;
+        q.y *= S.scale_f;
+    }
+    if (S.snap_op == 3) {
+        b.snap[i] = q;
+        if (rigid) b.asnap[i] = a;
+    }
+    b.P[i] = q;
+    if (rigid) b.ang[i] = a;
+    return q;
+}
+
+// The same operations on vertex slot v of particle p: the vertex snapshot, and the
+// scaling's translation of the vertex with its centre (kernelScalePositions,
+// rigid_bumpy/kernels.cu:1308-1333).  Reads the centre before b_slot_ops moves it.
+__device__ __forceinline__ void rb_vertex_ops(const BDev& b, const BSys& S, long long v, int p) {
+    double2 x = b.vpos[v];
+    if (S.snap_op == 2) x = b.vsnap[v];
+    if (S.snap_op == 1) b.vsnap[v] = x;
+    if (S.scale_f != 0.0) {
+        const double4 q = S.snap_op == 2 ? b.snap[p] : b.P[p];
+        const double nx = q.x * S.scale_f, ny = q.y * S.scale_f;
+        x.x += (nx - q.x);
+        x.y += (ny - q.y);
+    }
+    if (S.snap_op == 3) b.vsnap[v] = x;
+    b.vpos[v] = x;
+}
+
 // Each slot's controller work (b_prep_slot) and then its FIRE move (b_move_slot), in
 // one kernel: both touch only their own slot, so the order per slot is the order of
 // two separate passes.  Every condition that guards a ballot is uniform over a wave
 // (one system per wave).
-__device__ __forceinline__ void b_prep_slot(const BDev& b, long long i) {
-    const int s = b.sys_of[i];
-    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
-    const BSys& S = b.sys[s];
+// The prologue's rebuild test reads the displacement flags k_b_eval recorded at the end of
+// the previous minimisation, as dpm_minimize_fire's prologue reads the displacement
+// maxima of the previous one's closing k_disp_partials: the controller's scaling in
+// between is not measured (a restore rebuilds in its own step).
+__device__ __forceinline__ void b_prep_slot(const BDev& b, const BSys& S, long long i,
+                                            bool rigid) {
     const bool start = S.mode == BM_PROLOGUE;
     if (!start && S.snap_op == 0) return;  // wave-uniform: one system per wave
-    const bool live = i - S.off < S.n;
-    double d2 = 0.0;
-    if (live) {
-        // the controller's order: revertToLastState or setLastState, then scaleBox;
-        // op 3 (leaving a full decompression loop) saves after the scaling
-        double4 q = b.P[i];
-        if (S.snap_op == 2) q = b.snap[i];
-        if (S.snap_op == 1) b.snap[i] = q;
-        if (S.scale_f != 0.0) {
-            q.x *= S.scale_f;
-            q.y *= S.scale_f;
-        }
-        if (S.snap_op == 3) b.snap[i] = q;
-        b.P[i] = q;
-        if (start) {
-            b.vel[i] = make_double2(0.0, 0.0);  // minimizeFire: setVelocitiesToZero
-            double2 l = b.lref[i];
-            double dx = q.x - l.x, dy = q.y - l.y;
-            d2 = dx * dx + dy * dy;
-        }
+    if (i - S.off >= S.n) return;
+    b_slot_ops(b, S, i, rigid);
+    if (start) {
+        b.vel[i] = make_double2(0.0, 0.0);  // minimizeFire: setVelocitiesToZero
+        if (rigid) b.omg[i] = 0.0;
     }
-    if (!start) return;
-    // 3 max d^2 > thr holds exactly when 3 d^2 > thr holds for some slot (3x is monotone
-    // under rounding): one ballot per wave replaces the wave maximum
-    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > b.thr2) != 0;
-    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
 }
 
 // Fire::step first half (src/integrator/fire.cu:14-40): [uphill back-step with the
 // decreased dt, zero v] + kick(dt/2) + mix + drift(dt)
-__device__ __forceinline__ void b_move_slot(const BDev& b, long long i) {
-    const int s = b.sys_of[i];
-    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
-    const BSys& S = b.sys[s];
+__device__ __forceinline__ void b_move_slot(const BDev& b, const BSys& S, long long i) {
     if (S.mode != BM_FIRE) return;
     const bool live = i - S.off < S.n;
     double d2 = 0.0;
@@ -208,19 +278,90 @@ __device__ __forceinline__ void b_move_slot(const BDev& b, long long i) {
         double dx = q.x - l.x, dy = q.y - l.y;
         d2 = dx * dx + dy * dy;
     }
-    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > b.thr2) != 0;
+    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > S.thr2) != 0;
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
+}
+
+// The rigid Fire::step moves (rigid_fire_move); the centres and the sines and cosines of
+// the two rotations are left for k_rb_vmove (k_bfire_move's arithmetic).
+__device__ __forceinline__ void rb_move_slot(const BDev& b, const BSys& S, long long i) {
+    if (S.mode != BM_FIRE) return;
+    const bool live = i - S.off < S.n;
+    double d2 = 0.0;
+    if (live) {
+        double4 q = b.P[i];
+        double2 v = b.vel[i];
+        double a = b.ang[i], w = b.omg[i];
+        double2 old, mid;
+        double dth0, dth;
+        rigid_fire_move(q, v, a, w, b.frc[i], b.tq[i], b.moi[i], S.dt, S.alpha, S.uphill,
+                        S.rotation, old, mid, dth0, dth);
+        b.lpos[i] = old;
+        b.mpos[i] = mid;
+        if (S.uphill) {
+            double s0, c0;
+            dpm_sincos(dth0, &s0, &c0);
+            b.rot0[i] = make_double2(s0, c0);
+        }
+        double s1, c1;
+        dpm_sincos(dth, &s1, &c1);
+        b.rot1[i] = make_double2(s1, c1);
+        b.P[i] = q;
+        b.ang[i] = a;
+        b.vel[i] = v;
+        b.omg[i] = w;
+        double2 l = b.lref[i];
+        double dx = q.x - l.x, dy = q.y - l.y;
+        d2 = dx * dx + dy * dy;
+    }
+    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > S.thr2) != 0;
     if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_b_prep_move(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
-    b_prep_slot(b, i);
-    b_move_slot(b, i);
+    const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
+    const BSys& S = b.sys[s];
+    const bool rigid = S.kind == BK_RIGID;
+    b_prep_slot(b, S, i, rigid);
+    if (rigid) rb_move_slot(b, S, i);
+    else b_move_slot(b, S, i);
+}
+
+// Vertex slots: the controller's operations, before k_b_prep_move moves the centres.
+__global__ __launch_bounds__(BLOCK) void k_rb_vprep(BDev b) {
+    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= b.nvslot) return;
+    const int s = b.vsys_of[v];
+    if (s < 0) return;
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_PROLOGUE && S.snap_op == 0) return;
+    const int p = b.vpart[v];
+    if (p >= 0) rb_vertex_ops(b, S, v, p);
+}
+
+// Vertex slots: the rotations of the step's rigid moves (k_bfire_vertices).
+__global__ __launch_bounds__(BLOCK) void k_rb_vmove(BDev b) {
+    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= b.nvslot) return;
+    const int s = b.vsys_of[v];
+    if (s < 0) return;
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_FIRE) return;
+    const int p = b.vpart[v];
+    if (p < 0) return;
+    const int uphill = S.uphill;
+    const double2 r0 = uphill ? b.rot0[p] : make_double2(0.0, 1.0), r1 = b.rot1[p];
+    const double4 nq = b.P[p];
+    b.vpos[v] = bfire_vertex_one(b.vpos[v], b.lpos[p], b.mpos[p], make_double2(nq.x, nq.y), r0.x,
+                                 r0.y, r1.x, r1.y, uphill);
 }
 
 // Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
-// the controller's requests that b_prep_slot has applied.
+// the controller's requests that b_prep_slot has applied.  A restore step rebuilds
+// unconditionally (dpm_snapshot_restore).
 __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     // one wave per system: the lanes test the system's chunk flags together
     const int s = blockIdx.x, lane = threadIdx.x;
@@ -236,7 +377,7 @@ __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
     if (lane == 0) {
         S.snap_op = 0;
         S.scale_f = 0.0;
-        if (any) S.rebuild = 1;
+        if (any || mode == BM_RESTORE) S.rebuild = 1;
     }
 }
 
@@ -261,7 +402,7 @@ struct BCGrid {
 
 __device__ __forceinline__ bool bc_grid(const BDev& b, const BSys& S, BCGrid& g) {
     if (S.n < BC_MIN_N || S.n > BC_MAX_N) return false;
-    const double w = b.rc * (1.0 + 1e-9);
+    const double w = S.rc * (1.0 + 1e-9);
     g.nx = (int)fmin((double)BC_MAX_D, floor(S.Lx / w));
     g.ny = (int)fmin((double)BC_MAX_D, floor(S.Ly / w));
     if (g.nx < 3 || g.ny < 3) return false;
@@ -385,7 +526,7 @@ __device__ __forceinline__ void bc_build_item(const BDev& b, int2 ct, int* cend,
                 dx = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, 0);
                 dy = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, 0);
             }
-            if (dx * dx + dy * dy < b.rc2) {
+            if (dx * dx + dy * dy < S.rc2) {
                 if (cnt < b.cap) b.nbr[(long long)cnt * b.nslot + i] = (int)j;
                 cnt++;
             }
@@ -454,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
                     dx = pbc(qi.x, tp[u].x, p.Lx, p.invLx, p.halfLx, 0);
                     dy = pbc(qi.y, tp[u].y, p.Ly, p.invLy, p.halfLy, 0);
                 }
-                if (dx * dx + dy * dy < b.rc2) {
+                if (dx * dx + dy * dy < S.rc2) {
                     if (cnt < b.cap) b.nbr[(long long)cnt * b.nslot + i] = (int)j;
                     cnt++;
                 }
@@ -487,6 +628,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
+    if (S.kind == BK_RIGID) return;  // k_rb_vforce + k_rb_reduce
     const int kick = S.mode == BM_FIRE;
     const PairParams p = bparams(b, S);
     const bool live = i - S.off < S.n;
@@ -553,17 +695,220 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     }
 }
 
+// kernelUpdateVertexNeighborList (rigid_bumpy/kernels.cu:870-922) for the rigid systems
+// whose particle rows were rebuilt in this step: for vertex v of particle p, over p's
+// particle row in order, the particles q whose centre lies within vp of v, then q's
+// vertices in order within vc of v (k_bumpy_vertex_build's row order).
+__global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
+    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= b.nvslot) return;
+    const int s = b.vsys_of[v];
+    if (s < 0) return;  // free vertex slots (64-aligned: whole waves)
+    const BSys& S = b.sys[s];
+    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
+    const PairParams pp = bparams(b, S);
+    const int p = b.vpart[v];
+    int cnt = 0;
+    if (p >= 0) {
+        const double2 xv = b.vpos[v];
+        const int nn = min(b.nnum[p], b.cap);
+        for (int k = 0; k < nn; ++k) {
+            const int q = b.nbr[(long long)k * b.nslot + p];
+            const double4 cq = b.P[q];
+            double dx = pbc(xv.x, cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+            double dy = pbc(xv.y, cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+            if (!(dx * dx + dy * dy < S.vp2)) continue;
+            const int f0 = b.vst[q], nq = b.nvt[q];
+            for (int u = f0; u < f0 + nq; ++u) {
+                const double2 xu = b.vpos[u];
+                dx = pbc(xv.x, xu.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+                dy = pbc(xv.y, xu.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                if (dx * dx + dy * dy < S.vc2) {
+                    if (cnt < b.vcap) b.vnbr[(long long)cnt * b.nvslot + v] = u;
+                    cnt++;
+                }
+            }
+        }
+        b.vnum[v] = cnt;
+    }
+    const int mx = wave_max_i(cnt);
+    if ((threadIdx.x % WAVE) == 0) {
+        atomicMax(&b.flags[6], mx);
+        if (mx > b.vcap) atomicMax(&b.flags[0], 1);
+    }
+}
+
+// kernelCalcRigidBumpyForces1 (rigid_bumpy/kernels.cu:366-417) per vertex slot: force,
+// torque about the centre (calcTorque, include/particles/base/kernels.cuh:237-241) and
+// energy from the vertex row, in row order (k_bfire_forces's arithmetic).
+template <bool FASTDIV>
+__global__ __launch_bounds__(BLOCK) void k_rb_vforce(BDev b) {
+    const long long v = (long long)xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (v >= b.nvslot) return;
+    const int s = b.vsys_of[v];
+    if (s < 0) return;  // free vertex slots (64-aligned: whole waves)
+    const BSys& S = b.sys[s];
+    if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
+    const PairParams p = bparams(b, S);
+    const int pv = b.vpart[v];
+    const bool live = pv >= 0;
+    const long long vl = live ? v : S.voff;
+    const double rv = S.rv;
+    const double2 xv = b.vpos[vl];
+    const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
+    const int nn = live ? min(b.vnum[vl], b.vcap) : 0;
+    const int* col = b.vnbr + vl;
+    // two neighbours per batch, the next batch's indices loaded ahead; every ELL slot
+    // holds a valid vertex slot (zero-filled at allocation, only written with slots), and
+    // a slot past the row end adds +0.0, which leaves the sums unchanged
+    const int last = b.vcap - 1;
+    int un[2] = {col[0], col[(long long)min(1, last) * b.nvslot]};
+    double ax = 0.0, ay = 0.0, e = 0.0;
+    for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
+        const double2 x2[2] = {b.vpos[un[0]], b.vpos[un[1]]};
+        if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
+            un[0] = col[(long long)min(k + 2, last) * b.nvslot];
+            un[1] = col[(long long)min(k + 3, last) * b.nvslot];
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const bool valid = k + t < nn;
+            const double4 q = make_double4(x2[t].x, x2[t].y, rv, 0.0);
+            double fx = 0.0, fy = 0.0, ie = 0.0;
+            if constexpr (FASTDIV) {
+                if (p.fast) ie = pair_hot<true>(qv, q, valid, p, fx, fy);
+                else ie = pair_hot<true, true>(qv, q, valid, p, fx, fy);
+            } else if (valid) {
+                ie = pair_law(xv.x, xv.y, rv, q.x, q.y, rv, p, fx, fy);
+            }
+            ax += fx;
+            ay += fy;
+            e += ie;
+        }
+    }
+    if (live) {
+        const double4 cq = b.P[pv];
+        const double dx = xv.x - cq.x, dy = xv.y - cq.y;
+        const double t = ay * dx - ax * dy;
+        b.vfrc[v] = make_double2(0.0 + ax, 0.0 + ay);
+        b.vtq[v] = 0.0 + t;
+        b.vpe[v] = 0.0 + e;
+    }
+}
+
+// kernelCalcRigidBumpyParticleForces1 (rigid_bumpy/kernels.cu:419-442) per rigid
+// particle slot: force, torque and energy summed over the particle's vertices in vertex
+// order, the closing updateVelocities(dt/2) of a FIRE step (kernelUpdateRigidVelocities),
+// and the chunk partials of F.v, tau.w, PE and max(F^2, tau^2) (k_bfire_partials).
+__global__ __launch_bounds__(BLOCK) void k_rb_reduce(BDev b) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
+    const BSys& S = b.sys[s];
+    if (S.kind != BK_RIGID || (S.mode != BM_PROLOGUE && S.mode != BM_FIRE)) return;
+    const bool kick = S.mode == BM_FIRE;
+    const bool live = i - S.off < S.n;
+    double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
+    if (live) {
+        const int st = b.vst[i], nv = b.nvt[i];
+        double ax = 0.0, ay = 0.0, at = 0.0, e = 0.0;
+        for (int k = 0; k < nv; ++k) {
+            const double2 f = b.vfrc[st + k];
+            ax += f.x;
+            ay += f.y;
+            at += b.vtq[st + k];
+            e += b.vpe[st + k];
+        }
+        const double fx = 0.0 + ax, fy = 0.0 + ay, t = 0.0 + at;
+        pe = 0.0 + e;
+        b.frc[i] = make_double2(fx, fy);
+        b.tq[i] = t;
+        b.pe[i] = pe;
+        double2 v = b.vel[i];
+        double w = b.omg[i];
+        if (kick) {
+            const double hdt = S.dt / 2.0;
+            if (S.rotation) w += t * hdt / b.moi[i];
+            else w = 0.0;
+            const double m = b.P[i].w;
+            v.x += fx * hdt / m;
+            v.y += fy * hdt / m;
+            b.vel[i] = v;
+            b.omg[i] = w;
+        }
+        pxv = fx * v.x;
+        pyv = fy * v.y;
+        ptw = t * w;
+        f2 = fmax(fx * fx + fy * fy, t * t);
+    }
+    const double s0 = wave_tree_sum(pxv);
+    const double s1 = wave_tree_sum(pyv);
+    const double s2 = wave_tree_sum(pe);
+    const double s3 = wave_tree_sum(ptw);
+    const double m = wave_max(f2);
+    if ((threadIdx.x % WAVE) == 0) {
+        const long long ch = i / WAVE;
+        bpart(b, BP_PX)[ch] = s0;
+        bpart(b, BP_PY)[ch] = s1;
+        bpart(b, BP_PE)[ch] = s2;
+        bpart(b, BP_TW)[ch] = s3;
+        bpart(b, BP_F2)[ch] = m;
+    }
+}
+
+// Vertex ring, vertex masses and moment of inertia of the particles of one rigid system
+// (k_bumpy_init_vertices' arithmetic: kernelInitializeVerticesOnParticles,
+// rigid_bumpy/kernels.cu:1137-1162, kernelGetMomentsOfInertia, :1171-1189).
+__global__ __launch_bounds__(BLOCK) void k_rb_init(BDev b, long long off, int n, double rv) {
+    const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const long long p = off + k;
+    const double4 cq = b.P[p];
+    const double a = b.ang[p];
+    const int nv = b.nvt[p], st = b.vst[p];
+    const double2 c = make_double2(cq.x, cq.y);
+    for (int i = 0; i < nv; ++i) b.vpos[st + i] = ring_vertex(c, cq.z, a, nv, rv, i);
+    const double vm = cq.w / static_cast<double>(nv);
+    double I = 0.0;
+    for (int i = 0; i < nv; ++i) {
+        const double rx = b.vpos[st + i].x - c.x, ry = b.vpos[st + i].y - c.y;
+        I += vm * (rx * rx + ry * ry);
+    }
+    b.moi[p] = I;
+}
+
+// Particle::scaleBox's box update (particle.cu:1057-1064): side = sqrt(area) * f
+__device__ __forceinline__ void scale_box_of(BSys& S, double f) {
+    double side = sqrt(S.Lx * S.Ly);
+    double ns = side * f;
+    S.Lx = ns;
+    S.Ly = ns;
+    S.scale_f = f;
+}
+
 // one wave per system: minimizeFire's stop test (minimization.cu:96-105, with the
 // optional |F|max criterion) and Fire::step's decision for the next iteration
 // (fire.cu:14-33) -- k_fire_ctl's arithmetic on the system's chunk partials
 __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
     BSys& S = b.sys[blockIdx.x];
     const int mode = S.mode;
+    if (mode == BM_RESTORE) {  // the lists are current for the restored state: scaleBox
+        if (threadIdx.x == 0) {
+            scale_box_of(S, S.pend_f);
+            S.pend_f = 0.0;
+            S.rebuild = 0;
+            S.mode = BM_PROLOGUE;
+        }
+        return;
+    }
     if (mode != BM_PROLOGUE && mode != BM_FIRE) return;
+    const bool rigid = S.kind == BK_RIGID;
     const long long c0 = S.off / WAVE;
-    double px, py, pe, unused;
+    // (the fourth sum is the rigid spin power; a disk system sums its PE array again)
+    double px, py, pe, ptw;
     wave_final_sum4(bpart(b, BP_PX) + c0, bpart(b, BP_PY) + c0, bpart(b, BP_PE) + c0,
-                    bpart(b, BP_PE) + c0, S.nc, px, py, pe, unused);
+                    bpart(b, rigid ? BP_TW : BP_PE) + c0, S.nc, px, py, pe, ptw);
     double fm = 0.0;
     for (int k = threadIdx.x; k < S.nc; k += WAVE) fm = fmax(fm, bpart(b, BP_F2)[c0 + k]);
     fm = wave_max(fm);
@@ -598,7 +943,9 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
             return;
         }
     }
+    // RigidBumpy::getPowerFire adds sum(tau*w) (rigid_bumpy.cu:1181-1191)
     double power = px + py;
+    if (rigid) power = power + ptw;
     S.power = power;
     if (power > 0) {
         S.n_good++;
@@ -628,6 +975,98 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
 // calculateStressTensor (kernelCalcDiskStressTensor, disk/kernels.cu:203-251) and
 // countContacts (kernelCountDiskContacts, disk/kernels.cu:177-201) per particle, with
 // chunk partials of sigma_xx, sigma_yy, r^2 (packing fraction) and the contact count
+__device__ __forceinline__ void b_eval_disk(const BDev& b, const BSys& S, const PairParams& p,
+                                            long long i, double& sxx, double& syy,
+                                            double& z) {
+    b.vel[i] = make_double2(0.0, 0.0);
+    const double4 qi = b.P[i];
+    const double2 v = make_double2(0.0, 0.0);
+    const double m = qi.w;
+    double axx = m * v.x * v.x, ayy = m * v.y * v.y;
+    int c = 0;
+    const int nn = min(b.nnum[i], b.cap);
+    for (int k = 0; k < nn; ++k) {
+        int j = b.nbr[(long long)k * b.nslot + i];
+        double4 qj = b.P[j];
+        double xd = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, p.fast);
+        double yd = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, p.fast);
+        double fx, fy;
+        pair_law(qi.x, qi.y, qi.z, qj.x, qj.y, qj.z, p, fx, fy);
+        axx += xd * fx / 2;
+        ayy += yd * fy / 2;
+        double dist = sqrt(xd * xd + yd * yd);
+        if (dist < qi.z + qj.z) c++;
+    }
+    double area = S.Lx * S.Ly;
+    sxx = axx / area;
+    syy = ayy / area;
+    z = (double)c;
+}
+
+// The rigid forms: setVelocitiesToZero (v and omega), kernelCalcRigidBumpyStressTensor
+// (rigid_bumpy/kernels.cu:705-789: the vertex kinetic terms, here of zero velocity, and
+// the vertex-pair virial) and kernelCountRigidBumpyContacts (rigid_bumpy/kernels.cu:
+// 924-997: per neighbour q, the first of i's vertices touching a vertex of q counts),
+// the arithmetic of k_bumpy_stress and k_bumpy_contacts.
+__device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const PairParams& p,
+                                        long long i, double& sxx, double& syy, double& z) {
+    b.vel[i] = make_double2(0.0, 0.0);
+    b.omg[i] = 0.0;
+    const double4 qi = b.P[i];
+    const int st = b.vst[i], nv = b.nvt[i];
+    const double rv = S.rv;
+    const double vm = qi.w / static_cast<double>(nv);  // k_bumpy_init_vertices' vertex mass
+    const double vvx = 0.0, vvy = 0.0;                   // zero velocity and spin
+    double axx = 0.0, ayy = 0.0;
+    for (int a = 0; a < nv; ++a) {
+        const int v = st + a;
+        const double2 xv = b.vpos[v];
+        axx += vm * vvx * vvx;
+        ayy += vm * vvy * vvy;
+        const int nn = min(b.vnum[v], b.vcap);
+        for (int k = 0; k < nn; ++k) {
+            const int u = b.vnbr[(long long)k * b.nvslot + v];
+            if (b.vpart[u] == i) continue;
+            const double2 xu = b.vpos[u];
+            const double xd = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+            const double yd = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
+            double fx, fy;
+            pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+            axx += xd * fx / 2;
+            ayy += yd * fy / 2;
+        }
+    }
+    const double area = S.Lx * S.Ly;
+    sxx = axx / area;
+    syy = ayy / area;
+    const double vrs = 2 * rv;
+    int nc = 0;
+    const int nn = min(b.nnum[i], b.cap);
+    for (int k = 0; k < nn; ++k) {
+        const int q = b.nbr[(long long)k * b.nslot + i];
+        bool is_contact = false;
+        for (int a = 0; a < nv; ++a) {
+            const int v = st + a;
+            const double2 xv = b.vpos[v];
+            const int vn = min(b.vnum[v], b.vcap);
+            for (int k2 = 0; k2 < vn; ++k2) {
+                const int u = b.vnbr[(long long)k2 * b.nvslot + v];
+                if (b.vpart[u] != q) continue;
+                const double2 xu = b.vpos[u];
+                const double dx = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+                const double dy = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
+                const double dist = sqrt(dx * dx + dy * dy);
+                if (dist < vrs) {
+                    if (!is_contact) nc++;
+                    is_contact = true;
+                }
+            }
+            if (is_contact) break;
+        }
+    }
+    z = (double)nc;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
@@ -637,33 +1076,22 @@ __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     if (S.mode != BM_EVAL) return;
     const PairParams p = bparams(b, S);
     const bool live = i - S.off < S.n;
-    double sxx = 0.0, syy = 0.0, r2 = 0.0, z = 0.0;
+    double sxx = 0.0, syy = 0.0, r2 = 0.0, z = 0.0, d2 = 0.0;
     if (live) {
-        b.vel[i] = make_double2(0.0, 0.0);
-        const double4 qi = b.P[i];
-        const double2 v = make_double2(0.0, 0.0);
-        const double m = qi.w;
-        double axx = m * v.x * v.x, ayy = m * v.y * v.y;
-        int c = 0;
-        const int nn = min(b.nnum[i], b.cap);
-        for (int k = 0; k < nn; ++k) {
-            int j = b.nbr[(long long)k * b.nslot + i];
-            double4 qj = b.P[j];
-            double xd = pbc(qi.x, qj.x, p.Lx, p.invLx, p.halfLx, p.fast);
-            double yd = pbc(qi.y, qj.y, p.Ly, p.invLy, p.halfLy, p.fast);
-            double fx, fy;
-            pair_law(qi.x, qi.y, qi.z, qj.x, qj.y, qj.z, p, fx, fy);
-            axx += xd * fx / 2;
-            ayy += yd * fy / 2;
-            double dist = sqrt(xd * xd + yd * yd);
-            if (dist < qi.z + qj.z) c++;
-        }
-        double area = S.Lx * S.Ly;
-        sxx = axx / area;
-        syy = ayy / area;
-        r2 = qi.z * qi.z;
-        z = (double)c;
+        if (S.kind == BK_RIGID) rb_eval(b, S, p, i, sxx, syy, z);
+        else b_eval_disk(b, S, p, i, sxx, syy, z);
+        const double4 q = b.P[i];
+        r2 = q.z * q.z;
+        // the closing displacement maxima of dpm_minimize_fire (k_disp_partials), which
+        // the next minimisation's prologue tests: 3 max d^2 > thr holds exactly when
+        // 3 d^2 > thr holds for some slot (3x is monotone under rounding), so one ballot
+        // per wave replaces the wave maximum
+        const double2 l = b.lref[i];
+        const double dx = q.x - l.x, dy = q.y - l.y;
+        d2 = dx * dx + dy * dy;
     }
+    const bool far = __builtin_amdgcn_ballot_w64(3.0 * d2 > S.thr2) != 0;
+    if ((threadIdx.x % WAVE) == 0) bpart(b, BP_DMAX)[i / WAVE] = far ? 1.0 : 0.0;
     double s0 = wave_tree_sum(sxx);
     double s1 = wave_tree_sum(syy);
     double s2 = wave_tree_sum(r2);
@@ -677,32 +1105,15 @@ __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     }
 }
 
-// Particle::scaleBox's box update (particle.cu:1057-1064): side = sqrt(area) * f
-__device__ __forceinline__ void scale_box_of(BSys& S, double f) {
-    double side = sqrt(S.Lx * S.Ly);
-    double ns = side * f;
-    S.Lx = ns;
-    S.Ly = ns;
-    S.scale_f = f;
-}
-
 __device__ __forceinline__ void start_minimization(const BDev& b, BSys& S) {
     fire_reset(b, S);
     S.mode = BM_PROLOGUE;
 }
 
-// one wave per system: the jamming controller of dpm_jam after a minimisation
-__global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
-    BSys& S = b.sys[blockIdx.x];
-    if (S.mode != BM_EVAL) return;
-    const long long c0 = S.off / WAVE;
-    double sxx, syy, pe, r2;
-    wave_final_sum4(bpart(b, BP_SXX) + c0, bpart(b, BP_SYY) + c0, bpart(b, BP_PE) + c0,
-                    bpart(b, BP_R2) + c0, S.nc, sxx, syy, pe, r2);
-    double z = 0.0;
-    for (int k = threadIdx.x; k < S.nc; k += WAVE) z += bpart(b, BP_Z)[c0 + k];
-    z = wave_tree_sum(z);  // integer counts: exact in any order
-    if (threadIdx.x != 0) return;
+// The jamming controller of dpm_jam after a minimisation, for lane 0 of the system's
+// wave; returns whether the system has finished.
+__device__ __forceinline__ bool b_jam_decide(const BDev& b, BSys& S, double sxx, double syy,
+                                             double z) {
     S.fire_total += min(S.step, S.J.max_steps);
     const double pr = (sxx + syy) / 2;
     const double P_low = S.J.P_low, P_high = S.J.P_high;
@@ -778,7 +1189,13 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
         }
         if (stop) {
             finished = true;
-            if (restore) S.snap_op = 2;
+        } else if (restore && S.cstep + 1 < S.J.max_csteps) {
+            // dpm_snapshot_restore's full rebuild at the restored state runs in a step of
+            // its own (BM_RESTORE); k_b_ctl then scales the box
+            S.cstep += 1;
+            start_minimization(b, S);
+            S.mode = BM_RESTORE;
+            S.pend_f = L / L_prior;
         } else {
             scale_box_of(S, L / L_prior);
             S.cstep += 1;
@@ -786,12 +1203,50 @@ __global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
             else start_minimization(b, S);
         }
     }
-    if (finished) {
-        S.done_step = b.flags[3];
-        S.pe_final = pe;
-        const double PI = 3.141592653589793238462643383279502884197;
-        S.phi_final = r2 * PI / (S.Lx * S.Ly);
-        S.mode = BM_DONE;
+    return finished;
+}
+
+// one wave per system: the jamming controller of dpm_jam after a minimisation.  A
+// system that finishes gets the controller's last snapshot / scaling operations at once
+// (its slots see no further step).
+__global__ __launch_bounds__(WAVE) void k_b_jam(BDev b) {
+    BSys& S = b.sys[blockIdx.x];
+    if (S.mode != BM_EVAL) return;
+    const long long c0 = S.off / WAVE;
+    double sxx, syy, pe, r2;
+    wave_final_sum4(bpart(b, BP_SXX) + c0, bpart(b, BP_SYY) + c0, bpart(b, BP_PE) + c0,
+                    bpart(b, BP_R2) + c0, S.nc, sxx, syy, pe, r2);
+    double z = 0.0;
+    for (int k = threadIdx.x; k < S.nc; k += WAVE) z += bpart(b, BP_Z)[c0 + k];
+    z = wave_tree_sum(z);  // integer counts: exact in any order
+    __shared__ int apply;
+    if (threadIdx.x == 0) {
+        const bool finished = b_jam_decide(b, S, sxx, syy, z);
+        apply = finished && (S.snap_op != 0 || S.scale_f != 0.0);
+        if (finished) {
+            S.done_step = b.flags[3];
+            S.pe_final = pe;
+            const double PI = 3.141592653589793238462643383279502884197;
+            S.phi_final = r2 * PI / (S.Lx * S.Ly);
+            S.mode = BM_DONE;
+        }
+    }
+    __syncthreads();
+    if (apply) {
+        const bool rigid = S.kind == BK_RIGID;
+        if (rigid)  // vertices first: they read the centres before the particle pass
+            for (long long v = S.voff + threadIdx.x; v < S.voff + S.nvs; v += WAVE) {
+                const int p = b.vpart[v];
+                if (p >= 0) rb_vertex_ops(b, S, v, p);
+            }
+        __syncthreads();
+        for (long long i = S.off + threadIdx.x; i < S.off + S.n; i += WAVE)
+            b_slot_ops(b, S, i, rigid);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && S.mode == BM_DONE) {
+        S.snap_op = 0;
+        S.scale_f = 0.0;
         int k = atomicAdd(&b.flags[2], 1);
         b.done_list[k % b.nsys] = (int)blockIdx.x;
     }
@@ -813,10 +1268,20 @@ struct dpm_batch {
     std::vector<int> n;
     std::vector<dpm::BSys> hsys;
     std::vector<double> rmax, rmin, box0;  // per system: radii range, box side as set
-    double4* P_init = nullptr;              // positions as set (autotuning restores them)
+    // rigid bumpy systems: vertices (0: a disk system), vertex slots, first vertex slot,
+    // vertex radius, contact energy, rotation; nvslot = vertex slots of the arena
+    int64_t nvslot = 0;
+    std::vector<int64_t> nv, nvs, voff;
+    std::vector<double> rv, ec;
+    std::vector<int> rot;
+    // the state as set (autotuning restores it): positions, rigid angles and vertices
+    double4* P_init = nullptr;
+    double* A_init = nullptr;
+    double2* V_init = nullptr;
     double rc = 1.7, thr_sq = 3.0 * 0.15 * 0.15, e_c = 1.0, n_c = 2.0;
     int64_t steps_per_graph = 32;
     bool use_graph = true;
+    bool ran = false;  // a dpm_batch_jam / dpm_batch_minimize has started
     int* hflags = nullptr;  // pinned: [0] overflow, [1] largest row, [2] systems done
     hipEvent_t poll[2] = {nullptr, nullptr};
     hipGraph_t graph = nullptr;
@@ -869,8 +1334,10 @@ static int balloc_n(dpm_batch* b, T** p, int64_t n) {
     return 0;
 }
 
-// Stream, arena and tables for up to nsys systems in nslot slots (sys_of = -1: free).
-static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_cap) {
+// Stream, arena and tables for up to nsys systems in nslot slots and nvslot vertex slots
+// (sys_of, vsys_of, vpart = -1: free).
+static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_cap,
+                       int64_t nvslot) {
     (void)hipGetDevice(&b->device);
     BDev& d = b->d;
     d.nsys = nsys;
@@ -890,6 +1357,24 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
         balloc_n(b, &d.done_list, nsys))
         return -1;
     DPM_HIP(bsync(b, hipMemsetAsync(d.sys_of, 0xff, sizeof(int) * nslot, b->stream)));  // -1: free
+    d.nvslot = nvslot;
+    d.vcap = 32;
+    b->nvslot = nvslot;
+    if (nvslot > 0) {
+        if (balloc_n(b, &d.vsys_of, nvslot) || balloc_n(b, &d.vpart, nvslot) ||
+            balloc_n(b, &d.vnum, nvslot) || balloc_n(b, &d.vnbr, nvslot * d.vcap) ||
+            balloc_n(b, &d.vpos, nvslot) || balloc_n(b, &d.vsnap, nvslot) ||
+            balloc_n(b, &d.vfrc, nvslot) || balloc_n(b, &d.vtq, nvslot) ||
+            balloc_n(b, &d.vpe, nvslot) || balloc_n(b, &d.vst, nslot) ||
+            balloc_n(b, &d.nvt, nslot) || balloc_n(b, &d.ang, nslot) ||
+            balloc_n(b, &d.asnap, nslot) || balloc_n(b, &d.omg, nslot) ||
+            balloc_n(b, &d.tq, nslot) || balloc_n(b, &d.moi, nslot) ||
+            balloc_n(b, &d.lpos, nslot) || balloc_n(b, &d.mpos, nslot) ||
+            balloc_n(b, &d.rot0, nslot) || balloc_n(b, &d.rot1, nslot))
+            return -1;
+        DPM_HIP(bsync(b, hipMemsetAsync(d.vsys_of, 0xff, sizeof(int) * nvslot, b->stream)));
+        DPM_HIP(bsync(b, hipMemsetAsync(d.vpart, 0xff, sizeof(int) * nvslot, b->stream)));
+    }
     DPM_HIP(hipHostMalloc((void**)&b->hflags, 2 * 4 * sizeof(int), hipHostMallocDefault));
     for (int k = 0; k < 2; ++k)
         DPM_HIP(hipEventCreateWithFlags(&b->poll[k], hipEventDisableTiming));
@@ -899,7 +1384,6 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
 // The numbers every system shares, baked into the kernels' arguments.
 static void batch_common(dpm_batch* b, bool fastdiv) {
     BDev& d = b->d;
-    d.e_c = b->e_c;
     d.n_c = b->n_c;
     d.ncm1 = b->n_c - 1.0;
     d.two_nc = 2.0 * b->n_c;
@@ -907,9 +1391,47 @@ static void batch_common(dpm_batch* b, bool fastdiv) {
     double m = frexp(d.two_nc, &ex);
     d.inv2nc = (m == 0.5) ? 1.0 / d.two_nc : 0.0;
     d.fastdiv = fastdiv ? 1 : 0;
-    d.rc = b->rc;
-    d.rc2 = b->rc * b->rc;
-    d.thr2 = b->thr_sq;
+}
+
+// The short correctly rounded sqrt/division sequences apply (fastdiv_ok of dpm_system):
+// n_c == 2, e_c and every contact radius (a rigid system's: the vertex radius) within
+// [2^-100, 2^100].
+static bool fastdiv_range(double n_c, double e_c, double rlo, double rhi) {
+    const double lo = 0x1p-100, hi = 0x1p100;
+    return n_c == 2.0 && e_c >= lo && e_c <= hi && rlo >= lo && rhi <= hi;
+}
+
+// A system's constants.  Disks: the batch's cutoff, threshold and contact energy.  Rigid
+// bumpy systems: the rules of RigidBumpy::setNeighborSize (rigid_bumpy.cu:1278-1300)
+// with the standard multipliers -- list cutoff 1.5 d_max, threshold (0.2 r_v)^2, vertex
+// cutoff 1.5 (2 r_v), vertex-particle cutoff d_max -- and the half-box limit over every
+// contact and cutoff (fast_ok of dpm_system).
+static void sys_constants(const dpm_batch* b, int64_t s, double rmax, double rv, double e_c,
+                          int rotation, int64_t nv, int64_t nvs, long long voff, BSys& f) {
+    if (nv > 0) {
+        const double dmax = 2.0 * rmax;
+        f.kind = BK_RIGID;
+        f.rc = 1.5 * dmax;
+        f.thr2 = (0.2 * rv) * (0.2 * rv);
+        const double vc = 1.5 * 2.0 * rv;
+        f.vc2 = vc * vc;
+        f.vp2 = dmax * dmax;
+        f.lim = std::max({2.0 * rmax, f.rc, 2.0 * rv, vc, dmax});
+        f.rv = rv;
+        f.e_c = e_c;
+        f.rotation = rotation;
+        f.nv = (int)nv;
+        f.nvs = (int)nvs;
+        f.voff = voff;
+    } else {
+        f.kind = BK_DISK;
+        f.rc = b->rc;
+        f.thr2 = b->thr_sq;
+        f.lim = std::max(2.0 * rmax, b->rc);
+        f.e_c = b->e_c;
+    }
+    f.rc2 = f.rc * f.rc;
+    (void)s;
 }
 
 static BJam jam_constants(const dpm_jam_params* P) {
@@ -962,11 +1484,16 @@ static void enqueue_batch_step(dpm_batch* b) {
     const BDev& d = b->d;
     hipStream_t st = b->stream;
     dim3 slots((unsigned)((b->nslot + BLOCK - 1) / BLOCK));
+    const bool rigid = b->nvslot > 0;
+    dim3 vslots((unsigned)((b->nvslot + BLOCK - 1) / BLOCK));
+    if (rigid) hipLaunchKernelGGL(k_rb_vprep, vslots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_prep_move, slots, dim3(BLOCK), 0, st, d);
+    if (rigid) hipLaunchKernelGGL(k_rb_vmove, vslots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)std::min<long long>(d.nctiles, BC_GRID)),
                        dim3(BC_BLOCK), 0, st, d);
+    if (rigid) hipLaunchKernelGGL(k_rb_vbuild, vslots, dim3(BLOCK), 0, st, d);
     hipEvent_t ea = nullptr, eb = nullptr;
     if (b->timing) {
         ea = batch_event(b);
@@ -980,6 +1507,13 @@ static void enqueue_batch_step(dpm_batch* b) {
     if (b->timing) {
         (void)hipEventRecord(eb, st);
         b->ev_pending.push_back({ea, eb});
+    }
+    if (rigid) {
+        if (d.fastdiv)
+            hipLaunchKernelGGL(k_rb_vforce<true>, vslots, dim3(BLOCK), 0, st, d);
+        else
+            hipLaunchKernelGGL(k_rb_vforce<false>, vslots, dim3(BLOCK), 0, st, d);
+        hipLaunchKernelGGL(k_rb_reduce, slots, dim3(BLOCK), 0, st, d);
     }
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
@@ -1008,7 +1542,13 @@ using namespace dpm;
 
 extern "C" {
 
-dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
+}  // extern "C"
+
+namespace dpm {
+
+// n_vertices == nullptr: every system is a disk system.
+static dpm_batch* batch_create(int64_t n_systems, const int64_t* n_particles,
+                               const int64_t* n_vertices) {
     if (n_systems <= 0 || n_systems >= (int64_t)1 << 24 || !n_particles) {
         set_error("dpm_batch_create: bad arguments");
         return nullptr;
@@ -1024,6 +1564,30 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
     b->rmax.assign(n_systems, 0.0);
     b->rmin.assign(n_systems, 0.0);
     b->box0.assign(n_systems, 0.0);
+    b->nv.assign(n_systems, 0);
+    b->nvs.assign(n_systems, 0);
+    b->voff.assign(n_systems, 0);
+    b->rv.assign(n_systems, 0.0);
+    b->ec.assign(n_systems, 0.0);
+    b->rot.assign(n_systems, 1);
+    long long vacc = 0;
+    for (int64_t s = 0; s < n_systems && n_vertices; ++s) {
+        int64_t v = n_vertices[s];
+        if (v < 0 || v >= (int64_t)1 << 30 || (v > 0 && v < n_particles[s])) {
+            set_error("dpm_batch_create: a rigid system needs a vertex per particle at least");
+            delete b;
+            return nullptr;
+        }
+        b->nv[s] = v;
+        b->nvs[s] = (v + WAVE - 1) / WAVE * WAVE;
+        b->voff[s] = vacc;
+        vacc += b->nvs[s];
+    }
+    if (vacc >= (int64_t)1 << 31) {
+        set_error("dpm_batch_create: vertex arena exceeds 2^31 slots");
+        delete b;
+        return nullptr;
+    }
     long long acc = 0;
     std::vector<int2> tiles;
     for (int64_t s = 0; s < n_systems; ++s) {
@@ -1045,7 +1609,7 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
     }
     b->nslot = acc;
     b->ntiles = (int64_t)tiles.size();
-    bool ok = batch_alloc(b, n_systems, acc, b->ntiles) == 0;
+    bool ok = batch_alloc(b, n_systems, acc, b->ntiles, vacc) == 0;
     BDev& d = b->d;
     if (ok) {
         std::vector<int> so(acc);
@@ -1057,6 +1621,13 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
                                      hipMemcpyHostToDevice, b->stream)) == hipSuccess &&
              bsync(b, hipMemcpyAsync(d.tiles, tiles.data(), sizeof(int2) * tiles.size(),
                                      hipMemcpyHostToDevice, b->stream)) == hipSuccess;
+        if (ok && vacc > 0) {
+            std::vector<int> vso(vacc);
+            for (int64_t s = 0; s < n_systems; ++s)
+                for (long long k = 0; k < b->nvs[s]; ++k) vso[b->voff[s] + k] = (int)s;
+            ok = bsync(b, hipMemcpyAsync(d.vsys_of, vso.data(), sizeof(int) * vacc,
+                                         hipMemcpyHostToDevice, b->stream)) == hipSuccess;
+        }
         std::vector<int2> ct;
         for (int64_t s = 0; s < n_systems; ++s) cell_tiles(ct, (int)s, b->n[s]);
         b->nct = (int)ct.size();
@@ -1080,6 +1651,23 @@ dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
     return b;
 }
 
+}  // namespace dpm
+
+extern "C" {
+
+dpm_batch* dpm_batch_create(int64_t n_systems, const int64_t* n_particles) {
+    return batch_create(n_systems, n_particles, nullptr);
+}
+
+dpm_batch* dpm_batch_create_mixed(int64_t n_systems, const int64_t* n_particles,
+                                  const int64_t* n_vertices) {
+    if (!n_vertices) {
+        set_error("dpm_batch_create_mixed: null vertex counts");
+        return nullptr;
+    }
+    return batch_create(n_systems, n_particles, n_vertices);
+}
+
 void dpm_batch_destroy(dpm_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -1101,6 +1689,11 @@ int dpm_batch_set_system(dpm_batch* b, int64_t s, const double* x, const double*
                          const double* rad, const double* mass, double box_side) {
     if (s < 0 || s >= b->nsys || !(box_side > 0)) {
         set_error("dpm_batch_set_system: bad system index or box side");
+        return -1;
+    }
+    if (b->nv[s] > 0) {
+        set_error("dpm_batch_set_system: system " + std::to_string(s) +
+                  " is a rigid bumpy system (dpm_batch_set_rigid_system)");
         return -1;
     }
     int ns = b->n[s];
@@ -1137,10 +1730,96 @@ int dpm_batch_get_system(dpm_batch* b, int64_t s, double* x, double* y, double* 
         if (x) x[k] = h[k].x;
         if (y) y[k] = h[k].y;
     }
-    if (box_side) {
+    if (box_side) {  // the box as set until a run has started
         BSys S;
         BCOPY(&S, b->d.sys + s, sizeof(BSys), hipMemcpyDeviceToHost);
-        *box_side = S.Lx;
+        *box_side = b->ran ? S.Lx : b->box0[s];
+    }
+    return 0;
+}
+
+int dpm_batch_set_rigid_system(dpm_batch* b, int64_t s, const double* x, const double* y,
+                               const double* rad, const double* mass, const double* angle,
+                               const int64_t* vertices_per_particle, double vertex_radius,
+                               double e_c, int32_t rotation, double box_side) {
+    if (s < 0 || s >= b->nsys || !(box_side > 0)) {
+        set_error("dpm_batch_set_rigid_system: bad system index or box side");
+        return -1;
+    }
+    if (b->nv[s] == 0) {
+        set_error("dpm_batch_set_rigid_system: system " + std::to_string(s) +
+                  " was created without vertices");
+        return -1;
+    }
+    if (!x || !y || !rad || !mass || !angle || !vertices_per_particle ||
+        !(vertex_radius >= 0x1p-100 && vertex_radius <= 0x1p100) || !(e_c > 0)) {
+        set_error("dpm_batch_set_rigid_system: bad arguments");
+        return -1;
+    }
+    const int ns = b->n[s];
+    const long long off = b->off[s], voff = b->voff[s];
+    std::vector<double4> h(ns);
+    std::vector<double> a(ns);
+    std::vector<int> vst(ns), nvt(ns), vpart(b->nvs[s], -1);
+    double rm = 0.0;
+    long long acc = 0;
+    for (int k = 0; k < ns; ++k) {
+        if (!(rad[k] > vertex_radius) || !(mass[k] > 0) || vertices_per_particle[k] <= 0) {
+            set_error("dpm_batch_set_rigid_system: radii must exceed the vertex radius, "
+                      "masses be positive, and every particle have a vertex");
+            return -1;
+        }
+        h[k] = make_double4(x[k], y[k], rad[k], mass[k]);
+        a[k] = angle[k];
+        rm = std::max(rm, rad[k]);
+        vst[k] = (int)(voff + acc);
+        nvt[k] = (int)vertices_per_particle[k];
+        for (int64_t u = 0; u < vertices_per_particle[k] && acc + u < b->nv[s]; ++u)
+            vpart[acc + u] = (int)(off + k);
+        acc += vertices_per_particle[k];
+    }
+    if (acc != b->nv[s]) {
+        set_error("dpm_batch_set_rigid_system: vertex counts do not sum to the system's "
+                  "vertices");
+        return -1;
+    }
+    BDev& d = b->d;
+    BCOPY(d.P + off, h.data(), sizeof(double4) * ns, hipMemcpyHostToDevice);
+    BCOPY(d.ang + off, a.data(), sizeof(double) * ns, hipMemcpyHostToDevice);
+    BCOPY(d.vst + off, vst.data(), sizeof(int) * ns, hipMemcpyHostToDevice);
+    BCOPY(d.nvt + off, nvt.data(), sizeof(int) * ns, hipMemcpyHostToDevice);
+    BCOPY(d.vpart + voff, vpart.data(), sizeof(int) * vpart.size(), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_rb_init, pgrid(ns), dim3(BLOCK), 0, b->stream, d, off, ns,
+                       vertex_radius);
+    if (check_launch()) return -1;
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    b->rmax[s] = rm;
+    b->rmin[s] = rm;  // marks the system as set
+    b->rv[s] = vertex_radius;
+    b->ec[s] = e_c;
+    b->rot[s] = rotation != 0;
+    b->box0[s] = box_side;
+    b->hsys[s].Lx = b->hsys[s].Ly = box_side;
+    b->hsys[s].rmax = rm;
+    return 0;
+}
+
+int dpm_batch_get_rigid_system(dpm_batch* b, int64_t s, double* x, double* y, double* angle,
+                               double* vx, double* vy, double* box_side) {
+    if (s < 0 || s >= b->nsys || b->nv[s] == 0) {
+        set_error("dpm_batch_get_rigid_system: bad system index or not a rigid system");
+        return -1;
+    }
+    if (dpm_batch_get_system(b, s, x, y, box_side)) return -1;
+    const int ns = b->n[s];
+    if (angle) BCOPY(angle, b->d.ang + b->off[s], sizeof(double) * ns, hipMemcpyDeviceToHost);
+    if (vx || vy) {
+        std::vector<double2> h(b->nv[s]);
+        BCOPY(h.data(), b->d.vpos + b->voff[s], sizeof(double2) * h.size(), hipMemcpyDeviceToHost);
+        for (size_t k = 0; k < h.size(); ++k) {
+            if (vx) vx[k] = h[k].x;
+            if (vy) vy[k] = h[k].y;
+        }
     }
     return 0;
 }
@@ -1218,18 +1897,17 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
     }
     BDev& d = b->d;
     // common parameters (pair_params's derived numbers, dpm_jam's loop constants)
-    double rmin_all = 1e300, rmax_all = 0.0;
+    bool fastdiv = true;
     for (int64_t s = 0; s < b->nsys; ++s) {
         if (b->rmin[s] <= 0) {
             set_error("dpm_batch_jam: system " + std::to_string(s) + " has no particles set");
             return -1;
         }
-        rmin_all = std::min(rmin_all, b->rmin[s]);
-        rmax_all = std::max(rmax_all, b->rmax[s]);
+        fastdiv = fastdiv && (b->nv[s] > 0
+                                  ? fastdiv_range(b->n_c, b->ec[s], b->rv[s], b->rv[s])
+                                  : fastdiv_range(b->n_c, b->e_c, b->rmin[s], b->rmax[s]));
     }
-    const double lo = 0x1p-100, hi = 0x1p100;
-    batch_common(b, b->n_c == 2.0 && b->e_c >= lo && b->e_c <= hi && rmin_all >= lo &&
-                        rmax_all <= hi);
+    batch_common(b, fastdiv);
     const BJam J = jam_constants(P);
     // every system starts minimizeFire (decompression phase) with a fresh list
     for (int64_t s = 0; s < b->nsys; ++s) {
@@ -1241,6 +1919,8 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
         f.Lx = b->box0[s];
         f.Ly = b->box0[s];
         f.rmax = b->rmax[s];
+        sys_constants(b, s, b->rmax[s], b->rv[s], b->ec[s], b->rot[s], b->nv[s], b->nvs[s],
+                      b->voff[s], f);
         f.J = J;
         f.alpha = J.alpha0;
         f.dt = J.dt0;
@@ -1255,6 +1935,7 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
     }
     DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
                            b->stream));
+    b->ran = true;
     DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
     const int nt[2] = {(int)b->ntiles, b->nct};  // live until the synchronisation below
     DPM_HIP(hipMemcpyAsync(d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -1352,8 +2033,26 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
         return -1;
     }
     if (!b->P_init && balloc_n(b, &b->P_init, b->nslot)) return -1;
-    DPM_HIP(hipMemcpyAsync(b->P_init, b->d.P, sizeof(double4) * b->nslot,
-                           hipMemcpyDeviceToDevice, b->stream));
+    if (b->nvslot > 0 && !b->A_init &&
+        (balloc_n(b, &b->A_init, b->nslot) || balloc_n(b, &b->V_init, b->nvslot)))
+        return -1;
+    // the state as set: positions, and the rigid systems' angles and vertices
+    auto state_copy = [&](bool save) -> hipError_t {
+        auto cp = [&](void* dst, void* src, size_t n) {
+            return save ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream)
+                        : hipMemcpyAsync(src, dst, n, hipMemcpyDeviceToDevice, b->stream);
+        };
+        hipError_t e = cp(b->P_init, b->d.P, sizeof(double4) * b->nslot);
+        if (e == hipSuccess && b->nvslot > 0) e = cp(b->A_init, b->d.ang, sizeof(double) * b->nslot);
+        if (e == hipSuccess && b->nvslot > 0)
+            e = cp(b->V_init, b->d.vpos, sizeof(double2) * b->nvslot);
+        return e;
+    };
+    DPM_HIP(state_copy(true));
+    // the cutoff candidates apply to the disk systems (rigid systems keep their own)
+    double rmax_disk = 0.0;
+    for (int64_t s = 0; s < b->nsys; ++s)
+        if (b->nv[s] == 0) rmax_disk = std::max(rmax_disk, b->rmax[s]);
     const double rc0 = b->rc, thr0 = b->thr_sq;
     const int64_t ch0 = b->steps_per_graph;
     double best = 1e300, bc = rc0;
@@ -1366,7 +2065,7 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
                 return -1;
             }
             // rebuild when a particle moved half the skin (the threshold keeps its form)
-            double skin = cutoffs[i] - 2.0 * *std::max_element(b->rmax.begin(), b->rmax.end());
+            double skin = cutoffs[i] - 2.0 * rmax_disk;
             if (!(skin > 0)) continue;
             b->rc = cutoffs[i];
             b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
@@ -1376,13 +2075,11 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
                 (void)hipGraphDestroy(b->graph);
                 b->exec = nullptr;
             }
-            DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
-                                   hipMemcpyDeviceToDevice, b->stream));
+            DPM_HIP(state_copy(false));
             // one untimed launch unit first (graph capture and instantiation)
             int64_t done = 0;
             if (dpm_batch_jam(b, P, chunks[j], tmp.data(), &done)) return -1;
-            DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
-                                   hipMemcpyDeviceToDevice, b->stream));
+            DPM_HIP(state_copy(false));
             DPM_HIP(hipStreamSynchronize(b->stream));
             auto t0 = std::chrono::steady_clock::now();
             if (dpm_batch_jam(b, P, trial_steps, tmp.data(), &done)) return -1;
@@ -1403,7 +2100,7 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
         set_error("dpm_batch_autotune: no cutoff exceeds the largest contact distance");
         return -1;
     }
-    double skin = bc - 2.0 * *std::max_element(b->rmax.begin(), b->rmax.end());
+    double skin = bc - 2.0 * rmax_disk;
     b->rc = bc;
     b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
     b->steps_per_graph = bk;
@@ -1412,8 +2109,7 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
     }
-    DPM_HIP(hipMemcpyAsync(b->d.P, b->P_init, sizeof(double4) * b->nslot,
-                           hipMemcpyDeviceToDevice, b->stream));
+    DPM_HIP(state_copy(false));
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (best_cutoff) *best_cutoff = bc;
     if (best_chunk) *best_chunk = bk;
@@ -1436,20 +2132,17 @@ int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out) {
 }  // extern "C"
 
 // =========================================================================
-//                 serving loop (continuous batching of jam requests)
+//           serving loop (continuous batching of jam and minimisation requests)
 // =========================================================================
 // Requests (disk or rigid bumpy systems, each with its own jamming parameters) are
-// queued on submission and admitted while the device budget allows: disk systems take
-// a free region of the batched engine's arena and a system entry and join the running
-// batch at the next launch; rigid bumpy systems get a dpm_system of their own, jammed
-// by a worker thread on its own stream, overlapping the batch on the device.  A
-// finished disk system leaves the arena at once, so its space and entry serve the next
+// queued on submission and admitted while the device budget allows: a request takes a
+// free region of the batched engine's arena (and, for a rigid bumpy system, of its
+// vertex arena) and a system entry, and joins the running batch at the next launch.  A
+// finished system leaves the arena at once, so its space and entry serve the next
 // request.  Every result is the one the request's system gives when jammed alone.
-#include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
-#include <thread>
 
 struct dpm_server_req {
     int64_t id = 0;
@@ -1467,38 +2160,32 @@ struct dpm_server_req {
     double rside = 0.0;
     std::string err;
     int sys = -1;
-    long long off = -1, slots = 0;
+    long long off = -1, slots = 0, voff = -1, vslots = 0;
     int64_t bytes = 0;
-    std::vector<double4> staged;
 };
 
-// Locking: `mu` guards the request table, the queues and every counter; the batch
-// (arena, stream, graph) is driven only by the thread calling dpm_server_progress.  Rigid
-// bumpy requests go to a fixed pool of `workers` threads through `bumpy_queue`.
+// Locking: `mu` guards the request table, the queue and every counter; the batch (arena,
+// stream, graph) is driven only by the thread calling dpm_server_progress.
 struct dpm_server {
     dpm_batch* b = nullptr;
     int64_t budget = 0, used = 0;
-    int workers = 1, running_bumpy = 0;
     std::mutex mu;
-    std::condition_variable cv;                  // bumpy_queue not empty, or stopping
-    bool stopping = false;
     std::map<int64_t, dpm_server_req> reqs;
-    std::deque<int64_t> queue;                   // submitted, not admitted
-    std::deque<int64_t> bumpy_queue;             // admitted rigid requests for the pool
+    std::deque<int64_t> queue;                    // submitted, not admitted
     std::vector<int> free_sys;
-    std::map<long long, long long> free_ranges;  // first slot -> length
-    std::vector<int64_t> sys_req;                // system entry -> request id (0: free)
-    int64_t next_id = 1, seen_done = 0, finished = 0, disk_active = 0;
-    std::vector<std::thread> pool;
+    std::map<long long, long long> free_ranges;   // first slot -> length
+    std::map<long long, long long> vfree_ranges;  // first vertex slot -> length
+    std::vector<int64_t> sys_req;                 // system entry -> request id (0: free)
+    int64_t next_id = 1, seen_done = 0, finished = 0, active = 0;
     std::vector<int2> tiles_host;
-    bool fastdiv = true;
 };
 
 namespace dpm {
 
+// device bytes of one arena slot and of one vertex slot (arrays of batch_alloc)
 constexpr int64_t SLOT_BYTES = 32 + 32 + 16 + 16 + 16 + 8 + 4 + 4 + 32 * 4 + 8;
-
-static int64_t bumpy_bytes(int64_t n, int64_t nv) { return 1024 * n + 640 * nv; }
+constexpr int64_t RIGID_SLOT_BYTES = 4 + 4 + 8 * 5 + 16 * 4;
+constexpr int64_t VSLOT_BYTES = 4 + 4 + 4 + 32 * 4 + 16 * 3 + 8 + 8;
 
 static void serve_upload_tiles(dpm_server* sv) {
     dpm_batch* b = sv->b;
@@ -1524,132 +2211,36 @@ static void serve_upload_tiles(dpm_server* sv) {
                                   b->stream));
 }
 
-static bool take_range(dpm_server* sv, long long slots, long long* off) {
-    for (auto it = sv->free_ranges.begin(); it != sv->free_ranges.end(); ++it) {
+// first fit in a map of free ranges (first slot -> length)
+static bool take_range(std::map<long long, long long>& fr, long long slots, long long* off) {
+    for (auto it = fr.begin(); it != fr.end(); ++it) {
         if (it->second < slots) continue;
         *off = it->first;
         long long rest = it->second - slots, start = it->first + slots;
-        sv->free_ranges.erase(it);
-        if (rest) sv->free_ranges[start] = rest;
+        fr.erase(it);
+        if (rest) fr[start] = rest;
         return true;
     }
     return false;
 }
 
-static void give_range(dpm_server* sv, long long off, long long slots) {
-    auto it = sv->free_ranges.emplace(off, slots).first;
+static void give_range(std::map<long long, long long>& fr, long long off, long long slots) {
+    auto it = fr.emplace(off, slots).first;
     auto nx = std::next(it);
-    if (nx != sv->free_ranges.end() && it->first + it->second == nx->first) {
+    if (nx != fr.end() && it->first + it->second == nx->first) {
         it->second += nx->second;
-        sv->free_ranges.erase(nx);
+        fr.erase(nx);
     }
-    if (it != sv->free_ranges.begin()) {
+    if (it != fr.begin()) {
         auto pv = std::prev(it);
         if (pv->first + pv->second == it->first) {
             pv->second += it->second;
-            sv->free_ranges.erase(it);
+            fr.erase(it);
         }
     }
 }
 
-// A rigid bumpy request, jammed (or minimised) alone on its own dpm_system (same set-up
-// as a caller would make: cell list with cutoff 1.5 d_max, vertex cutoff 3 r_v,
-// threshold (0.2 r_v)^2, the rules of RigidBumpy::setNeighborSize,
-// rigid_bumpy.cu:1278-1300).  Runs on a pool thread without the server lock; the
-// request's entry stays in place while it runs (it is neither erased nor moved: the
-// table is a std::map and release refuses running requests).
-static void bumpy_job(dpm_server* sv, dpm_server_req* q) {
-    auto finish = [&](bool ok, const char* where, dpm_jam_result r, std::vector<double>&& rx,
-                      std::vector<double>&& ry, double side) {
-        std::lock_guard<std::mutex> g(sv->mu);
-        if (ok) {
-            q->res = r;
-            q->rx = std::move(rx);
-            q->ry = std::move(ry);
-            q->rside = side;
-            q->state = 2;
-        } else {
-            q->err = std::string(where) + ": " + dpm_last_error();
-            q->state = 3;
-        }
-        q->t_done = std::chrono::steady_clock::now();
-        sv->used -= q->bytes;
-        sv->running_bumpy--;
-        sv->finished++;
-    };
-    (void)hipSetDevice(sv->b->device);
-    dpm_system* s = dpm_create(DPM_BUMPY, q->n, q->nv);
-    if (!s) return finish(false, "dpm_create", {}, {}, {}, 0.0);
-    double dmax = 0.0;
-    for (double r : q->r) dmax = std::max(dmax, 2.0 * r);
-    double rc = 1.5 * dmax;
-    int64_t ncd = std::max<int64_t>(3, (int64_t)std::floor(q->side / rc));
-    double cs = q->side / ncd;
-    bool ok = !dpm_set_box(s, q->side, q->side) && !dpm_set_energy(s, q->e_c, 2.0) &&
-              !dpm_upload(s, DPM_F_X, q->x.data(), q->n) &&
-              !dpm_upload(s, DPM_F_Y, q->y.data(), q->n) &&
-              !dpm_upload(s, DPM_F_RAD, q->r.data(), q->n) &&
-              !dpm_upload(s, DPM_F_MASS, q->m.data(), q->n) &&
-              !dpm_upload(s, DPM_F_ANGLE, q->ang.data(), q->n) &&
-              !dpm_bumpy_setup(s, q->vcounts.data(), q->rv, 1) &&
-              !dpm_neighbor_config(s, DPM_NBR_CELL, rc, (0.2 * q->rv) * (0.2 * q->rv), ncd,
-                                   (0.5 * cs) * (0.5 * cs)) &&
-              !dpm_bumpy_neighbor_config(s, 1.5 * 2.0 * q->rv, dmax) && !dpm_neighbor_init(s);
-    // the inputs now live on the device (only this thread touches them while it runs)
-    for (auto* v : {&q->x, &q->y, &q->r, &q->m, &q->ang}) std::vector<double>().swap(*v);
-    std::vector<int64_t>().swap(q->vcounts);
-    dpm_jam_result r{};
-    if (ok && q->mode == DPM_REQ_MINIMIZE) {
-        dpm_fire_result fr{};
-        double pr = 0, phi = 0, pe = 0, nc = 0;
-        ok = !dpm_minimize_fire(s, &q->jam.fire, &fr) && !dpm_reduce(s, DPM_RED_PRESSURE, &pr) &&
-             !dpm_reduce(s, DPM_RED_CONTACTS, &nc) &&
-             !dpm_reduce(s, DPM_RED_PACKING_FRACTION, &phi) && !dpm_reduce(s, DPM_RED_PE, &pe);
-        r.fire_steps_total = fr.steps;
-        r.status = fr.converged ? 0 : 1;
-        r.pressure = pr;
-        r.packing_fraction = phi;
-        r.pe = pe;
-        r.box_length = q->side;
-        r.contacts = (int64_t)nc;
-    } else if (ok) {
-        ok = !dpm_jam(s, &q->jam, &r);
-    }
-    std::vector<double> rx(q->n), ry(q->n);
-    double lx = 0, ly = 0;
-    ok = ok && !dpm_download(s, DPM_F_X, rx.data(), q->n) &&
-         !dpm_download(s, DPM_F_Y, ry.data(), q->n) && !dpm_get_box(s, &lx, &ly);
-    dpm_destroy(s);
-    finish(ok, q->mode == DPM_REQ_MINIMIZE ? "rigid bumpy minimisation" : "rigid bumpy jam", r,
-           std::move(rx), std::move(ry), lx);
-}
-
-// A pool thread: takes admitted rigid requests until the server stops.
-static void bumpy_pool_thread(dpm_server* sv) {
-    for (;;) {
-        dpm_server_req* q = nullptr;
-        {
-            std::unique_lock<std::mutex> g(sv->mu);
-            sv->cv.wait(g, [sv] { return sv->stopping || !sv->bumpy_queue.empty(); });
-            if (sv->bumpy_queue.empty()) return;  // stopping, nothing left to run
-            q = &sv->reqs.at(sv->bumpy_queue.front());
-            sv->bumpy_queue.pop_front();
-        }
-        try {
-            bumpy_job(sv, q);
-        } catch (const std::exception& e) {  // e.g. bad_alloc: the request fails, not the process
-            std::lock_guard<std::mutex> g(sv->mu);
-            q->err = std::string("rigid bumpy worker: ") + e.what();
-            q->state = 3;
-            q->t_done = std::chrono::steady_clock::now();
-            sv->used -= q->bytes;
-            sv->running_bumpy--;
-            sv->finished++;
-        }
-    }
-}
-
-// Disk systems that finished since the last look leave the arena.  The stream is idle.
+// Systems that finished since the last look leave the arena.  The stream is idle.
 static int serve_collect(dpm_server* sv) {
     dpm_batch* b = sv->b;
     int fl[8];
@@ -1692,14 +2283,21 @@ static int serve_collect(dpm_server* sv) {
         // the system entry and its slots become free
         S.mode = BM_EMPTY;
         BCOPY(b->d.sys + s, &S, sizeof(BSys), hipMemcpyHostToDevice);
-        std::vector<int> neg(q.slots, -1);
+        std::vector<int> neg(std::max(q.slots, q.vslots), -1);
         BCOPY(b->d.sys_of + q.off, neg.data(), sizeof(int) * q.slots, hipMemcpyHostToDevice);
-        give_range(sv, q.off, q.slots);
+        give_range(sv->free_ranges, q.off, q.slots);
+        if (q.vslots) {
+            BCOPY(b->d.vsys_of + q.voff, neg.data(), sizeof(int) * q.vslots,
+                  hipMemcpyHostToDevice);
+            BCOPY(b->d.vpart + q.voff, neg.data(), sizeof(int) * q.vslots,
+                  hipMemcpyHostToDevice);
+            give_range(sv->vfree_ranges, q.voff, q.vslots);
+        }
         sv->free_sys.push_back(s);
         sv->sys_req[s] = 0;
         sv->used -= q.bytes;
         sv->finished++;
-        sv->disk_active--;
+        sv->active--;
         changed = true;
     }
     sv->seen_done = total;
@@ -1707,31 +2305,78 @@ static int serve_collect(dpm_server* sv) {
     return 0;
 }
 
-// Queued requests that fit join: disk systems into the arena, bumpy ones to a worker.
-static int serve_admit(dpm_server* sv) {
+// The request's particles (and a rigid system's angles, vertex layout and vertex ring)
+// into its arena ranges, then its system entry (minimizeFire, decompression phase).
+static int serve_place(dpm_server* sv, dpm_server_req& q, int s) {
     dpm_batch* b = sv->b;
+    BDev& d = b->d;
+    const long long off = q.off;
+    std::vector<double4> h(q.n);
+    double rm = 0.0;
+    for (int64_t i = 0; i < q.n; ++i) {
+        h[i] = make_double4(q.x[i], q.y[i], q.r[i], q.m[i]);
+        rm = std::max(rm, q.r[i]);
+    }
+    BCOPY(d.P + off, h.data(), sizeof(double4) * q.n, hipMemcpyHostToDevice);
+    std::vector<int> own(std::max(q.slots, q.vslots), s);
+    BCOPY(d.sys_of + off, own.data(), sizeof(int) * q.slots, hipMemcpyHostToDevice);
+    if (q.kind == DPM_BUMPY) {
+        std::vector<int> vst(q.n), nvt(q.n), vpart(q.vslots, -1);
+        long long acc = 0;
+        for (int64_t i = 0; i < q.n; ++i) {
+            vst[i] = (int)(q.voff + acc);
+            nvt[i] = (int)q.vcounts[i];
+            for (int64_t u = 0; u < q.vcounts[i]; ++u) vpart[acc + u] = (int)(off + i);
+            acc += q.vcounts[i];
+        }
+        BCOPY(d.ang + off, q.ang.data(), sizeof(double) * q.n, hipMemcpyHostToDevice);
+        BCOPY(d.vst + off, vst.data(), sizeof(int) * q.n, hipMemcpyHostToDevice);
+        BCOPY(d.nvt + off, nvt.data(), sizeof(int) * q.n, hipMemcpyHostToDevice);
+        BCOPY(d.vpart + q.voff, vpart.data(), sizeof(int) * q.vslots, hipMemcpyHostToDevice);
+        BCOPY(d.vsys_of + q.voff, own.data(), sizeof(int) * q.vslots, hipMemcpyHostToDevice);
+        hipLaunchKernelGGL(k_rb_init, pgrid(q.n), dim3(BLOCK), 0, b->stream, d, off, (int)q.n,
+                           q.rv);
+        if (check_launch()) return -1;
+        DPM_HIP(hipStreamSynchronize(b->stream));
+    }
+    BSys f{};
+    f.off = off;
+    f.n = (int)q.n;
+    f.nc = (int)((q.n + WAVE - 1) / WAVE);
+    f.Lx = f.Ly = q.side;
+    f.rmax = rm;
+    const bool rigid = q.kind == DPM_BUMPY;
+    sys_constants(b, s, rm, q.rv, q.e_c, 1, rigid ? q.nv : 0, q.vslots, q.voff, f);
+    f.J = jam_constants(&q.jam);
+    f.alpha = f.J.alpha0;
+    f.dt = f.J.dt0;
+    f.dt_max = 10.0 * f.J.dt0;
+    f.dt_min = 0.001 * f.J.dt0;
+    f.mode = BM_PROLOGUE;
+    f.rebuild = 1;
+    f.jphase = q.mode == DPM_REQ_MINIMIZE ? BJ_MIN : BJ_DECOMP;
+    f.status = 1;
+    f.L_low = f.L_high = f.L_last = -1;
+    BCOPY(d.sys + s, &f, sizeof(BSys), hipMemcpyHostToDevice);
+    return 0;
+}
+
+// Queued requests that fit join the arena.
+static int serve_admit(dpm_server* sv) {
     bool tiles_changed = false;
     for (auto it = sv->queue.begin(); it != sv->queue.end();) {
         dpm_server_req& q = sv->reqs.at(*it);
-        if (sv->used + q.bytes > sv->budget) {
+        if (sv->used + q.bytes > sv->budget || sv->free_sys.empty()) {
             ++it;
             continue;
         }
-        if (q.kind == DPM_BUMPY) {
-            if (sv->running_bumpy >= sv->workers) {
-                ++it;
-                continue;
-            }
-            q.state = 1;
-            sv->used += q.bytes;
-            sv->running_bumpy++;
-            sv->bumpy_queue.push_back(q.id);
-            sv->cv.notify_one();
-            it = sv->queue.erase(it);
+        long long off, voff = -1;
+        if (!take_range(sv->free_ranges, q.slots, &off)) {
+            ++it;
             continue;
         }
-        long long off;
-        if (sv->free_sys.empty() || !take_range(sv, q.slots, &off)) {
+        if (q.vslots && !take_range(sv->vfree_ranges, q.vslots, &voff)) {
+            give_range(sv->free_ranges, off, q.slots);
             ++it;
             continue;
         }
@@ -1739,36 +2384,15 @@ static int serve_admit(dpm_server* sv) {
         sv->free_sys.pop_back();
         q.sys = s;
         q.off = off;
+        q.voff = voff;
         q.state = 1;
         sv->used += q.bytes;
         sv->sys_req[s] = q.id;
-        sv->disk_active++;
-        // particles, slot ownership and the system entry (minimizeFire, decompression)
-        BCOPY(b->d.P + off, q.staged.data(), sizeof(double4) * q.n, hipMemcpyHostToDevice);
-        std::vector<int> own(q.slots, s);
-        BCOPY(b->d.sys_of + off, own.data(), sizeof(int) * q.slots, hipMemcpyHostToDevice);
-        BSys f{};
-        f.off = off;
-        f.n = (int)q.n;
-        f.nc = (int)((q.n + WAVE - 1) / WAVE);
-        f.Lx = f.Ly = q.side;
-        double rm = 0.0;
-        for (double r : q.r) rm = std::max(rm, r);
-        f.rmax = rm;
-        f.J = jam_constants(&q.jam);
-        f.alpha = f.J.alpha0;
-        f.dt = f.J.dt0;
-        f.dt_max = 10.0 * f.J.dt0;
-        f.dt_min = 0.001 * f.J.dt0;
-        f.mode = BM_PROLOGUE;
-        f.rebuild = 1;
-        f.jphase = q.mode == DPM_REQ_MINIMIZE ? BJ_MIN : BJ_DECOMP;
-        f.status = 1;
-        f.L_low = f.L_high = f.L_last = -1;
-        BCOPY(b->d.sys + s, &f, sizeof(BSys), hipMemcpyHostToDevice);
+        sv->active++;
+        if (serve_place(sv, q, s)) return -1;
         // the request's inputs now live in the arena
-        std::vector<double4>().swap(q.staged);
-        for (auto* v : {&q.x, &q.y, &q.r, &q.m}) std::vector<double>().swap(*v);
+        for (auto* v : {&q.x, &q.y, &q.r, &q.m, &q.ang}) std::vector<double>().swap(*v);
+        std::vector<int64_t>().swap(q.vcounts);
         tiles_changed = true;
         it = sv->queue.erase(it);
     }
@@ -1780,8 +2404,8 @@ static int serve_admit(dpm_server* sv) {
 
 extern "C" {
 
-// Every server entry point turns a C++ exception (allocation failure, thread creation)
-// into an error return: nothing is thrown across the C ABI.
+// Every server entry point turns a C++ exception (allocation failure) into an error
+// return: nothing is thrown across the C ABI.
 #define SERVER_GUARD(fail_value)                                   \
     catch (const std::exception& e) {                             \
         set_error(std::string("dpm_server: ") + e.what());         \
@@ -1792,22 +2416,11 @@ extern "C" {
         return fail_value;                                         \
     }
 
-static void server_stop(dpm_server* sv) {
-    {
-        std::lock_guard<std::mutex> g(sv->mu);
-        sv->stopping = true;
-    }
-    sv->cv.notify_all();
-    for (auto& t : sv->pool)
-        if (t.joinable()) t.join();
-    sv->pool.clear();
-}
-
 dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t budget_bytes,
-                              int32_t bumpy_workers) {
+                              int64_t vertex_slots) {
     if (max_systems <= 0 || max_systems >= (int64_t)1 << 24 || arena_slots < WAVE ||
-        arena_slots >= (int64_t)1 << 31 || budget_bytes <= 0 || bumpy_workers < 0 ||
-        bumpy_workers > 64) {
+        arena_slots >= (int64_t)1 << 31 || budget_bytes <= 0 || vertex_slots < 0 ||
+        vertex_slots >= (int64_t)1 << 31) {
         set_error("dpm_server_create: bad arguments");
         return nullptr;
     }
@@ -1820,12 +2433,13 @@ dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t 
         sv = new dpm_server();
         sv->b = new dpm_batch();
         int64_t slots = arena_slots / WAVE * WAVE;
+        int64_t vslots = vertex_slots / WAVE * WAVE;
         dpm_batch* b = sv->b;
         b->nsys = max_systems;
         b->nslot = slots;
         b->ntiles = slots / WAVE;
         std::string err;
-        if (batch_alloc(b, max_systems, slots, b->ntiles)) {
+        if (batch_alloc(b, max_systems, slots, b->ntiles, vslots)) {
             err = dpm_last_error();
         } else {
             std::vector<BSys> empty(max_systems);
@@ -1836,17 +2450,14 @@ dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t 
         }
         if (err.empty()) {
             sv->budget = budget_bytes;
-            sv->workers = bumpy_workers;
             sv->sys_req.assign(max_systems, 0);
             for (int64_t s = max_systems - 1; s >= 0; --s) sv->free_sys.push_back((int)s);
             sv->free_ranges[0] = slots;
-            batch_common(b, true);
+            if (vslots) sv->vfree_ranges[0] = vslots;
+            batch_common(b, true);  // every request's pair law is checked at submission
             if (b->use_graph && ensure_batch_graph(b, b->steps_per_graph)) err = dpm_last_error();
         }
-        if (err.empty())
-            for (int w = 0; w < bumpy_workers; ++w) sv->pool.emplace_back(bumpy_pool_thread, sv);
         if (!err.empty()) {
-            server_stop(sv);
             dpm_batch_destroy(sv->b);
             delete sv;
             set_error("dpm_server_create: " + err);
@@ -1855,7 +2466,6 @@ dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t 
         return sv;
     } catch (const std::exception& e) {
         if (sv) {
-            server_stop(sv);
             if (sv->b) dpm_batch_destroy(sv->b);
             delete sv;
         }
@@ -1864,20 +2474,8 @@ dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t 
     }
 }
 
-// Waits for the rigid requests already handed to the pool; queued ones never start.
 void dpm_server_destroy(dpm_server* sv) {
     if (!sv) return;
-    {
-        std::lock_guard<std::mutex> g(sv->mu);
-        // requests not yet taken by a pool thread are dropped
-        for (int64_t id : sv->bumpy_queue) {
-            auto& q = sv->reqs.at(id);
-            q.state = 3;
-            sv->running_bumpy--;
-        }
-        sv->bumpy_queue.clear();
-    }
-    server_stop(sv);
     dpm_batch_destroy(sv->b);
     delete sv;
 }
@@ -1906,28 +2504,26 @@ int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) try {
             set_error("dpm_server_submit: radii must lie in [2^-100, 2^100], masses > 0");
             return -1;
         }
+    q.slots = (q.n + WAVE - 1) / WAVE * WAVE;
+    q.bytes = q.slots * SLOT_BYTES;
+    if (q.slots > sv->b->nslot) {
+        set_error("dpm_server_submit: system larger than the arena");
+        return -1;
+    }
     if (q.kind == DPM_DISK) {
         if (q.e_c != sv->b->e_c) {
             set_error("dpm_server_submit: disk requests share the server's contact energy");
             return -1;
         }
-        q.slots = (q.n + WAVE - 1) / WAVE * WAVE;
-        q.bytes = q.slots * SLOT_BYTES;
-        if (q.slots > sv->b->nslot) {
-            set_error("dpm_server_submit: system larger than the arena");
-            return -1;
-        }
-        q.staged.resize(q.n);
-        for (int64_t i = 0; i < q.n; ++i)
-            q.staged[i] = make_double4(q.x[i], q.y[i], q.r[i], q.m[i]);
     } else {
-        if (!r->angle || !r->vertices_per_particle || !(r->vertex_radius > 0) ||
-            r->n_vertices <= 0) {
-            set_error("dpm_server_submit: rigid bumpy requests need angles, vertex counts and "
-                      "a vertex radius");
+        if (!r->angle || !r->vertices_per_particle ||
+            !(r->vertex_radius >= 0x1p-100 && r->vertex_radius <= 0x1p100) ||
+            r->n_vertices < q.n || !(q.e_c >= 0x1p-100 && q.e_c <= 0x1p100)) {
+            set_error("dpm_server_submit: rigid bumpy requests need angles, vertex counts, a "
+                      "vertex radius in [2^-100, 2^100] and a contact energy in that range");
             return -1;
         }
-        if (sv->workers == 0) {
+        if (sv->b->nvslot == 0) {
             set_error("dpm_server_submit: this server takes no rigid bumpy requests");
             return -1;
         }
@@ -1935,7 +2531,25 @@ int64_t dpm_server_submit(dpm_server* sv, const dpm_request* r) try {
         q.ang.assign(r->angle, r->angle + q.n);
         q.vcounts.assign(r->vertices_per_particle, r->vertices_per_particle + q.n);
         q.rv = r->vertex_radius;
-        q.bytes = bumpy_bytes(q.n, q.nv);
+        int64_t acc = 0;
+        for (int64_t i = 0; i < q.n; ++i) {
+            if (q.vcounts[i] <= 0 || !(q.r[i] > q.rv)) {
+                set_error("dpm_server_submit: every particle needs a vertex and a radius above "
+                          "the vertex radius");
+                return -1;
+            }
+            acc += q.vcounts[i];
+        }
+        if (acc != q.nv) {
+            set_error("dpm_server_submit: vertex counts do not sum to n_vertices");
+            return -1;
+        }
+        q.vslots = (q.nv + WAVE - 1) / WAVE * WAVE;
+        if (q.vslots > sv->b->nvslot) {
+            set_error("dpm_server_submit: system larger than the vertex arena");
+            return -1;
+        }
+        q.bytes += q.slots * RIGID_SLOT_BYTES + q.vslots * VSLOT_BYTES;
     }
     if (q.bytes > sv->budget) {
         set_error("dpm_server_submit: request exceeds the device budget");
@@ -1954,14 +2568,13 @@ SERVER_GUARD(-1)
 int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished_total,
                         int64_t* in_flight, int64_t* queued) try {
     dpm_batch* b = sv->b;
-    int64_t disk_active = 0, running_bumpy = 0;
+    int64_t active = 0;
     {
         std::lock_guard<std::mutex> g(sv->mu);
         if (serve_collect(sv) || serve_admit(sv)) return -1;
-        disk_active = sv->disk_active;
-        running_bumpy = sv->running_bumpy;
+        active = sv->active;
     }
-    if (disk_active > 0 && batched_steps > 0) {
+    if (active > 0 && batched_steps > 0) {
         int64_t done = 0;
         while (done < batched_steps) {
             int64_t k = std::min<int64_t>(b->steps_per_graph, batched_steps - done);
@@ -1977,12 +2590,10 @@ int dpm_server_progress(dpm_server* sv, int64_t batched_steps, int64_t* finished
         DPM_HIP(hipStreamSynchronize(b->stream));
         std::lock_guard<std::mutex> g(sv->mu);
         if (serve_collect(sv) || serve_admit(sv)) return -1;
-    } else if (running_bumpy > 0) {
-        std::this_thread::sleep_for(std::chrono::microseconds(200));
     }
     std::lock_guard<std::mutex> g(sv->mu);
     if (finished_total) *finished_total = sv->finished;
-    if (in_flight) *in_flight = sv->disk_active + sv->running_bumpy;
+    if (in_flight) *in_flight = sv->active;
     if (queued) *queued = (int64_t)sv->queue.size();
     return 0;
 }

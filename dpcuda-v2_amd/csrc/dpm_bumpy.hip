@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "dpm_kernels.hpp"
-#include "dpm_trig.hpp"
+#include "dpm_rigid.hpp"
 
 namespace dpm {
 
@@ -212,29 +212,6 @@ __global__ __launch_bounds__(BLOCK) void k_badam_update(Dev d, double alpha, dou
     }
 }
 
-// kernelMixRigidVelocitiesAndForces, rigid_bumpy/kernels.cu:89-121
-__device__ __forceinline__ void rigid_mix_one(double2& v, double2 f, double& w, double t,
-                                              double alpha) {
-    double fn = sqrt(f.x * f.x + f.y * f.y);
-    double vn = sqrt(v.x * v.x + v.y * v.y);
-    double mr = 0.0;
-    if (fn > 1e-16) {
-        mr = vn / fn * alpha;
-    } else {
-        v.x = 0.0;
-        v.y = 0.0;
-    }
-    double tn = fabs(t), wn = fabs(w), tmr = 0.0;
-    if (tn > 1e-16) {
-        tmr = wn / tn * alpha;
-    } else {
-        w = 0.0;
-        tmr = 0.0;
-    }
-    v = make_double2(v.x * (1 - alpha) + f.x * mr, v.y * (1 - alpha) + f.y * mr);
-    w = w * (1 - alpha) + t * tmr;
-}
-
 __global__ void k_bumpy_mix(Dev d, double alpha) {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= d.N) return;
@@ -429,14 +406,9 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
     double4 cq = d.P0[p];
     double a = d.angle[p], r = cq.z;
     int nv = d.nverts[p], st = d.vstart[p];
-    double inc = 2 * M_PI / nv;
-    double inner = r - rv;
     double2 c = make_double2(cq.x, cq.y);
     for (int i = 0; i < nv; ++i) {
-        double va = i * inc + a;
-        double sv, cv;
-        dpm_sincos(va, &sv, &cv);
-        d.vpos[st + i] = make_double2(c.x + inner * cv, c.y + inner * sv);
+        d.vpos[st + i] = ring_vertex(c, r, a, nv, rv, i);
         d.vpart[st + i] = (int)p;
     }
     double vm = cq.w / static_cast<double>(nv);
@@ -487,21 +459,10 @@ __device__ __forceinline__ BMove bfire_move_one(const Dev& d, int64_t i, double 
                                                 double& cd) {
     BMove m;
     double4 q = d.P0[i];
-    double2 v = d.vel[i], f = d.frc[i];
-    double a = d.angle[i], w = d.omega[i], t = d.torque[i];
-    m.old = make_double2(q.x, q.y);
-    m.dth0 = 0.0;
-    if (uphill) {  // updatePositions(-dt/2) with the decreased dt; setVelocitiesToZero
-        double bdt = -dt / 2.0;
-        double ddx = v.x * bdt, ddy = v.y * bdt;
-        m.dth0 = w * bdt;
-        q.x = q.x + ddx;
-        q.y = q.y + ddy;
-        a = a + m.dth0;
-        v = make_double2(0.0, 0.0);
-        w = 0.0;
-    }
-    m.mid = make_double2(q.x, q.y);
+    double2 v = d.vel[i];
+    double a = d.angle[i], w = d.omega[i];
+    rigid_fire_move(q, v, a, w, d.frc[i], d.torque[i], d.moi[i], dt, alpha, uphill, rotation,
+                    m.old, m.mid, m.dth0, m.dth);
     d.lpos[i] = m.old;
     d.mpos[i] = m.mid;
     if (uphill) {
@@ -509,47 +470,17 @@ __device__ __forceinline__ BMove bfire_move_one(const Dev& d, int64_t i, double 
         dpm_sincos(m.dth0, &s0, &c0);
         d.rot0[i] = make_double2(s0, c0);
     }
-    // updateVelocities(dt/2) (kernelUpdateRigidVelocities)
-    double hdt = dt / 2.0;
-    if (rotation) w += t * hdt / d.moi[i];
-    else w = 0.0;
-    v.x += f.x * hdt / q.w;
-    v.y += f.y * hdt / q.w;
-    rigid_mix_one(v, f, w, t, alpha);
-    // updatePositions(dt) (kernelUpdateRigidPositions)
-    double ddx = v.x * dt, ddy = v.y * dt, dda = w * dt;
-    m.dth = dda;
-    d.dth[i] = dda;
+    d.dth[i] = m.dth;
     double s1, c1;
-    dpm_sincos(dda, &s1, &c1);
+    dpm_sincos(m.dth, &s1, &c1);
     d.rot1[i] = make_double2(s1, c1);
-    double nx = q.x + ddx, ny = q.y + ddy;
-    m.now = make_double2(nx, ny);
-    d.P0[i] = make_double4(nx, ny, q.z, q.w);
-    d.angle[i] = a + dda;
+    m.now = make_double2(q.x, q.y);
+    d.P0[i] = q;
+    d.angle[i] = a;
     d.vel[i] = v;
     d.omega[i] = w;
-    disp_of(d, i, nx, ny, nd, cd);
+    disp_of(d, i, q.x, q.y, nd, cd);
     return m;
-}
-
-// kernelTranslateAndRotateVertices1 (rigid_bumpy/kernels.cu:152-201) for one vertex: the
-// back-step rotation by dth0 about the old centre onto the back-stepped one (when
-// uphill), then the rotation by dth about that centre onto the new one.  (s0, c0) and
-// (s1, c1) are dpm_sincos of dth0 and dth.
-__device__ __forceinline__ double2 bfire_vertex_one(double2 x, double2 old, double2 mid,
-                                                    double2 now, double s0, double c0,
-                                                    double s1, double c1, int uphill) {
-    if (uphill) {
-        double rx = x.x - old.x, ry = x.y - old.y;
-        double rxr = rx * c0 - ry * s0;
-        double ryr = rx * s0 + ry * c0;
-        x = make_double2(mid.x + rxr, mid.y + ryr);
-    }
-    double rx = x.x - mid.x, ry = x.y - mid.y;
-    double rxr = rx * c1 - ry * s1;
-    double ryr = rx * s1 + ry * c1;
-    return make_double2(now.x + rxr, now.y + ryr);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bfire_move(Dev d, NbrParams nb, int pos,

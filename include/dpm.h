@@ -228,11 +228,12 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* n_vertices_per_particle, doubl
 
 /* ---- batched jamming engine (no reference counterpart: the reference jams one system per
  * process; batching is a TODO there, todo.txt:43) ---------------------------------------------
- * Many independent disk systems of uneven N in one device arena.  dpm_batch_jam runs dpm_jam
- * (the Verlet-list form) on every system at once: one launch sequence per step advances each
- * system's own minimizeFire / controller state, and finished systems drop out.  Each
- * system's result is bit-identical to dpm_jam on a dpm_system holding the same particles
- * with the Verlet method, cutoff and threshold given here.
+ * Many independent systems of uneven N -- disks, and with dpm_batch_create_mixed rigid bumpy
+ * systems beside them -- in one device arena.  dpm_batch_jam runs dpm_jam (the Verlet-list
+ * form) on every system at once: one launch sequence per step advances each system's own
+ * minimizeFire / controller state, and finished systems drop out.  Each disk system's
+ * result is bit-identical to dpm_jam on a dpm_system holding the same particles with the
+ * Verlet method, cutoff and threshold given here.
  * dpm_batch_config: pair law (e_c, n_c), Verlet cutoff and rebuild threshold (3 max d^2 >
  * disp_thresh_sq), common to all systems.  dpm_batch_jam: out[] has n_systems entries;
  * status -1 marks a system still running after max_batched_steps; steps_done returns the
@@ -244,6 +245,26 @@ void dpm_batch_destroy(dpm_batch* b);
 int dpm_batch_set_system(dpm_batch* b, int64_t system, const double* x, const double* y,
                          const double* radius, const double* mass, double box_side);
 int dpm_batch_get_system(dpm_batch* b, int64_t system, double* x, double* y, double* box_side);
+/* Rigid bumpy systems in the arena: n_vertices[s] > 0 makes system s a rigid bumpy system of
+ * that many vertices (0: a disk system), set with dpm_batch_set_rigid_system (the geometry of
+ * dpm_bumpy_setup: vertex rings from the angles, vertex masses, moments of inertia;
+ * RigidBumpy::initializeVerticesOnParticles, rigid_bumpy.cu:476-478, :562-568).  A rigid
+ * system jams with its own contact energy e_c (n_c of dpm_batch_config) and the neighbour
+ * rules of RigidBumpy::setNeighborSize (rigid_bumpy.cu:1278-1300) with the standard
+ * multipliers: Verlet cutoff 1.5 d_max, rebuild threshold (0.2 r_v)^2, vertex cutoff
+ * 1.5 (2 r_v), vertex-particle cutoff d_max.  Its result is bit-identical to dpm_jam on a
+ * dpm_system set up with dpm_bumpy_setup and those cutoffs with the Verlet method.
+ * dpm_batch_get_rigid_system returns the centres, angles, vertex positions (n_vertices
+ * entries, particle order) and box side; any output may be null. */
+dpm_batch* dpm_batch_create_mixed(int64_t n_systems, const int64_t* n_particles,
+                                  const int64_t* n_vertices);
+int dpm_batch_set_rigid_system(dpm_batch* b, int64_t system, const double* x, const double* y,
+                               const double* radius, const double* mass, const double* angle,
+                               const int64_t* vertices_per_particle, double vertex_radius,
+                               double e_c, int32_t rotation, double box_side);
+int dpm_batch_get_rigid_system(dpm_batch* b, int64_t system, double* x, double* y,
+                               double* angle, double* vertex_x, double* vertex_y,
+                               double* box_side);
 int dpm_batch_config(dpm_batch* b, double e_c, double n_c, double cutoff, double disp_thresh_sq);
 int dpm_batch_set_option(dpm_batch* b, int option, int64_t value);
 int dpm_batch_jam(dpm_batch* b, const dpm_jam_params* p, int64_t max_batched_steps,
@@ -277,19 +298,20 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* p, int64_t trial_step
 /* ---- serving loop (no reference counterpart) ---------------------------------------------
  * Continuous batching of jamming and minimisation requests.  dpm_server_submit queues a
  * request (arrays are copied); dpm_server_progress admits queued requests while the device
- * budget allows (disk systems join the running batch of the batched engine, whose arena
- * holds arena_slots particle slots and max_systems systems; rigid bumpy systems go to a
- * fixed pool of bumpy_workers threads, each jamming one system at a time on its own
- * stream), advances the batch by batched_steps and collects finished requests.
+ * budget allows -- each joins the running batch of the batched engine, whose arena holds
+ * arena_slots particle slots, vertex_slots vertex slots (rigid bumpy requests; 0: the
+ * server takes none) and max_systems systems -- advances the batch by batched_steps and
+ * collects finished requests.
  * Disk requests share the server's pair law (e_c = 1, n_c = 2, Verlet cutoff 1.7, rebuild at
- * 3 max d^2 > 3 (0.15)^2).  A DPM_REQ_JAM request's result equals dpm_jam on the request's
- * system alone; a DPM_REQ_MINIMIZE request runs only minimizeFire with jam.fire and its
- * result is that of dpm_batch_minimize (status 0 converged, 1 not).
+ * 3 max d^2 > 3 (0.15)^2); a rigid bumpy request brings its own e_c and follows the rigid
+ * rules of dpm_batch_set_rigid_system (rotation on).  A DPM_REQ_JAM request's result
+ * equals dpm_jam on the request's system alone (Verlet method); a DPM_REQ_MINIMIZE request
+ * runs only minimizeFire with jam.fire and its result is that of dpm_batch_minimize (status
+ * 0 converged, 1 not).
  * dpm_server_result: state 0 queued, 1 running, 2 done (out, latency from submission to
  * completion, final positions and box side are filled), 3 failed.  dpm_server_release
- * forgets a done or failed request (its result memory is freed).  dpm_server_destroy waits
- * for the rigid requests the pool is running.  Entry points of one server may be called from
- * several threads; dpm_server_progress calls must not overlap. */
+ * forgets a done or failed request (its result memory is freed).  Entry points of one
+ * server may be called from several threads; dpm_server_progress calls must not overlap. */
 enum { DPM_REQ_JAM = 0, DPM_REQ_MINIMIZE = 1 };
 typedef struct dpm_server dpm_server;
 typedef struct {
@@ -304,7 +326,7 @@ typedef struct {
     dpm_jam_params jam;
 } dpm_request;
 dpm_server* dpm_server_create(int64_t max_systems, int64_t arena_slots, int64_t budget_bytes,
-                              int32_t bumpy_workers);
+                              int64_t vertex_slots);
 void dpm_server_destroy(dpm_server* s);
 int64_t dpm_server_submit(dpm_server* s, const dpm_request* r);
 int dpm_server_progress(dpm_server* s, int64_t batched_steps, int64_t* finished_total,

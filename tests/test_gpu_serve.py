@@ -1,7 +1,7 @@
 """GPU tests of the serving loop (dpm_server_*): requests that arrive while others run
 are admitted as memory frees up, and every result equals the request's system jammed
-(or minimised) alone, bit for bit -- by dpm_jam on the GPU and, for disk systems, by the
-oracle's jam (orc_jam, Verlet lists)."""
+(or minimised) alone, bit for bit -- by dpm_jam on the GPU (Verlet method; rigid bumpy
+systems with the rigid neighbour rules) and by the oracle's jam (orc_jam)."""
 import numpy as np
 import pytest
 
@@ -32,8 +32,8 @@ def alone_disk(st, prm, backend=None):
     return r, s.download(P.F_X), s.download(P.F_Y)
 
 
-def alone_bumpy(st, prm, e_c):
-    s = W.make_bumpy_system(P.System, st, e_c=e_c)
+def alone_bumpy(st, prm, e_c, backend=None):
+    s = W.make_bumpy_system(P.System, st, e_c=e_c, backend=backend, method=P.NBR_VERLET)
     r = s.jam(prm)
     return r, s.download(P.F_X), s.download(P.F_Y)
 
@@ -57,7 +57,7 @@ def test_disk_requests_arriving_during_a_batch():
     sts = [W.disk_config(n, phi, seed=s) for n, phi, s in
            ((64, 0.80, 11), (130, 0.81, 12), (90, 0.79, 13), (200, 0.80, 14), (75, 0.82, 15))]
     prms = [jam(1e-3), jam(5e-4), jam(2e-3), jam(1e-3), jam(8e-4)]
-    sv = P.Server(max_systems=8, arena_slots=1024, budget_bytes=1 << 30, bumpy_workers=0)
+    sv = P.Server(max_systems=8, arena_slots=1024, budget_bytes=1 << 30, vertex_slots=0)
     ids = [sv.submit(st.x, st.y, st.rad, st.mass, st.side, p) for st, p in
            zip(sts[:2], prms[:2])]
     for _ in range(20):  # the first two run a while, then the others arrive
@@ -82,7 +82,7 @@ def test_large_disk_requests_with_cell_grid_rows():
                        fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
                                           fmax_target=1e-8, max_steps=300))
     sts = [W.disk_config(n, 0.80, seed=s) for n, s in ((2100, 31), (300, 32), (4000, 33))]
-    sv = P.Server(max_systems=4, arena_slots=4224, budget_bytes=1 << 30, bumpy_workers=0)
+    sv = P.Server(max_systems=4, arena_slots=4224, budget_bytes=1 << 30, vertex_slots=0)
     ids = [sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm) for st in sts]
     run_until_done(sv, ids)
     for k, st in enumerate(sts):
@@ -96,7 +96,7 @@ def test_admission_waits_for_space():
     # an arena of 256 slots holds one 200-particle system at a time: requests queue
     sts = [W.disk_config(n, 0.80, seed=20 + n) for n in (200, 150, 180)]
     prm = jam(max_steps=600)
-    sv = P.Server(max_systems=4, arena_slots=256, budget_bytes=1 << 30, bumpy_workers=0)
+    sv = P.Server(max_systems=4, arena_slots=256, budget_bytes=1 << 30, vertex_slots=0)
     ids = [sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm) for st in sts]
     done, inflight, queued = sv.progress(1)
     assert inflight == 1 and queued == 2
@@ -112,18 +112,19 @@ def test_rigid_bumpy_request_next_to_disks():
     d = W.disk_config(100, 0.8, seed=4)
     prm_b = jam(1e-3, max_steps=400)
     prm_d = jam(1e-3)
-    sv = P.Server(max_systems=4, arena_slots=512, budget_bytes=1 << 30, bumpy_workers=1)
+    sv = P.Server(max_systems=4, arena_slots=512, budget_bytes=1 << 30, vertex_slots=4096)
     ib = sv.submit(b.x, b.y, b.rad, b.mass, b.side, prm_b, e_c=e_c, angle=b.angle,
                    vertices_per_particle=b.nverts, vertex_radius=b.rv)
     idd = sv.submit(d.x, d.y, d.rad, d.mass, d.side, prm_d)
     run_until_done(sv, [ib, idd])
     check(sv.result(idd), alone_disk(d, prm_d))
     check(sv.result(ib), alone_bumpy(b, prm_b, e_c))
+    check(sv.result(ib), alone_bumpy(b, prm_b, e_c, backend=oracle()))
     sv.close()
 
 
 def test_bad_requests_fail_loudly():
-    sv = P.Server(max_systems=2, arena_slots=128, budget_bytes=1 << 20, bumpy_workers=0)
+    sv = P.Server(max_systems=2, arena_slots=128, budget_bytes=1 << 20, vertex_slots=0)
     d = W.disk_config(200, 0.8, seed=1)
     with pytest.raises(P.DpmError):  # larger than the arena
         sv.submit(d.x, d.y, d.rad, d.mass, d.side, jam())
@@ -160,7 +161,8 @@ def test_config5_poisson_mix_alone_and_in_the_mix():
     import time
     reqs = W.serve_requests(10, 50.0, 8)
     assert sum(q.bumpy for q in reqs) == 3
-    sv = P.Server(max_systems=64, arena_slots=1 << 15, budget_bytes=1 << 32, bumpy_workers=2)
+    sv = P.Server(max_systems=64, arena_slots=1 << 15, budget_bytes=1 << 32,
+                  vertex_slots=1 << 19)
     ids = [None] * len(reqs)
     t0, nxt = time.perf_counter(), 0
     while nxt < len(reqs) or not all(sv.result(i)["state"] == 2 for i in ids):
@@ -191,13 +193,13 @@ def test_config5_poisson_mix_alone_and_in_the_mix():
 
 
 def test_minimization_requests():
-    """DPM_REQ_MINIMIZE: minimizeFire alone (disk systems in the batch, rigid bumpy ones on
-    a worker); the result equals dpm_minimize_fire on the system alone followed by the
-    reductions, and for disks the oracle's; a request cannot be released while it runs."""
+    """DPM_REQ_MINIMIZE: minimizeFire alone (disk and rigid bumpy systems in the batch); the
+    result equals dpm_minimize_fire on the system alone followed by the reductions, and the
+    oracle's; a request cannot be released while it runs."""
     fire_prm = relaxed(1e-3, max_steps=3000)
     d1, d2 = W.disk_config(300, 0.86, seed=61), W.disk_config(1500, 0.84, seed=62)
     b = W.bumpy_config(40, 0.75, seed=63)
-    sv = P.Server(max_systems=8, arena_slots=4096, budget_bytes=1 << 30, bumpy_workers=1)
+    sv = P.Server(max_systems=8, arena_slots=4096, budget_bytes=1 << 30, vertex_slots=4096)
     ids = [submit_request(sv, st, fire_prm, mode=P.REQ_MINIMIZE) for st in (d1, d2, b)]
     with pytest.raises(P.DpmError):
         sv.release(ids[0])  # queued
@@ -207,10 +209,10 @@ def test_minimization_requests():
         r = got["result"]
         say("minimisation request", st.n, r.status, r.fire_steps_total)
         bumpy = hasattr(st, "angle")
-        backends = (None,) if bumpy else (None, oracle())
-        for be in backends:
+        for be in (None, oracle()):
             if bumpy:
-                s = W.make_bumpy_system(P.System, st, backend=be, e_c=W.bumpy_contact_scale(st))
+                s = W.make_bumpy_system(P.System, st, backend=be, e_c=W.bumpy_contact_scale(st),
+                                        method=P.NBR_VERLET)
             else:
                 s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2),
                                        backend=be)
