@@ -454,7 +454,12 @@ def oracle_jam_sample(P, W, jobs, budget_s):
     be = P.Backend(oracle_so, "orc_", gpu=False)
     done, t = 0, 0.0
     for st, prm in jobs:
-        s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2), backend=be)
+        if hasattr(st, "angle"):
+            s = W.make_bumpy_system(P.System, st, backend=be, e_c=W.bumpy_contact_scale(st),
+                                    method=P.NBR_VERLET)
+        else:
+            s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2),
+                                   backend=be)
         c0 = time.perf_counter()
         s.jam(prm)
         t += time.perf_counter() - c0
@@ -473,19 +478,29 @@ def jam_batch(args, P, W):
     particles, from a cell grid above; the same result bits), both on the same systems.
     value = systems jammed (status 0) per second by the batched engine."""
     import numpy as np
-    sts = W.batched_systems(args.systems, args.seed, args.batch_n_lo, args.batch_n_hi,
-                            args.jam_phi0)
+    rigid = args.batch_kind == "rigid"
+    if rigid:  # rigid bumpy systems (20/28 vertices), the rigid neighbour rules
+        ns0 = W.batched_sizes(args.systems, args.seed, args.batch_n_lo, args.batch_n_hi)
+        sts = [W.bumpy_config(int(n), args.jam_phi0, seed=args.seed + 1 + k)
+               for k, n in enumerate(ns0)]
+    else:
+        sts = W.batched_systems(args.systems, args.seed, args.batch_n_lo, args.batch_n_hi,
+                                args.jam_phi0)
     ns = np.array([st.n for st in sts])
     prm = _jam_prm(P, args)
     rc, thr_sq = 1.7, 3.0 * 0.15 * 0.15
-    print(f"[bench] batched jamming: {len(sts)} systems, {int(ns.sum())} particles",
-          file=sys.stderr, flush=True)
-    b = P.Batch([st.n for st in sts])
+    print(f"[bench] batched jamming: {len(sts)} {args.batch_kind} systems, {int(ns.sum())} "
+          f"particles", file=sys.stderr, flush=True)
+    b = P.Batch([st.n for st in sts], [st.nv for st in sts] if rigid else None)
     b.config(1.0, 2.0, rc, thr_sq)
     b.set_option(P.OPT_CHUNK, args.batch_chunk)
     b.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
     for k, st in enumerate(sts):
-        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+        if rigid:
+            b.set_rigid_system(k, st.x, st.y, st.rad, st.mass, st.angle, st.nverts, st.rv,
+                               st.side, e_c=W.bumpy_contact_scale(st))
+        else:
+            b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
     tuned = None
     if args.autotune:
         cut, chunk, ms = b.autotune(prm, trial_steps=args.autotune_steps)
@@ -511,7 +526,11 @@ def jam_batch(args, P, W):
     t_plain, same = [], True
     for k in pick:
         st = sts[int(k)]
-        s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
+        if rigid:
+            s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st),
+                                    method=P.NBR_VERLET)
+        else:
+            s = W.make_disk_system(P.System, st, W.neighbor_spec(st.side, method=2))
         s.set_option(P.OPT_GRAPH, 0 if args.no_graph else 1)
         s.synchronize()
         c0 = time.perf_counter()
@@ -524,9 +543,11 @@ def jam_batch(args, P, W):
         s.close()
     plain_jams_s = 1.0 / (sum(t_plain) / len(t_plain))
     value = jammed / elapsed
-    print("[bench] force-pass roofline (event-timed batched steps)", file=sys.stderr,
-          flush=True)
-    roof = batch_force_roofline(P, sts, args.roofline_steps)
+    roof = None
+    if not rigid:
+        print("[bench] force-pass roofline (event-timed batched steps)", file=sys.stderr,
+              flush=True)
+        roof = batch_force_roofline(P, sts, args.roofline_steps)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
@@ -546,12 +567,14 @@ def jam_batch(args, P, W):
         "data": f"synthetic (seeded uniform positions; N log-uniform in [{args.batch_n_lo}, "
                 f"{args.batch_n_hi}])",
         "config": {
-            "workload": f"{len(sts)} bidisperse disk systems (1.0/1.4, 50/50), N log-uniform "
+            "workload": f"{len(sts)} bidisperse "
+                        f"{'rigid bumpy (20/28 vertices)' if rigid else 'disk'} systems "
+                        f"(1.0/1.4, 50/50), N log-uniform "
                         f"in [{args.batch_n_lo}, {args.batch_n_hi}], phi0={args.jam_phi0}, "
                         f"compress by strain {args.jam_strain} with FIRE to |F|max<"
                         f"{args.jam_fmax} (at most {args.jam_fire_steps} steps each), pressure "
                         f"bisection to {args.jam_pressure} (window {args.jam_window}), Verlet "
-                        f"rc=1.7",
+                        f"rc={'1.5 d_max' if rigid else '1.7'}",
             "systems": len(sts),
             "particles": int(ns.sum()),
             "jammed_status0": jammed,
@@ -601,12 +624,17 @@ def serve(args, P, W):
     t0 = time.perf_counter()
     nxt = 0
     done = 0
+    next_note = 10.0
     while done < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
             ids[nxt] = submit(sv, reqs[nxt])
             nxt += 1
         done, inflight, queued = sv.progress(args.batch_chunk)
+        if now >= next_note:  # a progress line every ten seconds
+            print(f"[bench] {now:.0f} s: {done} done, {inflight} running, {queued} queued",
+                  file=sys.stderr, flush=True)
+            next_note += 10.0
         if inflight == 0 and queued == 0 and nxt < len(reqs):
             time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (time.perf_counter() - t0))))
     elapsed = time.perf_counter() - t0
@@ -623,6 +651,8 @@ def serve(args, P, W):
     for k in range(k_plain):
         q = reqs[k]
         st, prm = q.state, _jam_prm(P, args, q.pressure)
+        print(f"[bench] plain request {k}: {'rigid' if q.bumpy else 'disk'} N={st.n}",
+              file=sys.stderr, flush=True)
         if q.bumpy:
             s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st),
                                     method=P.NBR_VERLET)
@@ -729,6 +759,7 @@ def main():
     ap.add_argument("--jam-fire-steps", type=int, default=100000)
     ap.add_argument("--jam-csteps", type=int, default=100000)
     ap.add_argument("--batch-chunk", type=int, default=32)
+    ap.add_argument("--batch-kind", choices=("disk", "rigid"), default="disk")
     ap.add_argument("--batch-n-lo", type=int, default=64)
     ap.add_argument("--batch-n-hi", type=int, default=2048)
     ap.add_argument("--batch-max-steps", type=int, default=10**9)

@@ -25,7 +25,7 @@
 // A rigid system also owns vertex slots [voff_s, voff_s + nvs_s) (voff_s a multiple of
 // 64; a padding vertex has vpart = -1), its particles' vertices contiguous in particle
 // order (vst = first vertex slot, nvt = count):
-//   vpos, vsnap, vfrc : double2     vtq, vpe : double     vpart, vsys_of, vnum : int32
+//   vpos, vsnap : double2                 vpart, vsys_of, vnum : int32
 //   vnbr      : int32 ELL, column-major vnbr[k*V + vslot] (V = vertex slots)
 //   ang, asnap, omg, tq, moi : double per particle slot; lpos, mpos, rot0, rot1 : the
 //   centres and rotations of the step's moves, read by the vertex pass
@@ -47,10 +47,11 @@
 //   k_rb_vprep  per vertex: the controller's snapshot / scaling on the vertices (before
 //               k_b_prep_move moves the centres they follow)
 //   k_rb_vmove  per vertex: the rotations of the step's rigid moves
-//   k_rb_vbuild per vertex: vertex rows from the particle rows (after the builds)
-//   k_rb_vforce per vertex: force, torque and energy from the vertex row
-//   k_rb_reduce per particle: the sums over the particle's vertices in vertex order, the
-//               closing kick and the chunk partials
+//   k_rb_vbuild per particle (a half-wave over its vertices): vertex rows from the
+//               particle rows (after the builds)
+//   k_rb_force  per 64-particle chunk: vertex forces, torques and energies from the vertex
+//               rows into LDS, their sums over each particle's vertices in vertex order,
+//               the closing kick and the chunk partials
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -139,8 +140,7 @@ struct BDev {
     long long nvslot;
     int vcap, pad2;
     int *vsys_of, *vpart, *vnum, *vnbr, *vst, *nvt;
-    double2 *vpos, *vsnap, *vfrc;
-    double *vtq, *vpe;
+    double2 *vpos, *vsnap;
     double *ang, *asnap, *omg, *tq, *moi;
     double2 *lpos, *mpos, *rot0, *rot1;
 };
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
     if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
-    if (S.kind == BK_RIGID) return;  // k_rb_vforce + k_rb_reduce
+    if (S.kind == BK_RIGID) return;  // k_rb_force
     const int kick = S.mode == BM_FIRE;
     const PairParams p = bparams(b, S);
     const bool live = i - S.off < S.n;
@@ -698,136 +698,170 @@ __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
 // kernelUpdateVertexNeighborList (rigid_bumpy/kernels.cu:870-922) for the rigid systems
 // whose particle rows were rebuilt in this step: for vertex v of particle p, over p's
 // particle row in order, the particles q whose centre lies within vp of v, then q's
-// vertices in order within vc of v (k_bumpy_vertex_build's row order).
+// vertices in order within vc of v (k_bumpy_vertex_build's row order).  A half-wave per
+// particle slot, its lanes over the particle's vertices: the row entries and q's vertex
+// positions are the same address for the 32 lanes (one load serves them all).  Two
+// consecutive slots share a wave, and a 64-slot chunk belongs to one system, so every
+// early return is uniform over the wave.  q's vertices are scanned only when v lies
+// within ring_reach of q's centre (vertex_ring_near): the rows are unchanged.
+constexpr int RB_HALF = 32;
 __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
-    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (v >= b.nvslot) return;
-    const int s = b.vsys_of[v];
-    if (s < 0) return;  // free vertex slots (64-aligned: whole waves)
+    const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const long long i = t / RB_HALF;
+    const int lane = (int)(t % RB_HALF);
+    if (i >= b.nslot) return;
+    const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
-    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
+    if (S.kind != BK_RIGID || !S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL ||
+        S.mode == BM_EMPTY)
+        return;
     const PairParams pp = bparams(b, S);
-    const int p = b.vpart[v];
-    int cnt = 0;
-    if (p >= 0) {
-        const double2 xv = b.vpos[v];
-        const int nn = min(b.nnum[p], b.cap);
-        for (int k = 0; k < nn; ++k) {
-            const int q = b.nbr[(long long)k * b.nslot + p];
-            const double4 cq = b.P[q];
-            double dx = pbc(xv.x, cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
-            double dy = pbc(xv.y, cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
-            if (!(dx * dx + dy * dy < S.vp2)) continue;
-            const int f0 = b.vst[q], nq = b.nvt[q];
-            for (int u = f0; u < f0 + nq; ++u) {
-                const double2 xu = b.vpos[u];
-                dx = pbc(xv.x, xu.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
-                dy = pbc(xv.y, xu.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
-                if (dx * dx + dy * dy < S.vc2) {
-                    if (cnt < b.vcap) b.vnbr[(long long)cnt * b.nvslot + v] = u;
-                    cnt++;
+    const double vc = sqrt(S.vc2);
+    int mx = 0;
+    if (i - S.off < S.n) {
+        const int st = b.vst[i], nv = b.nvt[i];
+        const int nn = min(b.nnum[i], b.cap);
+        for (int a = lane; a < nv; a += RB_HALF) {
+            const long long v = st + a;
+            const double2 xv = b.vpos[v];
+            int cnt = 0;
+            for (int k = 0; k < nn; ++k) {
+                const int q = b.nbr[(long long)k * b.nslot + i];
+                const double4 cq = b.P[q];
+                double dx = pbc(xv.x, cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+                double dy = pbc(xv.y, cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                const double d2 = dx * dx + dy * dy;
+                if (!(d2 < S.vp2) || !vertex_ring_near(d2, cq.z, S.rv, vc)) continue;
+                const int f0 = b.vst[q], nq = b.nvt[q];
+                for (int u = f0; u < f0 + nq; ++u) {
+                    const double2 xu = b.vpos[u];
+                    dx = pbc(xv.x, xu.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+                    dy = pbc(xv.y, xu.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                    if (dx * dx + dy * dy < S.vc2) {
+                        if (cnt < b.vcap) b.vnbr[(long long)cnt * b.nvslot + v] = u;
+                        cnt++;
+                    }
                 }
             }
+            b.vnum[v] = cnt;
+            mx = max(mx, cnt);
         }
-        b.vnum[v] = cnt;
     }
-    const int mx = wave_max_i(cnt);
+    mx = wave_max_i(mx);
     if ((threadIdx.x % WAVE) == 0) {
         atomicMax(&b.flags[6], mx);
         if (mx > b.vcap) atomicMax(&b.flags[0], 1);
     }
 }
 
-// kernelCalcRigidBumpyForces1 (rigid_bumpy/kernels.cu:366-417) per vertex slot: force,
-// torque about the centre (calcTorque, include/particles/base/kernels.cuh:237-241) and
-// energy from the vertex row, in row order (k_bfire_forces's arithmetic).
+// The force phase of a rigid FIRE step for one 64-particle chunk per workgroup:
+// kernelCalcRigidBumpyForces1 (rigid_bumpy/kernels.cu:366-417) for every vertex of the
+// chunk's particles -- force, torque about the centre (calcTorque,
+// include/particles/base/kernels.cuh:237-241) and energy from the vertex row, in row
+// order -- into LDS, a piece of RB_PIECE vertices at a time; their per-particle sums in
+// vertex order (kernelCalcRigidBumpyParticleForces1, :419-442; four lanes per particle,
+// one per quantity, the running sums carried across pieces); then the closing
+// updateVelocities(dt/2) of a FIRE step (kernelUpdateRigidVelocities) and the chunk
+// partials of F.v, tau.w, PE and max(F^2, tau^2) (k_bfire_forces + k_bfire_partials'
+// arithmetic).  The per-vertex values never leave LDS.
+constexpr int RB_PIECE = 1024;
+constexpr int RB_FT = 512;  // threads per chunk: the vertex phase has ~25 vertices per slot
 template <bool FASTDIV>
-__global__ __launch_bounds__(BLOCK) void k_rb_vforce(BDev b) {
-    const long long v = (long long)xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (v >= b.nvslot) return;
-    const int s = b.vsys_of[v];
-    if (s < 0) return;  // free vertex slots (64-aligned: whole waves)
-    const BSys& S = b.sys[s];
-    if (S.mode != BM_PROLOGUE && S.mode != BM_FIRE) return;
-    const PairParams p = bparams(b, S);
-    const int pv = b.vpart[v];
-    const bool live = pv >= 0;
-    const long long vl = live ? v : S.voff;
-    const double rv = S.rv;
-    const double2 xv = b.vpos[vl];
-    const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
-    const int nn = live ? min(b.vnum[vl], b.vcap) : 0;
-    const int* col = b.vnbr + vl;
-    // two neighbours per batch, the next batch's indices loaded ahead; every ELL slot
-    // holds a valid vertex slot (zero-filled at allocation, only written with slots), and
-    // a slot past the row end adds +0.0, which leaves the sums unchanged
-    const int last = b.vcap - 1;
-    int un[2] = {col[0], col[(long long)min(1, last) * b.nvslot]};
-    double ax = 0.0, ay = 0.0, e = 0.0;
-    for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
-        const double2 x2[2] = {b.vpos[un[0]], b.vpos[un[1]]};
-        if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
-            un[0] = col[(long long)min(k + 2, last) * b.nvslot];
-            un[1] = col[(long long)min(k + 3, last) * b.nvslot];
-        }
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const bool valid = k + t < nn;
-            const double4 q = make_double4(x2[t].x, x2[t].y, rv, 0.0);
-            double fx = 0.0, fy = 0.0, ie = 0.0;
-            if constexpr (FASTDIV) {
-                if (p.fast) ie = pair_hot<true>(qv, q, valid, p, fx, fy);
-                else ie = pair_hot<true, true>(qv, q, valid, p, fx, fy);
-            } else if (valid) {
-                ie = pair_law(xv.x, xv.y, rv, q.x, q.y, rv, p, fx, fy);
-            }
-            ax += fx;
-            ay += fy;
-            e += ie;
-        }
-    }
-    if (live) {
-        const double4 cq = b.P[pv];
-        const double dx = xv.x - cq.x, dy = xv.y - cq.y;
-        const double t = ay * dx - ax * dy;
-        b.vfrc[v] = make_double2(0.0 + ax, 0.0 + ay);
-        b.vtq[v] = 0.0 + t;
-        b.vpe[v] = 0.0 + e;
-    }
-}
-
-// kernelCalcRigidBumpyParticleForces1 (rigid_bumpy/kernels.cu:419-442) per rigid
-// particle slot: force, torque and energy summed over the particle's vertices in vertex
-// order, the closing updateVelocities(dt/2) of a FIRE step (kernelUpdateRigidVelocities),
-// and the chunk partials of F.v, tau.w, PE and max(F^2, tau^2) (k_bfire_partials).
-__global__ __launch_bounds__(BLOCK) void k_rb_reduce(BDev b) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= b.nslot) return;
-    const int s = b.sys_of[i];
-    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
+__global__ __launch_bounds__(RB_FT) void k_rb_force(BDev b) {
+    const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * WAVE;  // chunk's slot
+    if (c0 >= b.nslot) return;
+    const int s = b.sys_of[c0];
+    if (s < 0) return;  // a free chunk (64-aligned)
     const BSys& S = b.sys[s];
     if (S.kind != BK_RIGID || (S.mode != BM_PROLOGUE && S.mode != BM_FIRE)) return;
-    const bool kick = S.mode == BM_FIRE;
-    const bool live = i - S.off < S.n;
-    double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
-    if (live) {
-        const int st = b.vst[i], nv = b.nvt[i];
-        double ax = 0.0, ay = 0.0, at = 0.0, e = 0.0;
-        for (int k = 0; k < nv; ++k) {
-            const double2 f = b.vfrc[st + k];
-            ax += f.x;
-            ay += f.y;
-            at += b.vtq[st + k];
-            e += b.vpe[st + k];
+    const PairParams p = bparams(b, S);
+    const long long end = min(c0 + WAVE, S.off + S.n);  // live slots [c0, end)
+    if (end <= c0) return;
+    __shared__ double sv[4][RB_PIECE];
+    __shared__ double sres[4][WAVE];
+    const int tid = threadIdx.x;
+    const int j = tid >> 2, comp = tid & 3;  // the summing lane: particle c0 + j, quantity
+    const bool sums = tid < 4 * WAVE && c0 + j < end;
+    const int jst = sums ? b.vst[c0 + j] : 0, jnv = sums ? b.nvt[c0 + j] : 0;
+    const long long lo = b.vst[c0], hi = (long long)b.vst[end - 1] + b.nvt[end - 1];
+    const double rv = S.rv;
+    const int last = b.vcap - 1;
+    double acc = 0.0;
+    for (long long base = lo; base < hi; base += RB_PIECE) {
+        const long long top = min(base + RB_PIECE, hi);
+        for (long long v = base + tid; v < top; v += RB_FT) {
+            const double2 xv = b.vpos[v];
+            const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
+            const int nn = min(b.vnum[v], b.vcap);
+            const int* col = b.vnbr + v;
+            double ax = 0.0, ay = 0.0, e = 0.0;
+            if constexpr (FASTDIV) {
+                // two neighbours per batch, the next batch's indices loaded ahead; every
+                // ELL slot holds a valid vertex slot (zero-filled at allocation, only
+                // written with slots), and a slot past the row end adds +0.0
+                int u0 = col[0], u1 = col[(long long)min(1, last) * b.nvslot];
+                for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
+                    const double2 x0 = b.vpos[u0], x1 = b.vpos[u1];
+                    if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
+                        u0 = col[(long long)min(k + 2, last) * b.nvslot];
+                        u1 = col[(long long)min(k + 3, last) * b.nvslot];
+                    }
+                    double fx, fy, ie;
+                    const double4 q0 = make_double4(x0.x, x0.y, rv, 0.0);
+                    const double4 q1 = make_double4(x1.x, x1.y, rv, 0.0);
+                    if (p.fast) ie = pair_hot<true>(qv, q0, k < nn, p, fx, fy);
+                    else ie = pair_hot<true, true>(qv, q0, k < nn, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                    e += ie;
+                    if (p.fast) ie = pair_hot<true>(qv, q1, k + 1 < nn, p, fx, fy);
+                    else ie = pair_hot<true, true>(qv, q1, k + 1 < nn, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                    e += ie;
+                }
+            } else {
+                for (int k = 0; k < nn; ++k) {
+                    const double2 xu = b.vpos[col[(long long)k * b.nvslot]];
+                    double fx, fy;
+                    const double ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                    e += ie;
+                }
+            }
+            const double4 cq = b.P[b.vpart[v]];
+            const double dx = xv.x - cq.x, dy = xv.y - cq.y;
+            const double t = ay * dx - ax * dy;
+            const int l = (int)(v - base);
+            sv[0][l] = 0.0 + ax;
+            sv[1][l] = 0.0 + ay;
+            sv[2][l] = 0.0 + t;
+            sv[3][l] = 0.0 + e;
         }
-        const double fx = 0.0 + ax, fy = 0.0 + ay, t = 0.0 + at;
-        pe = 0.0 + e;
+        __syncthreads();
+        if (sums) {  // this particle's vertices inside the piece, in vertex order
+            const long long a0 = max((long long)jst, base), a1 = min((long long)jst + jnv, top);
+            const double* src = sv[comp];
+            for (long long a = a0; a < a1; ++a) acc += src[a - base];
+        }
+        __syncthreads();  // the next piece reuses the LDS
+    }
+    if (sums) sres[comp][j] = 0.0 + acc;
+    __syncthreads();
+    if (tid >= WAVE) return;  // wave 0: one lane per particle slot of the chunk
+    const long long i = c0 + tid;
+    double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
+    if (i < end) {
+        const double fx = sres[0][tid], fy = sres[1][tid], t = sres[2][tid];
+        pe = sres[3][tid];
         b.frc[i] = make_double2(fx, fy);
         b.tq[i] = t;
         b.pe[i] = pe;
         double2 v = b.vel[i];
         double w = b.omg[i];
-        if (kick) {
+        if (S.mode == BM_FIRE) {
             const double hdt = S.dt / 2.0;
             if (S.rotation) w += t * hdt / b.moi[i];
             else w = 0.0;
@@ -847,8 +881,8 @@ __global__ __launch_bounds__(BLOCK) void k_rb_reduce(BDev b) {
     const double s2 = wave_tree_sum(pe);
     const double s3 = wave_tree_sum(ptw);
     const double m = wave_max(f2);
-    if ((threadIdx.x % WAVE) == 0) {
-        const long long ch = i / WAVE;
+    if (tid == 0) {
+        const long long ch = c0 / WAVE;
         bpart(b, BP_PX)[ch] = s0;
         bpart(b, BP_PY)[ch] = s1;
         bpart(b, BP_PE)[ch] = s2;
@@ -1006,8 +1040,8 @@ __device__ __forceinline__ void b_eval_disk(const BDev& b, const BSys& S, const 
 // The rigid forms: setVelocitiesToZero (v and omega), kernelCalcRigidBumpyStressTensor
 // (rigid_bumpy/kernels.cu:705-789: the vertex kinetic terms, here of zero velocity, and
 // the vertex-pair virial) and kernelCountRigidBumpyContacts (rigid_bumpy/kernels.cu:
-// 924-997: per neighbour q, the first of i's vertices touching a vertex of q counts),
-// the arithmetic of k_bumpy_stress and k_bumpy_contacts.
+// 924-997: a neighbour q counts once when a vertex of i touches a vertex of q), the
+// arithmetic of k_bumpy_stress and the count of k_bumpy_contacts.
 __device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const PairParams& p,
                                         long long i, double& sxx, double& syy, double& z) {
     b.vel[i] = make_double2(0.0, 0.0);
@@ -1039,31 +1073,32 @@ __device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const Pair
     const double area = S.Lx * S.Ly;
     sxx = axx / area;
     syy = ayy / area;
+    // a neighbour q of i counts when some vertex of i touches some vertex of q; every
+    // vertex-row entry belongs to a particle of i's row (the rows are built from it), so
+    // the touching entries mark row positions in a mask and the count is its population
     const double vrs = 2 * rv;
-    int nc = 0;
     const int nn = min(b.nnum[i], b.cap);
-    for (int k = 0; k < nn; ++k) {
-        const int q = b.nbr[(long long)k * b.nslot + i];
-        bool is_contact = false;
-        for (int a = 0; a < nv; ++a) {
-            const int v = st + a;
-            const double2 xv = b.vpos[v];
-            const int vn = min(b.vnum[v], b.vcap);
-            for (int k2 = 0; k2 < vn; ++k2) {
-                const int u = b.vnbr[(long long)k2 * b.nvslot + v];
-                if (b.vpart[u] != q) continue;
-                const double2 xu = b.vpos[u];
-                const double dx = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
-                const double dy = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
-                const double dist = sqrt(dx * dx + dy * dy);
-                if (dist < vrs) {
-                    if (!is_contact) nc++;
-                    is_contact = true;
+    unsigned long long seen = 0;  // row positions (the row capacity is at most 64)
+    for (int a = 0; a < nv; ++a) {
+        const int v = st + a;
+        const double2 xv = b.vpos[v];
+        const int vn = min(b.vnum[v], b.vcap);
+        for (int k2 = 0; k2 < vn; ++k2) {
+            const int u = b.vnbr[(long long)k2 * b.nvslot + v];
+            const double2 xu = b.vpos[u];
+            const double dx = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+            const double dy = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
+            const double dist = sqrt(dx * dx + dy * dy);
+            if (!(dist < vrs)) continue;
+            const int q = b.vpart[u];
+            for (int k = 0; k < nn; ++k)
+                if (b.nbr[(long long)k * b.nslot + i] == q) {
+                    seen |= 1ull << k;
+                    break;
                 }
-            }
-            if (is_contact) break;
         }
     }
+    const int nc = __popcll(seen);
     z = (double)nc;
 }
 
@@ -1346,7 +1381,7 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
     // parts = ceil(n / BC_PER) <= n / BC_PER + n / BC_MIN_N for n >= BC_MIN_N, and the
     // systems share the nslot slots
     d.nctiles = nslot / BC_PER + nslot / BC_MIN_N + 1;
-    d.cap = 32;
+    d.cap = 32;  // at most 64: rb_eval marks row positions in a 64-bit mask
     DPM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     if (balloc_n(b, &d.sys, nsys) || balloc_n(b, &d.sys_of, nslot) || balloc_n(b, &d.P, nslot) ||
         balloc_n(b, &d.snap, nslot) || balloc_n(b, &d.vel, nslot) || balloc_n(b, &d.frc, nslot) ||
@@ -1364,8 +1399,7 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
         if (balloc_n(b, &d.vsys_of, nvslot) || balloc_n(b, &d.vpart, nvslot) ||
             balloc_n(b, &d.vnum, nvslot) || balloc_n(b, &d.vnbr, nvslot * d.vcap) ||
             balloc_n(b, &d.vpos, nvslot) || balloc_n(b, &d.vsnap, nvslot) ||
-            balloc_n(b, &d.vfrc, nvslot) || balloc_n(b, &d.vtq, nvslot) ||
-            balloc_n(b, &d.vpe, nvslot) || balloc_n(b, &d.vst, nslot) ||
+            balloc_n(b, &d.vst, nslot) ||
             balloc_n(b, &d.nvt, nslot) || balloc_n(b, &d.ang, nslot) ||
             balloc_n(b, &d.asnap, nslot) || balloc_n(b, &d.omg, nslot) ||
             balloc_n(b, &d.tq, nslot) || balloc_n(b, &d.moi, nslot) ||
@@ -1493,7 +1527,9 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)std::min<long long>(d.nctiles, BC_GRID)),
                        dim3(BC_BLOCK), 0, st, d);
-    if (rigid) hipLaunchKernelGGL(k_rb_vbuild, vslots, dim3(BLOCK), 0, st, d);
+    if (rigid)
+        hipLaunchKernelGGL(k_rb_vbuild, dim3((unsigned)(b->nslot * RB_HALF / BLOCK)), dim3(BLOCK),
+                           0, st, d);
     hipEvent_t ea = nullptr, eb = nullptr;
     if (b->timing) {
         ea = batch_event(b);
@@ -1509,11 +1545,11 @@ static void enqueue_batch_step(dpm_batch* b) {
         b->ev_pending.push_back({ea, eb});
     }
     if (rigid) {
+        dim3 chunks((unsigned)(b->nslot / WAVE));
         if (d.fastdiv)
-            hipLaunchKernelGGL(k_rb_vforce<true>, vslots, dim3(BLOCK), 0, st, d);
+            hipLaunchKernelGGL(k_rb_force<true>, chunks, dim3(RB_FT), 0, st, d);
         else
-            hipLaunchKernelGGL(k_rb_vforce<false>, vslots, dim3(BLOCK), 0, st, d);
-        hipLaunchKernelGGL(k_rb_reduce, slots, dim3(BLOCK), 0, st, d);
+            hipLaunchKernelGGL(k_rb_force<false>, chunks, dim3(RB_FT), 0, st, d);
     }
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
@@ -2185,7 +2221,7 @@ namespace dpm {
 // device bytes of one arena slot and of one vertex slot (arrays of batch_alloc)
 constexpr int64_t SLOT_BYTES = 32 + 32 + 16 + 16 + 16 + 8 + 4 + 4 + 32 * 4 + 8;
 constexpr int64_t RIGID_SLOT_BYTES = 4 + 4 + 8 * 5 + 16 * 4;
-constexpr int64_t VSLOT_BYTES = 4 + 4 + 4 + 32 * 4 + 16 * 3 + 8 + 8;
+constexpr int64_t VSLOT_BYTES = 4 + 4 + 4 + 32 * 4 + 16 * 2;
 
 static void serve_upload_tiles(dpm_server* sv) {
     dpm_batch* b = sv->b;

@@ -231,8 +231,12 @@ __device__ __forceinline__ bool within_v(double ax, double ay, double bx, double
 
 // kernelUpdateVertexNeighborList, rigid_bumpy/kernels.cu:870-922
 // gated: run only when the rebuild chain chose a list rebuild
-__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams nb, int gated) {
+// ring: the vertices sit on their rings (dpm_system::vring), so q's vertices are scanned
+// only when v lies within reach of q's centre (vertex_ring_near); the rows are unchanged
+__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams nb, double rv,
+                                                              int ring, int gated) {
     if (gated && d.ctl->kind < 1) return;
+    const double vc = sqrt(nb.vc2);
     int mx = 0;
     for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < d.Nv;
          v += (int64_t)gridDim.x * BLOCK) {
@@ -243,7 +247,10 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams n
         for (int k = 0; k < nn; ++k) {
             int q = d.nbr[(int64_t)k * d.N + p];
             double4 cq = d.P0[q];
-            if (within_v(xv.x, xv.y, cq.x, cq.y, nb, nb.vp2)) {
+            const double dx = pbc(xv.x, cq.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+            const double dy = pbc(xv.y, cq.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+            const double d2 = dx * dx + dy * dy;
+            if (d2 < nb.vp2 && (!ring || vertex_ring_near(d2, cq.z, rv, vc))) {
                 int f0 = d.vstart[q], nv = d.nverts[q];
                 for (int u = f0; u < f0 + nv; ++u) {
                     double2 xu = d.vpos[u];
@@ -840,7 +847,7 @@ void bumpy_enqueue_vertex_build(dpm_system* s, int gated) {
     // grid-stride: a gated launch that skips stays cheap
     dim3 g(std::min<unsigned>(pgrid(s->Nv).x, 2048u));
     hipLaunchKernelGGL(k_bumpy_vertex_build, g, dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
-                       gated);
+                       s->rv, (int)s->vring, gated);
 }
 
 // Vertex half of the cell reorder: new vertex starts from the permuted counts, then
@@ -898,6 +905,33 @@ int bumpy_stress(dpm_system* s) {
     hipLaunchKernelGGL(k_bumpy_stress, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, p, s->rv,
                        s->box[0] * s->box[1]);
     return check_launch();
+}
+
+// Whether every vertex lies at r - r_v from its particle's centre to a relative 1e-9
+// (vertex_ring_near's assumption): true after dpm_bumpy_setup, checked again on the host
+// after an upload of positions, radii or vertex positions.
+int bumpy_check_ring(dpm_system* s) {
+    std::vector<double4> P(s->N);
+    std::vector<int> st(s->N), nv(s->N);
+    std::vector<double2> vp(s->Nv);
+    if (scopy(s, P.data(), s->d.P0, sizeof(double4) * s->N, hipMemcpyDeviceToHost) ||
+        scopy(s, st.data(), s->d.vstart, sizeof(int) * s->N, hipMemcpyDeviceToHost) ||
+        scopy(s, nv.data(), s->d.nverts, sizeof(int) * s->N, hipMemcpyDeviceToHost) ||
+        scopy(s, vp.data(), s->d.vpos, sizeof(double2) * s->Nv, hipMemcpyDeviceToHost))
+        return -1;
+    auto img = [](double d, double L) { return d - L * std::round(d / L); };
+    bool ok = true;
+    for (int64_t p = 0; p < s->N && ok; ++p) {
+        const double inner = P[p].z - s->rv;
+        for (int k = 0; k < nv[p] && ok; ++k) {
+            const double2 u = vp[st[p] + k];
+            const double dx = img(u.x - P[p].x, s->box[0]), dy = img(u.y - P[p].y, s->box[1]);
+            ok = inner > 0 && std::fabs(std::sqrt(dx * dx + dy * dy) - inner) <= 1e-9 * inner;
+        }
+    }
+    s->vring = ok;
+    s->gen++;  // captured graphs bake the flag
+    return 0;
 }
 
 int bumpy_contacts(dpm_system* s) {
@@ -1149,7 +1183,9 @@ int dpm_bumpy_setup(dpm_system* s, const int64_t* nvp, double rv, int rotation) 
     hipLaunchKernelGGL(k_bumpy_init_vertices, pgrid(s->N), dim3(BLOCK), 0, s->stream, s->d, rv);
     if (check_launch()) return -1;
     DPM_HIP(hipStreamSynchronize(s->stream));
-    return refresh_rmax(s);
+    s->vring = rv > 0;  // rings of radius r - r_v (radii above r_v: checked below)
+    if (refresh_rmax(s)) return -1;
+    return bumpy_check_ring(s);
 }
 
 }  // extern "C"

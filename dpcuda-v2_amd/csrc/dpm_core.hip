@@ -2237,7 +2237,20 @@ int dpm_set_energy(dpm_system* s, double e_c, double n_c) {
     return 0;
 }
 
+static int upload_field(dpm_system* s, int f, const void* data, int64_t count);
+
 int dpm_upload(dpm_system* s, int f, const void* data, int64_t count) {
+    if (upload_field(s, f, data, count)) return -1;
+    // rigid bumpy: whether the vertices still sit on their rings (the vertex list build's
+    // scan gate assumes it)
+    if (s->kind == DPM_BUMPY && s->nv_max > 0 &&
+        (f == DPM_F_X || f == DPM_F_Y || f == DPM_F_RAD || f == DPM_F_VERTEX_X ||
+         f == DPM_F_VERTEX_Y))
+        return bumpy_check_ring(s);
+    return 0;
+}
+
+static int upload_field(dpm_system* s, int f, const void* data, int64_t count) {
     std::vector<int> sidx;
     if (fetch_sidx(s, sidx)) return -1;
     if (s->kind == DPM_BUMPY && ((f >= DPM_F_ANGLE && f <= DPM_F_MOI) || f >= DPM_F_VERTEX_X))

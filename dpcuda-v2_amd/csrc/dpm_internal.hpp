@@ -155,6 +155,7 @@ struct dpm_system {
     int64_t phys_cap = 0;
     int64_t vphys_cap = 0;
     int64_t nv_max = 0;         // most vertices of any particle
+    bool vring = false;         // every vertex at r - r_v from its centre (bumpy_check_ring)
     int64_t ncells_alloc = 0;
     bool nbr_ready = false;
     // rigid bumpy

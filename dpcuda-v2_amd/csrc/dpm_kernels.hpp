@@ -182,6 +182,7 @@ void enqueue_list_build(dpm_system* s, int gated);
 void reset_ctl(dpm_system* s, int config);
 int ensure_cells(dpm_system* s, int64_t nc);
 int update_verlet_grid(dpm_system* s);
+int bumpy_check_ring(dpm_system* s);
 void collect_timing(dpm_system* s);
 
 // Brackets one launch with HIP events on the system stream when timing is on.

@@ -62,6 +62,16 @@ __device__ __forceinline__ double2 ring_vertex(double2 c, double r, double a, in
     return make_double2(c.x + inner * cv, c.y + inner * sv);
 }
 
+// Whether a vertex v at squared distance d2 from the centre of particle q (radius r_q,
+// vertex radius rv) can have one of q's vertices within vc: every vertex of q lies at
+// r_q - rv from its centre (to rounding of the rotations, far below the margin), so by
+// the triangle inequality none can when d2 >= (r_q - rv + vc)^2; the test keeps a
+// relative margin of 1e-6 and so only skips vertex scans that would list nothing.
+__device__ __forceinline__ bool vertex_ring_near(double d2, double rq, double rv, double vc) {
+    const double reach = rq - rv + vc;
+    return d2 < reach * reach * (1.0 + 1e-6);
+}
+
 // The Fire::step moves of one rigid particle (src/integrator/fire.cu:20-40 with the
 // rigid overrides, rigid_bumpy/kernels.cu:21-150): [uphill: updatePositions(-dt/2) with
 // the decreased dt, zero v and omega] + updateVelocities(dt/2) + mix + updatePositions(dt).
