@@ -32,20 +32,19 @@
 //
 // One batched step is a fixed sequence of launches:
 //   k_b_prep_move  per slot: snapshot save/restore and box scaling requested by the
-//               controller, zero velocities (minimizeFire prologue); then the FIRE move
-//               of systems inside a minimisation (Fire::step first half) and its
-//               displacement flags
-//   k_b_decide  per system: Verlet rebuild when 3 max d^2 > thr^2
-//   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles
+//               controller (for a rigid particle with its vertices), zero velocities
+//               (minimizeFire prologue); then the FIRE move of systems inside a
+//               minimisation (Fire::step first half) and its displacement flags
+//   k_b_build   per (system, 256-slot tile): O(n^2) row build through LDS tiles, for the
+//               systems whose Verlet test (3 max d^2 > thr^2, b_rebuild_now) asks for it
 //   k_b_build_cells  systems of 128+ particles: the same rows from a cell grid in LDS
 //   k_b_force   forces (+ closing kick) and the chunk partials of P, PE, max F^2
-//   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision
+//   k_b_ctl     per system: minimizeFire's stop test and Fire::step's decision; clears
+//               the controller's applied requests and counts the step
 //   k_b_eval    after a minimisation: zero v, stress, contacts, r^2 partials and the
 //               displacement flags the next prologue tests
 //   k_b_jam     per system: the jamming controller (next minimisation, or done)
 // and for rigid systems (launched when the arena has vertex slots):
-//   k_rb_vprep  per vertex: the controller's snapshot / scaling on the vertices (before
-//               k_b_prep_move moves the centres they follow)
 //   k_rb_vmove  per vertex: the rotations of the step's rigid moves
 //   k_rb_vbuild per particle (a half-wave over its vertices): vertex rows from the
 //               particle rows (after the builds)
@@ -131,7 +130,8 @@ struct BDev {
     long long nctiles; // capacity of ctiles (the cell-grid build's grid size)
     int* flags;        // [0] capacity overflow, [1] largest row, [2] systems finished,
                        // [3] batched steps executed, [4] build tiles in use,
-                       // [5] cell-grid workgroups in use
+                       // [5] cell-grid workgroups in use, [6] largest vertex row,
+                       // [7] smallest system built from the cell grid
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double n_c, ncm1, two_nc, inv2nc;
@@ -242,6 +242,10 @@ __device__ __forceinline__ void b_prep_slot(const BDev& b, const BSys& S, long l
     const bool start = S.mode == BM_PROLOGUE;
     if (!start && S.snap_op == 0) return;  // wave-uniform: one system per wave
     if (i - S.off >= S.n) return;
+    if (rigid) {  // the particle's vertices first: they read the centre before it moves
+        const int st = b.vst[i], nv = b.nvt[i];
+        for (int k = 0; k < nv; ++k) rb_vertex_ops(b, S, st + k, (int)i);
+    }
     b_slot_ops(b, S, i, rigid);
     if (start) {
         b.vel[i] = make_double2(0.0, 0.0);  // minimizeFire: setVelocitiesToZero
@@ -330,18 +334,6 @@ __global__ __launch_bounds__(BLOCK) void k_b_prep_move(BDev b) {
     else b_move_slot(b, S, i);
 }
 
-// Vertex slots: the controller's operations, before k_b_prep_move moves the centres.
-__global__ __launch_bounds__(BLOCK) void k_rb_vprep(BDev b) {
-    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (v >= b.nvslot) return;
-    const int s = b.vsys_of[v];
-    if (s < 0) return;
-    const BSys& S = b.sys[s];
-    if (S.mode != BM_PROLOGUE && S.snap_op == 0) return;
-    const int p = b.vpart[v];
-    if (p >= 0) rb_vertex_ops(b, S, v, p);
-}
-
 // Vertex slots: the rotations of the step's rigid moves (k_bfire_vertices).
 __global__ __launch_bounds__(BLOCK) void k_rb_vmove(BDev b) {
     const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
@@ -359,26 +351,18 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vmove(BDev b) {
                                  r0.y, r1.x, r1.y, uphill);
 }
 
-// Particle::checkForVerletListUpdate (particle.cu:1141-1147) per system; also clears
-// the controller's requests that b_prep_slot has applied.  A restore step rebuilds
-// unconditionally (dpm_snapshot_restore).
-__global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
-    // one wave per system: the lanes test the system's chunk flags together
-    const int s = blockIdx.x, lane = threadIdx.x;
-    if (s == 0 && lane == 0) b.flags[3] += 1;  // one thread of the step counts it
-    BSys& S = b.sys[s];
-    const int mode = S.mode;
+// Whether system S rebuilds its rows in this step (called by a whole wave): the initial
+// build, a restore step (dpm_snapshot_restore's full rebuild), or
+// Particle::checkForVerletListUpdate (particle.cu:1141-1147) on the displacement flags of
+// the step (the move's, or at a prologue those of the last minimisation's end): some
+// 3 d^2 > thr^2.
+__device__ __forceinline__ bool b_rebuild_now(const BDev& b, const BSys& S) {
+    if (S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return false;
+    if (S.rebuild || S.mode == BM_RESTORE) return true;
+    const double* D = bpart(b, BP_DMAX) + S.off / WAVE;  // per wave: some 3 d^2 > thr
     bool far = false;
-    if (mode == BM_PROLOGUE || mode == BM_FIRE) {
-        const double* D = bpart(b, BP_DMAX) + S.off / WAVE;  // per wave: some 3 d^2 > thr
-        for (int c = lane; c < S.nc; c += WAVE) far = far || D[c] != 0.0;
-    }
-    const bool any = __builtin_amdgcn_ballot_w64(far) != 0;
-    if (lane == 0) {
-        S.snap_op = 0;
-        S.scale_f = 0.0;
-        if (any || mode == BM_RESTORE) S.rebuild = 1;
-    }
+    for (int c = threadIdx.x % WAVE; c < S.nc; c += WAVE) far = far || D[c] != 0.0;
+    return __builtin_amdgcn_ballot_w64(far) != 0;
 }
 
 // Systems of BC_MIN_N particles and more get their rows from a cell grid instead of the
@@ -387,7 +371,8 @@ __global__ __launch_bounds__(WAVE) void k_b_decide(BDev b) {
 // parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
 // so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
 // hold the same indices in the same ascending order as the all-pairs scan.
-constexpr int BC_MIN_N = 128;      // systems from this size up use the cell grid
+constexpr int BC_MIN_N = 128;      // systems from this size up may use the cell grid
+constexpr int BC_SMALL_N = 1025;   // ... and do when the batch's all-pairs work is large
 constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
 constexpr int BC_MAX_D = 96;       // cells per side at most
 constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
@@ -401,7 +386,7 @@ struct BCGrid {
 };
 
 __device__ __forceinline__ bool bc_grid(const BDev& b, const BSys& S, BCGrid& g) {
-    if (S.n < BC_MIN_N || S.n > BC_MAX_N) return false;
+    if (S.n < b.flags[7] || S.n > BC_MAX_N) return false;
     const double w = S.rc * (1.0 + 1e-9);
     g.nx = (int)fmin((double)BC_MAX_D, floor(S.Lx / w));
     g.ny = (int)fmin((double)BC_MAX_D, floor(S.Ly / w));
@@ -424,7 +409,7 @@ __device__ __forceinline__ int bc_coord(double x, double L, double cs, int n) {
 __device__ __forceinline__ void bc_build_item(const BDev& b, int2 ct, int* cend, int* lst,
                                               int* wsum) {
     const BSys& S = b.sys[ct.x];
-    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL || S.mode == BM_EMPTY) return;
+    if (!b_rebuild_now(b, S)) return;  // the same verdict in every wave of the workgroup
     BCGrid g;
     if (!bc_grid(b, S, g)) return;
     const int part = ct.y, parts = (S.n + BC_PER - 1) / BC_PER;
@@ -563,7 +548,7 @@ __global__ __launch_bounds__(BLOCK) void k_b_build(BDev b) {
     if ((int)blockIdx.x >= b.flags[4]) return;
     const int2 t = b.tiles[blockIdx.x];
     const BSys& S = b.sys[t.x];
-    if (!S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL) return;
+    if (!b_rebuild_now(b, S)) return;  // the same verdict in every wave of the block
     BCGrid g;
     if (bc_grid(b, S, g)) return;  // k_b_build_cells builds this system
     __shared__ double2 tp[BLOCK];
@@ -713,9 +698,7 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
     const BSys& S = b.sys[s];
-    if (S.kind != BK_RIGID || !S.rebuild || S.mode == BM_DONE || S.mode == BM_EVAL ||
-        S.mode == BM_EMPTY)
-        return;
+    if (S.kind != BK_RIGID || !b_rebuild_now(b, S)) return;
     const PairParams pp = bparams(b, S);
     const double vc = sqrt(S.vc2);
     int mx = 0;
@@ -765,8 +748,29 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
 // updateVelocities(dt/2) of a FIRE step (kernelUpdateRigidVelocities) and the chunk
 // partials of F.v, tau.w, PE and max(F^2, tau^2) (k_bfire_forces + k_bfire_partials'
 // arithmetic).  The per-vertex values never leave LDS.
-constexpr int RB_PIECE = 1024;
-constexpr int RB_FT = 512;  // threads per chunk: the vertex phase has ~25 vertices per slot
+constexpr int RB_FT = 1024;             // threads per chunk
+constexpr int RB_PIECE = 2 * RB_FT;     // vertices per LDS piece: two per thread
+
+// Force, torque and energy of vertex slot v (pair_law's path: the batch's pair law does not
+// allow the short sequences)
+__device__ __forceinline__ void rb_vforce_law(const BDev& b, const PairParams& p, double rv,
+                                              long long v, double& ax, double& ay,
+                                              double& e) {
+    const double2 xv = b.vpos[v];
+    const int nn = min(b.vnum[v], b.vcap);
+    ax = 0.0;
+    ay = 0.0;
+    e = 0.0;
+    for (int k = 0; k < nn; ++k) {
+        const double2 xu = b.vpos[b.vnbr[(long long)k * b.nvslot + v]];
+        double fx, fy;
+        const double ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+        ax += fx;
+        ay += fy;
+        e += ie;
+    }
+}
+
 template <bool FASTDIV>
 __global__ __launch_bounds__(RB_FT) void k_rb_force(BDev b) {
     const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * WAVE;  // chunk's slot
@@ -787,58 +791,73 @@ __global__ __launch_bounds__(RB_FT) void k_rb_force(BDev b) {
     const long long lo = b.vst[c0], hi = (long long)b.vst[end - 1] + b.nvt[end - 1];
     const double rv = S.rv;
     const int last = b.vcap - 1;
+    const long long row1 = (long long)min(1, last) * b.nvslot;
     double acc = 0.0;
     for (long long base = lo; base < hi; base += RB_PIECE) {
         const long long top = min(base + RB_PIECE, hi);
-        for (long long v = base + tid; v < top; v += RB_FT) {
-            const double2 xv = b.vpos[v];
-            const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
-            const int nn = min(b.vnum[v], b.vcap);
-            const int* col = b.vnbr + v;
-            double ax = 0.0, ay = 0.0, e = 0.0;
-            if constexpr (FASTDIV) {
-                // two neighbours per batch, the next batch's indices loaded ahead; every
-                // ELL slot holds a valid vertex slot (zero-filled at allocation, only
-                // written with slots), and a slot past the row end adds +0.0
-                int u0 = col[0], u1 = col[(long long)min(1, last) * b.nvslot];
-                for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
-                    const double2 x0 = b.vpos[u0], x1 = b.vpos[u1];
-                    if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
-                        u0 = col[(long long)min(k + 2, last) * b.nvslot];
-                        u1 = col[(long long)min(k + 3, last) * b.nvslot];
-                    }
-                    double fx, fy, ie;
-                    const double4 q0 = make_double4(x0.x, x0.y, rv, 0.0);
-                    const double4 q1 = make_double4(x1.x, x1.y, rv, 0.0);
-                    if (p.fast) ie = pair_hot<true>(qv, q0, k < nn, p, fx, fy);
-                    else ie = pair_hot<true, true>(qv, q0, k < nn, p, fx, fy);
-                    ax += fx;
-                    ay += fy;
-                    e += ie;
-                    if (p.fast) ie = pair_hot<true>(qv, q1, k + 1 < nn, p, fx, fy);
-                    else ie = pair_hot<true, true>(qv, q1, k + 1 < nn, p, fx, fy);
-                    ax += fx;
-                    ay += fy;
-                    e += ie;
+        // two vertices per thread, their rows walked together so that the loads of both
+        // are in flight at once (a lane without a vertex walks an empty row)
+        const long long va = base + tid, vb = va + RB_FT;
+        const bool oka = va < top, okb = vb < top;
+        double ra[3] = {0.0, 0.0, 0.0}, rb[3] = {0.0, 0.0, 0.0};  // ax, ay, e
+        if constexpr (FASTDIV) {
+            const long long sa = oka ? va : lo, sb = okb ? vb : sa;
+            const double2 xa = b.vpos[sa], xb = b.vpos[sb];
+            const int na = oka ? min(b.vnum[sa], b.vcap) : 0;
+            const int nb = okb ? min(b.vnum[sb], b.vcap) : 0;
+            const int nm = max(na, nb);
+            const int* ca = b.vnbr + sa;
+            const int* cb = b.vnbr + sb;
+            // every ELL slot holds a valid vertex slot (zero-filled at allocation, only
+            // written with slots), and a slot past the row end adds +0.0
+            int a0 = ca[0], a1 = ca[row1], b0 = cb[0], b1 = cb[row1];
+            const double4 qa = make_double4(xa.x, xa.y, rv, 0.0);
+            const double4 qb = make_double4(xb.x, xb.y, rv, 0.0);
+            for (int k = 0; __builtin_amdgcn_ballot_w64(k < nm) != 0; k += 2) {
+                const double2 ya0 = b.vpos[a0], ya1 = b.vpos[a1];
+                const double2 yb0 = b.vpos[b0], yb1 = b.vpos[b1];
+                if (__builtin_amdgcn_ballot_w64(k + 2 < nm) != 0) {
+                    const long long r2 = (long long)min(k + 2, last) * b.nvslot;
+                    const long long r3 = (long long)min(k + 3, last) * b.nvslot;
+                    a0 = ca[r2];
+                    a1 = ca[r3];
+                    b0 = cb[r2];
+                    b1 = cb[r3];
                 }
-            } else {
-                for (int k = 0; k < nn; ++k) {
-                    const double2 xu = b.vpos[col[(long long)k * b.nvslot]];
-                    double fx, fy;
-                    const double ie = pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
-                    ax += fx;
-                    ay += fy;
-                    e += ie;
+                const double2 ys[4] = {ya0, ya1, yb0, yb1};
+                const bool ok[4] = {k < na, k + 1 < na, k < nb, k + 1 < nb};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const double4 q = make_double4(ys[t].x, ys[t].y, rv, 0.0);
+                    const double4 me = t < 2 ? qa : qb;
+                    double fx, fy, ie;
+                    if (p.fast) ie = pair_hot<true>(me, q, ok[t], p, fx, fy);
+                    else ie = pair_hot<true, true>(me, q, ok[t], p, fx, fy);
+                    double* r = t < 2 ? ra : rb;
+                    r[0] += fx;
+                    r[1] += fy;
+                    r[2] += ie;
                 }
             }
+        } else {
+            if (oka) rb_vforce_law(b, p, rv, va, ra[0], ra[1], ra[2]);
+            if (okb) rb_vforce_law(b, p, rv, vb, rb[0], rb[1], rb[2]);
+        }
+        // calcTorque (include/particles/base/kernels.cuh:237-241) about the owner's centre
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const long long v = h ? vb : va;
+            if (!(h ? okb : oka)) continue;
+            const double* r = h ? rb : ra;
+            const double2 xv = b.vpos[v];
             const double4 cq = b.P[b.vpart[v]];
             const double dx = xv.x - cq.x, dy = xv.y - cq.y;
-            const double t = ay * dx - ax * dy;
+            const double t = r[1] * dx - r[0] * dy;
             const int l = (int)(v - base);
-            sv[0][l] = 0.0 + ax;
-            sv[1][l] = 0.0 + ay;
+            sv[0][l] = 0.0 + r[0];
+            sv[1][l] = 0.0 + r[1];
             sv[2][l] = 0.0 + t;
-            sv[3][l] = 0.0 + e;
+            sv[3][l] = 0.0 + r[2];
         }
         __syncthreads();
         if (sums) {  // this particle's vertices inside the piece, in vertex order
@@ -927,6 +946,12 @@ __device__ __forceinline__ void scale_box_of(BSys& S, double f) {
 __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
     BSys& S = b.sys[blockIdx.x];
     const int mode = S.mode;
+    if (threadIdx.x == 0) {
+        if (blockIdx.x == 0) b.flags[3] += 1;  // one thread of the step counts it
+        // the controller's requests of the last step are applied (k_b_prep_move)
+        S.snap_op = 0;
+        S.scale_f = 0.0;
+    }
     if (mode == BM_RESTORE) {  // the lists are current for the restored state: scaleBox
         if (threadIdx.x == 0) {
             scale_box_of(S, S.pend_f);
@@ -1340,6 +1365,12 @@ static void cell_tiles(std::vector<int2>& out, int s, int n) {
     for (int part = 0; part < (n + BC_PER - 1) / BC_PER; ++part) out.push_back(make_int2(s, part));
 }
 
+// The smallest system that gets its rows from the cell grid (flags[7]): every system of
+// BC_MIN_N particles and more when the batch's all-pairs work (sum of n^2) is large,
+// else only systems above 1,024 particles, whose all-pairs build takes longer than the
+// cell grid's fixed chain of phases.  Either build gives the same rows.
+static int cell_threshold(double sum_n2) { return sum_n2 > 3e8 ? BC_MIN_N : BC_SMALL_N; }
+
 // Copies and fills are ordered on the batch's own stream (never the legacy stream, which
 // would tie them to graph captures of other threads' systems).
 static hipError_t bsync(dpm_batch* b, hipError_t e) {
@@ -1520,10 +1551,8 @@ static void enqueue_batch_step(dpm_batch* b) {
     dim3 slots((unsigned)((b->nslot + BLOCK - 1) / BLOCK));
     const bool rigid = b->nvslot > 0;
     dim3 vslots((unsigned)((b->nvslot + BLOCK - 1) / BLOCK));
-    if (rigid) hipLaunchKernelGGL(k_rb_vprep, vslots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_prep_move, slots, dim3(BLOCK), 0, st, d);
     if (rigid) hipLaunchKernelGGL(k_rb_vmove, vslots, dim3(BLOCK), 0, st, d);
-    hipLaunchKernelGGL(k_b_decide, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_build, dim3((unsigned)b->ntiles), dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_build_cells, dim3((unsigned)std::min<long long>(d.nctiles, BC_GRID)),
                        dim3(BC_BLOCK), 0, st, d);
@@ -1973,8 +2002,12 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
                            b->stream));
     b->ran = true;
     DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
+    double sum_n2 = 0.0;
+    for (int64_t s = 0; s < b->nsys; ++s) sum_n2 += (double)b->n[s] * b->n[s];
     const int nt[2] = {(int)b->ntiles, b->nct};  // live until the synchronisation below
+    const int cth = cell_threshold(sum_n2);
     DPM_HIP(hipMemcpyAsync(d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    DPM_HIP(hipMemcpyAsync(d.flags + 7, &cth, sizeof(int), hipMemcpyHostToDevice, b->stream));
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (b->exec) {  // the parameters are baked into the graph's kernel arguments
         (void)hipGraphExecDestroy(b->exec);
@@ -2233,8 +2266,14 @@ static void serve_upload_tiles(dpm_server* sv) {
             sv->tiles_host.push_back(make_int2((int)s, (int)(q.off + t)));
     }
     std::vector<int2> ct;
+    double sum_n2 = 0.0;
     for (size_t s = 0; s < sv->sys_req.size(); ++s)
-        if (sv->sys_req[s]) cell_tiles(ct, (int)s, (int)sv->reqs.at(sv->sys_req[s]).n);
+        if (sv->sys_req[s]) {
+            const int64_t n = sv->reqs.at(sv->sys_req[s]).n;
+            cell_tiles(ct, (int)s, (int)n);
+            sum_n2 += (double)n * n;
+        }
+    const int cth = cell_threshold(sum_n2);
     // the bound of batch_alloc: ct never exceeds d.nctiles entries
     int nt[2] = {(int)sv->tiles_host.size(), (int)std::min<long long>(ct.size(), b->d.nctiles)};
     if (nt[0])
@@ -2244,6 +2283,8 @@ static void serve_upload_tiles(dpm_server* sv) {
         (void)bsync(b, hipMemcpyAsync(b->d.ctiles, ct.data(), sizeof(int2) * nt[1],
                                       hipMemcpyHostToDevice, b->stream));
     (void)bsync(b, hipMemcpyAsync(b->d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice,
+                                  b->stream));
+    (void)bsync(b, hipMemcpyAsync(b->d.flags + 7, &cth, sizeof(int), hipMemcpyHostToDevice,
                                   b->stream));
 }
 
