@@ -130,8 +130,7 @@ struct BDev {
     long long nctiles; // capacity of ctiles (the cell-grid build's grid size)
     int* flags;        // [0] capacity overflow, [1] largest row, [2] systems finished,
                        // [3] batched steps executed, [4] build tiles in use,
-                       // [5] cell-grid workgroups in use, [6] largest vertex row,
-                       // [7] smallest system built from the cell grid
+                       // [5] cell-grid workgroups in use, [6] largest vertex row
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double n_c, ncm1, two_nc, inv2nc;
@@ -371,8 +370,7 @@ __device__ __forceinline__ bool b_rebuild_now(const BDev& b, const BSys& S) {
 // parts-th block of 1024 particles).  Cells are at least rc (1 + 1e-9) wide,
 // so every particle within rc of i lies in the 3 x 3 cells around i's, and the rows
 // hold the same indices in the same ascending order as the all-pairs scan.
-constexpr int BC_MIN_N = 128;      // systems from this size up may use the cell grid
-constexpr int BC_SMALL_N = 1025;   // ... and do when the batch's all-pairs work is large
+constexpr int BC_MIN_N = 128;      // systems from this size up use the cell grid
 constexpr int BC_MAX_N = 16384;    // particles per system the LDS lists hold
 constexpr int BC_MAX_D = 96;       // cells per side at most
 constexpr int BC_CELLS = BC_MAX_D * BC_MAX_D;
@@ -386,7 +384,7 @@ struct BCGrid {
 };
 
 __device__ __forceinline__ bool bc_grid(const BDev& b, const BSys& S, BCGrid& g) {
-    if (S.n < b.flags[7] || S.n > BC_MAX_N) return false;
+    if (S.n < BC_MIN_N || S.n > BC_MAX_N) return false;
     const double w = S.rc * (1.0 + 1e-9);
     g.nx = (int)fmin((double)BC_MAX_D, floor(S.Lx / w));
     g.ny = (int)fmin((double)BC_MAX_D, floor(S.Ly / w));
@@ -748,8 +746,8 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
 // updateVelocities(dt/2) of a FIRE step (kernelUpdateRigidVelocities) and the chunk
 // partials of F.v, tau.w, PE and max(F^2, tau^2) (k_bfire_forces + k_bfire_partials'
 // arithmetic).  The per-vertex values never leave LDS.
-constexpr int RB_FT = 1024;             // threads per chunk
-constexpr int RB_PIECE = 2 * RB_FT;     // vertices per LDS piece: two per thread
+constexpr int RB_FT = 512;              // threads per chunk
+constexpr int RB_PIECE = 2 * RB_FT;     // vertices per LDS piece
 
 // Force, torque and energy of vertex slot v (pair_law's path: the batch's pair law does not
 // allow the short sequences)
@@ -795,69 +793,49 @@ __global__ __launch_bounds__(RB_FT) void k_rb_force(BDev b) {
     double acc = 0.0;
     for (long long base = lo; base < hi; base += RB_PIECE) {
         const long long top = min(base + RB_PIECE, hi);
-        // two vertices per thread, their rows walked together so that the loads of both
-        // are in flight at once (a lane without a vertex walks an empty row)
-        const long long va = base + tid, vb = va + RB_FT;
-        const bool oka = va < top, okb = vb < top;
-        double ra[3] = {0.0, 0.0, 0.0}, rb[3] = {0.0, 0.0, 0.0};  // ax, ay, e
-        if constexpr (FASTDIV) {
-            const long long sa = oka ? va : lo, sb = okb ? vb : sa;
-            const double2 xa = b.vpos[sa], xb = b.vpos[sb];
-            const int na = oka ? min(b.vnum[sa], b.vcap) : 0;
-            const int nb = okb ? min(b.vnum[sb], b.vcap) : 0;
-            const int nm = max(na, nb);
-            const int* ca = b.vnbr + sa;
-            const int* cb = b.vnbr + sb;
-            // every ELL slot holds a valid vertex slot (zero-filled at allocation, only
-            // written with slots), and a slot past the row end adds +0.0
-            int a0 = ca[0], a1 = ca[row1], b0 = cb[0], b1 = cb[row1];
-            const double4 qa = make_double4(xa.x, xa.y, rv, 0.0);
-            const double4 qb = make_double4(xb.x, xb.y, rv, 0.0);
-            for (int k = 0; __builtin_amdgcn_ballot_w64(k < nm) != 0; k += 2) {
-                const double2 ya0 = b.vpos[a0], ya1 = b.vpos[a1];
-                const double2 yb0 = b.vpos[b0], yb1 = b.vpos[b1];
-                if (__builtin_amdgcn_ballot_w64(k + 2 < nm) != 0) {
-                    const long long r2 = (long long)min(k + 2, last) * b.nvslot;
-                    const long long r3 = (long long)min(k + 3, last) * b.nvslot;
-                    a0 = ca[r2];
-                    a1 = ca[r3];
-                    b0 = cb[r2];
-                    b1 = cb[r3];
-                }
-                const double2 ys[4] = {ya0, ya1, yb0, yb1};
-                const bool ok[4] = {k < na, k + 1 < na, k < nb, k + 1 < nb};
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const double4 q = make_double4(ys[t].x, ys[t].y, rv, 0.0);
-                    const double4 me = t < 2 ? qa : qb;
-                    double fx, fy, ie;
-                    if (p.fast) ie = pair_hot<true>(me, q, ok[t], p, fx, fy);
-                    else ie = pair_hot<true, true>(me, q, ok[t], p, fx, fy);
-                    double* r = t < 2 ? ra : rb;
-                    r[0] += fx;
-                    r[1] += fy;
-                    r[2] += ie;
-                }
-            }
-        } else {
-            if (oka) rb_vforce_law(b, p, rv, va, ra[0], ra[1], ra[2]);
-            if (okb) rb_vforce_law(b, p, rv, vb, rb[0], rb[1], rb[2]);
-        }
-        // calcTorque (include/particles/base/kernels.cuh:237-241) about the owner's centre
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const long long v = h ? vb : va;
-            if (!(h ? okb : oka)) continue;
-            const double* r = h ? rb : ra;
+        for (long long v = base + tid; v < top; v += RB_FT) {
+            double ax = 0.0, ay = 0.0, e = 0.0;
             const double2 xv = b.vpos[v];
+            if constexpr (FASTDIV) {
+                const double4 qv = make_double4(xv.x, xv.y, rv, 0.0);
+                const int nn = min(b.vnum[v], b.vcap);
+                const int* col = b.vnbr + v;
+                // two neighbours per batch, the next batch's indices loaded ahead; every
+                // ELL slot holds a valid vertex slot (zero-filled at allocation, only
+                // written with slots), and a slot past the row end adds +0.0
+                int u0 = col[0], u1 = col[row1];
+                for (int k = 0; __builtin_amdgcn_ballot_w64(k < nn) != 0; k += 2) {
+                    const double2 x0 = b.vpos[u0], x1 = b.vpos[u1];
+                    if (__builtin_amdgcn_ballot_w64(k + 2 < nn) != 0) {
+                        u0 = col[(long long)min(k + 2, last) * b.nvslot];
+                        u1 = col[(long long)min(k + 3, last) * b.nvslot];
+                    }
+                    const double4 q0 = make_double4(x0.x, x0.y, rv, 0.0);
+                    const double4 q1 = make_double4(x1.x, x1.y, rv, 0.0);
+                    double fx, fy, ie;
+                    if (p.fast) ie = pair_hot<true>(qv, q0, k < nn, p, fx, fy);
+                    else ie = pair_hot<true, true>(qv, q0, k < nn, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                    e += ie;
+                    if (p.fast) ie = pair_hot<true>(qv, q1, k + 1 < nn, p, fx, fy);
+                    else ie = pair_hot<true, true>(qv, q1, k + 1 < nn, p, fx, fy);
+                    ax += fx;
+                    ay += fy;
+                    e += ie;
+                }
+            } else {
+                rb_vforce_law(b, p, rv, v, ax, ay, e);
+            }
+            // calcTorque (include/particles/base/kernels.cuh:237-241) about the centre
             const double4 cq = b.P[b.vpart[v]];
             const double dx = xv.x - cq.x, dy = xv.y - cq.y;
-            const double t = r[1] * dx - r[0] * dy;
+            const double t = ay * dx - ax * dy;
             const int l = (int)(v - base);
-            sv[0][l] = 0.0 + r[0];
-            sv[1][l] = 0.0 + r[1];
+            sv[0][l] = 0.0 + ax;
+            sv[1][l] = 0.0 + ay;
             sv[2][l] = 0.0 + t;
-            sv[3][l] = 0.0 + r[2];
+            sv[3][l] = 0.0 + e;
         }
         __syncthreads();
         if (sums) {  // this particle's vertices inside the piece, in vertex order
@@ -1364,12 +1342,6 @@ static void cell_tiles(std::vector<int2>& out, int s, int n) {
     if (n < BC_MIN_N || n > BC_MAX_N) return;
     for (int part = 0; part < (n + BC_PER - 1) / BC_PER; ++part) out.push_back(make_int2(s, part));
 }
-
-// The smallest system that gets its rows from the cell grid (flags[7]): every system of
-// BC_MIN_N particles and more when the batch's all-pairs work (sum of n^2) is large,
-// else only systems above 1,024 particles, whose all-pairs build takes longer than the
-// cell grid's fixed chain of phases.  Either build gives the same rows.
-static int cell_threshold(double sum_n2) { return sum_n2 > 3e8 ? BC_MIN_N : BC_SMALL_N; }
 
 // Copies and fills are ordered on the batch's own stream (never the legacy stream, which
 // would tie them to graph captures of other threads' systems).
@@ -2002,12 +1974,8 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
                            b->stream));
     b->ran = true;
     DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
-    double sum_n2 = 0.0;
-    for (int64_t s = 0; s < b->nsys; ++s) sum_n2 += (double)b->n[s] * b->n[s];
     const int nt[2] = {(int)b->ntiles, b->nct};  // live until the synchronisation below
-    const int cth = cell_threshold(sum_n2);
     DPM_HIP(hipMemcpyAsync(d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    DPM_HIP(hipMemcpyAsync(d.flags + 7, &cth, sizeof(int), hipMemcpyHostToDevice, b->stream));
     DPM_HIP(hipStreamSynchronize(b->stream));
     if (b->exec) {  // the parameters are baked into the graph's kernel arguments
         (void)hipGraphExecDestroy(b->exec);
@@ -2266,14 +2234,8 @@ static void serve_upload_tiles(dpm_server* sv) {
             sv->tiles_host.push_back(make_int2((int)s, (int)(q.off + t)));
     }
     std::vector<int2> ct;
-    double sum_n2 = 0.0;
     for (size_t s = 0; s < sv->sys_req.size(); ++s)
-        if (sv->sys_req[s]) {
-            const int64_t n = sv->reqs.at(sv->sys_req[s]).n;
-            cell_tiles(ct, (int)s, (int)n);
-            sum_n2 += (double)n * n;
-        }
-    const int cth = cell_threshold(sum_n2);
+        if (sv->sys_req[s]) cell_tiles(ct, (int)s, (int)sv->reqs.at(sv->sys_req[s]).n);
     // the bound of batch_alloc: ct never exceeds d.nctiles entries
     int nt[2] = {(int)sv->tiles_host.size(), (int)std::min<long long>(ct.size(), b->d.nctiles)};
     if (nt[0])
@@ -2283,8 +2245,6 @@ static void serve_upload_tiles(dpm_server* sv) {
         (void)bsync(b, hipMemcpyAsync(b->d.ctiles, ct.data(), sizeof(int2) * nt[1],
                                       hipMemcpyHostToDevice, b->stream));
     (void)bsync(b, hipMemcpyAsync(b->d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice,
-                                  b->stream));
-    (void)bsync(b, hipMemcpyAsync(b->d.flags + 7, &cth, sizeof(int), hipMemcpyHostToDevice,
                                   b->stream));
 }
 
