@@ -360,9 +360,11 @@ class System:
         self._c("bumpy_area", _ptr(a), self.n)
         return a
 
-    def neighbor_stats(self):
+    def neighbor_stats(self, totals=True):
+        """totals False skips the row-entry total (a download of every row length)."""
         v = [C.c_int64() for _ in range(4)]
-        self._c("neighbor_stats", *[C.byref(x) for x in v])
+        self._c("neighbor_stats", *[C.byref(x) if (k != 1 or totals) else None
+                                    for k, x in enumerate(v)])
         return dict(max_alloc=v[0].value, total_entries=v[1].value, n_rebuilds=v[2].value,
                     n_cell_rebuilds=v[3].value)
 

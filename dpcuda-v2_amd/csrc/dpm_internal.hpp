@@ -128,6 +128,7 @@ struct Graph {
     hipGraph_t graph = nullptr;
     uint64_t gen = ~0ull;
     double dt = 0.0;
+    int64_t chunk = 0;  // the set marker: the chunk length its graphs were captured for
 };
 
 }  // namespace dpm
