@@ -699,34 +699,62 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
     if (S.kind != BK_RIGID || !b_rebuild_now(b, S)) return;
     const PairParams pp = bparams(b, S);
     const double vc = sqrt(S.vc2);
+    const bool shift = fmin(pp.halfLx, pp.halfLy) > vertex_shift_limit(S.rmax, S.rv, vc);
+    const int hs = (int)(threadIdx.x % WAVE) / RB_HALF * RB_HALF;  // the half's first lane
+    const unsigned lt = (1u << lane) - 1u;  // the half's lanes below this one
     int mx = 0;
-    if (i - S.off < S.n) {
+    if (i - S.off < S.n) {  // uniform over the half-wave
         const int st = b.vst[i], nv = b.nvt[i];
         const int nn = min(b.nnum[i], b.cap);
-        for (int a = lane; a < nv; a += RB_HALF) {
-            const long long v = st + a;
-            const double2 xv = b.vpos[v];
+        for (int a0 = 0; a0 < nv; a0 += RB_HALF) {
+            // lane: vertex a0 + lane of this particle; the rows of the half's vertices grow
+            // together, q by q, each q's vertices tested by all 32 lanes at once for one
+            // gated vertex after another
+            const bool has = a0 + lane < nv;
+            const double2 xv = has ? b.vpos[st + a0 + lane] : make_double2(0.0, 0.0);
             int cnt = 0;
             for (int k = 0; k < nn; ++k) {
                 const int q = b.nbr[(long long)k * b.nslot + i];
                 const double4 cq = b.P[q];
-                double dx = pbc(xv.x, cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
-                double dy = pbc(xv.y, cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                const double kx = image_index(xv.x - cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+                const double ky = image_index(xv.y - cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                const double dx = pbc_known(xv.x, cq.x, pp.Lx, kx, pp.fast);  // = pbc(xv, cq)
+                const double dy = pbc_known(xv.y, cq.y, pp.Ly, ky, pp.fast);
                 const double d2 = dx * dx + dy * dy;
-                if (!(d2 < S.vp2) || !vertex_ring_near(d2, cq.z, S.rv, vc)) continue;
+                const bool gate = has && d2 < S.vp2 && vertex_ring_near(d2, cq.z, S.rv, vc);
+                unsigned gm = (unsigned)(__builtin_amdgcn_ballot_w64(gate) >> hs);
+                if (gm == 0) continue;
                 const int f0 = b.vst[q], nq = b.nvt[q];
-                for (int u = f0; u < f0 + nq; ++u) {
-                    const double2 xu = b.vpos[u];
-                    dx = pbc(xv.x, xu.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
-                    dy = pbc(xv.y, xu.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
-                    if (dx * dx + dy * dy < S.vc2) {
-                        if (cnt < b.vcap) b.vnbr[(long long)cnt * b.nvslot + v] = u;
-                        cnt++;
+                for (int u0 = 0; u0 < nq; u0 += RB_HALF) {
+                    const bool uin = u0 + lane < nq;
+                    const int u = f0 + u0 + lane;
+                    const double2 xu = uin ? b.vpos[u] : make_double2(0.0, 0.0);
+                    for (unsigned m = gm; m != 0; m &= m - 1) {  // gated vertices, in order
+                        const int g = __builtin_ctz(m);
+                        const double gx = __shfl(xv.x, g, RB_HALF), gy = __shfl(xv.y, g, RB_HALF);
+                        double ex, ey;
+                        if (shift) {
+                            ex = pbc_known(gx, xu.x, pp.Lx, __shfl(kx, g, RB_HALF), pp.fast);
+                            ey = pbc_known(gy, xu.y, pp.Ly, __shfl(ky, g, RB_HALF), pp.fast);
+                        } else {
+                            ex = pbc(gx, xu.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
+                            ey = pbc(gy, xu.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
+                        }
+                        const bool hit = uin && ex * ex + ey * ey < S.vc2;
+                        const unsigned hm = (unsigned)(__builtin_amdgcn_ballot_w64(hit) >> hs);
+                        if (hm == 0) continue;
+                        // ascending u: lane order; vertex g's row continues at its count
+                        const int pos = __shfl(cnt, g, RB_HALF) + __builtin_popcount(hm & lt);
+                        if (hit && pos < b.vcap)
+                            b.vnbr[(long long)pos * b.nvslot + st + a0 + g] = u;
+                        if (lane == g) cnt += __builtin_popcount(hm);
                     }
                 }
             }
-            b.vnum[v] = cnt;
-            mx = max(mx, cnt);
+            if (has) {
+                b.vnum[st + a0 + lane] = cnt;
+                mx = max(mx, cnt);
+            }
         }
     }
     mx = wave_max_i(mx);

@@ -234,9 +234,12 @@ __device__ __forceinline__ bool within_v(double ax, double ay, double bx, double
 // ring: the vertices sit on their rings (dpm_system::vring), so q's vertices are scanned
 // only when v lies within reach of q's centre (vertex_ring_near); the rows are unchanged
 __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams nb, double rv,
-                                                              int ring, int gated) {
+                                                              double rmax, int ring,
+                                                              int gated) {
     if (gated && d.ctl->kind < 1) return;
     const double vc = sqrt(nb.vc2);
+    // the centre's image serves q's vertices (pbc_known): needs the ring gate
+    const bool shift = ring && fmin(nb.halfLx, nb.halfLy) > vertex_shift_limit(rmax, rv, vc);
     int mx = 0;
     for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < d.Nv;
          v += (int64_t)gridDim.x * BLOCK) {
@@ -247,14 +250,24 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams n
         for (int k = 0; k < nn; ++k) {
             int q = d.nbr[(int64_t)k * d.N + p];
             double4 cq = d.P0[q];
-            const double dx = pbc(xv.x, cq.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
-            const double dy = pbc(xv.y, cq.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+            const double kx = image_index(xv.x - cq.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+            const double ky = image_index(xv.y - cq.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+            const double dx = pbc_known(xv.x, cq.x, nb.Lx, kx, nb.fast);  // = pbc(xv, cq)
+            const double dy = pbc_known(xv.y, cq.y, nb.Ly, ky, nb.fast);
             const double d2 = dx * dx + dy * dy;
             if (d2 < nb.vp2 && (!ring || vertex_ring_near(d2, cq.z, rv, vc))) {
                 int f0 = d.vstart[q], nv = d.nverts[q];
                 for (int u = f0; u < f0 + nv; ++u) {
                     double2 xu = d.vpos[u];
-                    if (within_v(xv.x, xv.y, xu.x, xu.y, nb, nb.vc2)) {
+                    bool in;
+                    if (shift) {
+                        const double ux = pbc_known(xv.x, xu.x, nb.Lx, kx, nb.fast);
+                        const double uy = pbc_known(xv.y, xu.y, nb.Ly, ky, nb.fast);
+                        in = ux * ux + uy * uy < nb.vc2;
+                    } else {
+                        in = within_v(xv.x, xv.y, xu.x, xu.y, nb, nb.vc2);
+                    }
+                    if (in) {
                         if (cnt < d.vcap) d.vnbr[(int64_t)cnt * d.Nv + v] = u;
                         cnt++;
                     }
@@ -847,7 +860,7 @@ void bumpy_enqueue_vertex_build(dpm_system* s, int gated) {
     // grid-stride: a gated launch that skips stays cheap
     dim3 g(std::min<unsigned>(pgrid(s->Nv).x, 2048u));
     hipLaunchKernelGGL(k_bumpy_vertex_build, g, dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
-                       s->rv, (int)s->vring, gated);
+                       s->rv, s->rmax, (int)s->vring, gated);
 }
 
 // Vertex half of the cell reorder: new vertex starts from the permuted counts, then

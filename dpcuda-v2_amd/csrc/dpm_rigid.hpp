@@ -72,6 +72,30 @@ __device__ __forceinline__ bool vertex_ring_near(double d2, double rq, double rv
     return d2 < reach * reach * (1.0 + 1e-6);
 }
 
+// Vertex rows through the centre's image.  For a vertex v and a particle q that passed
+// vertex_ring_near, every vertex u of q lies within ring reach + (r_q - r_v) of v; when
+// half the box exceeds that bound (vertex_shift_ok), the minimum image of v - u is the
+// image of v - x_q: pbc(v, u) = (v - u) - L k with the same k.  image_index gives that k
+// as pbc forms it (0 when |d| < L/2 on the fast path, where pbc returns d itself), and
+// pbc_known applies it with pbc's own operations, so the bits are pbc's.
+__device__ __forceinline__ double image_index(double d, double L, double invL, double halfL,
+                                              int fast) {
+    if (fast) return fabs(d) < halfL ? 0.0 : round(d * invL);
+    return round(d / L);
+}
+
+__device__ __forceinline__ double pbc_known(double a, double b, double L, double k, int fast) {
+    const double d = a - b;
+    if (fast && k == 0.0) return d;
+    return d - L * k;
+}
+
+// The half box a system needs for the image shortcut: ring reach plus ring radius of the
+// largest particle (vertex_ring_near's relative margin included).
+__device__ __host__ inline double vertex_shift_limit(double rmax, double rv, double vc) {
+    return (2.0 * rmax - 2.0 * rv + vc) * (1.0 + 1e-6);
+}
+
 // The Fire::step moves of one rigid particle (src/integrator/fire.cu:20-40 with the
 // rigid overrides, rigid_bumpy/kernels.cu:21-150): [uphill: updatePositions(-dt/2) with
 // the decreased dt, zero v and omega] + updateVelocities(dt/2) + mix + updatePositions(dt).
