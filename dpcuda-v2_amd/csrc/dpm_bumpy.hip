@@ -285,6 +285,80 @@ __global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_build(Dev d, NbrParams n
     }
 }
 
+// The same rows for vertices on their rings (dpm_system::vring): a half-wave per particle,
+// lanes over its vertices; q's vertices are tested against one gated vertex at a time by
+// all 32 lanes (q's row entry and centre are the same address for the half), and the hits
+// append to that vertex's row in ascending order (ballot ranks).  Grid-stride over the
+// particles.
+constexpr int VB_HALF = 32;
+__global__ __launch_bounds__(BLOCK) void k_bumpy_vertex_rows(Dev d, NbrParams nb, double rv,
+                                                             double rmax, int gated) {
+    if (gated && d.ctl->kind < 1) return;
+    const double vc = sqrt(nb.vc2);
+    const bool shift = fmin(nb.halfLx, nb.halfLy) > vertex_shift_limit(rmax, rv, vc);
+    const int lane = (int)(threadIdx.x % VB_HALF);
+    const int hs = (int)(threadIdx.x % WAVE) / VB_HALF * VB_HALF;
+    const unsigned lt = (1u << lane) - 1u;
+    const int64_t halves = (int64_t)gridDim.x * (BLOCK / VB_HALF);
+    int mx = 0;
+    for (int64_t p = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / VB_HALF; p < d.N;
+         p += halves) {
+        const int st = d.vstart[p], nv = d.nverts[p];
+        const int nn = min(d.nnum[p], d.cap);
+        for (int a0 = 0; a0 < nv; a0 += VB_HALF) {
+            const bool has = a0 + lane < nv;
+            const double2 xv = has ? d.vpos[st + a0 + lane] : make_double2(0.0, 0.0);
+            int cnt = 0;
+            for (int k = 0; k < nn; ++k) {
+                const int q = d.nbr[(int64_t)k * d.N + p];
+                const double4 cq = d.P0[q];
+                const double kx = image_index(xv.x - cq.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+                const double ky = image_index(xv.y - cq.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+                const double dx = pbc_known(xv.x, cq.x, nb.Lx, kx, nb.fast);  // = pbc(xv, cq)
+                const double dy = pbc_known(xv.y, cq.y, nb.Ly, ky, nb.fast);
+                const double d2 = dx * dx + dy * dy;
+                const bool gate = has && d2 < nb.vp2 && vertex_ring_near(d2, cq.z, rv, vc);
+                const unsigned gm = (unsigned)(__builtin_amdgcn_ballot_w64(gate) >> hs);
+                if (gm == 0) continue;
+                const int f0 = d.vstart[q], nq = d.nverts[q];
+                for (int u0 = 0; u0 < nq; u0 += VB_HALF) {
+                    const bool uin = u0 + lane < nq;
+                    const int u = f0 + u0 + lane;
+                    const double2 xu = uin ? d.vpos[u] : make_double2(0.0, 0.0);
+                    for (unsigned m = gm; m != 0; m &= m - 1) {
+                        const int g = __builtin_ctz(m);
+                        const double gx = __shfl(xv.x, g, VB_HALF), gy = __shfl(xv.y, g, VB_HALF);
+                        double ex, ey;
+                        if (shift) {
+                            ex = pbc_known(gx, xu.x, nb.Lx, __shfl(kx, g, VB_HALF), nb.fast);
+                            ey = pbc_known(gy, xu.y, nb.Ly, __shfl(ky, g, VB_HALF), nb.fast);
+                        } else {
+                            ex = pbc(gx, xu.x, nb.Lx, nb.invLx, nb.halfLx, nb.fast);
+                            ey = pbc(gy, xu.y, nb.Ly, nb.invLy, nb.halfLy, nb.fast);
+                        }
+                        const bool hit = uin && ex * ex + ey * ey < nb.vc2;
+                        const unsigned hm = (unsigned)(__builtin_amdgcn_ballot_w64(hit) >> hs);
+                        if (hm == 0) continue;
+                        const int pos = __shfl(cnt, g, VB_HALF) + __builtin_popcount(hm & lt);
+                        if (hit && pos < d.vcap)
+                            d.vnbr[(int64_t)pos * d.Nv + st + a0 + g] = u;
+                        if (lane == g) cnt += __builtin_popcount(hm);
+                    }
+                }
+            }
+            if (has) {
+                d.vnum[st + a0 + lane] = cnt;
+                mx = max(mx, cnt);
+            }
+        }
+    }
+    int m = block_max_int(mx);
+    if (threadIdx.x == 0) {
+        atomicMax(&d.ctl->max_vnum, m);
+        atomicMax(&d.ctl->max_vnum_all, m);
+    }
+}
+
 // exclusive scan of nverts (new order) into vstart for the vertex reorder
 __global__ __launch_bounds__(1024) void k_scan_counts(const Ctl* ctl, int gated, const int* cnt,
                                                       int* start, int64_t n) {
@@ -858,6 +932,12 @@ int bumpy_ensure_vcap(dpm_system* s, int64_t cap) {
 // particle list build; overflow is detected by the chain's end kernel.
 void bumpy_enqueue_vertex_build(dpm_system* s, int gated) {
     // grid-stride: a gated launch that skips stays cheap
+    if (s->vring) {
+        dim3 g(std::min<unsigned>(pgrid(s->N * VB_HALF).x, 2048u));
+        hipLaunchKernelGGL(k_bumpy_vertex_rows, g, dim3(BLOCK), 0, s->stream, s->d,
+                           nbr_params(s), s->rv, s->rmax, gated);
+        return;
+    }
     dim3 g(std::min<unsigned>(pgrid(s->Nv).x, 2048u));
     hipLaunchKernelGGL(k_bumpy_vertex_build, g, dim3(BLOCK), 0, s->stream, s->d, nbr_params(s),
                        s->rv, s->rmax, (int)s->vring, gated);
