@@ -31,6 +31,8 @@ plausible slip (a dropped term, a wrong sign or index, a transposed operand) fai
     kernels.cu:482-702)              central differences of the ring-pair energy
   jamming controller (runArthurJam-  hand-derived trace on a square lattice whose
     WithFire.cpp:64-106)             pressure P(L) is closed form
+  FIRE's force floor                 a rigid packing translated by 1e4 reaches the same
+                                     minimum but not |F|max < 1e-12
 """
 import math
 
@@ -806,3 +808,26 @@ def test_jam_controller_known_answer_on_a_square_lattice():
     assert 0.05 <= r.pressure <= 0.06
     assert r.pe == pytest.approx(2 * n * 0.5 * (1 - a) ** 2, rel=1e-9)
     assert r.packing_fraction == pytest.approx(math.pi * 0.25 / (a * a), rel=1e-12)
+
+
+def test_fire_force_floor_follows_coordinate_precision():
+    """Why FIRE of the 65,536-particle rigid packing stops near |F|max = 2.5e-12: the
+    forces cannot resolve below the precision of the coordinates.  The same 256-particle
+    rigid packing relaxes to |F|max < 1e-12 in a few thousand steps; translated by
+    10,000 (a periodic copy of the same system, coordinates ~200x coarser in the last
+    place) it reaches the same minimum -- the energy agrees to 1e-9 relative -- but its
+    largest force stays above 1e-11 however long FIRE runs, at the largest step."""
+    st = W.bumpy_config(256, 0.80, seed=12345)
+    prm = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=1e-12,
+                        max_steps=12000)
+    out = []
+    for shift in (0.0, 1e4):
+        st.x, st.y = st.x + shift, st.y + shift
+        s = W.make_bumpy_system(P.System, st, backend=oracle(), e_c=W.bumpy_contact_scale(st))
+        r = s.minimize_fire(prm)
+        out.append((r, s.reduce(P.RED_PE)))
+        st.x, st.y = st.x - shift, st.y - shift
+    (r0, e0), (r1, e1) = out
+    assert r0.converged and math.sqrt(r0.fmax_sq) < 1e-12
+    assert not r1.converged and math.sqrt(r1.fmax_sq) > 1e-11 and r1.dt == 0.1
+    assert e1 == pytest.approx(e0, rel=1e-9)
