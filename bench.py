@@ -504,9 +504,11 @@ def jam_batch(args, P, W):
     tuned = None
     if args.autotune:
         cut, chunk, ms = b.autotune(prm, trial_steps=args.autotune_steps)
-        tuned = {"cutoff": cut, "steps_per_launch": chunk, "trial_ms_per_step": ms}
-        print(f"[bench] autotune: cutoff {cut}, {chunk} steps per launch", file=sys.stderr,
-              flush=True)
+        blk, pct, ms = b.autotune_shape(prm, trial_steps=args.autotune_steps)
+        tuned = {"cutoff": cut, "steps_per_launch": chunk, "force_block": blk,
+                 "cell_pct": pct, "trial_ms_per_step": ms}
+        print(f"[bench] autotune: cutoff {cut}, {chunk} steps per launch, force block {blk},"
+              f" cells {pct}% of the cutoff", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     res, steps = b.jam(prm, args.batch_max_steps)
     elapsed = time.perf_counter() - t0

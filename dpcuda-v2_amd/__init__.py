@@ -47,6 +47,7 @@ PAIR_INT_FIELDS = {5, 6, 20}
  RED_STRESS_XX, RED_STRESS_XY, RED_STRESS_YX, RED_STRESS_YY, RED_VERTEX_CONTACTS) = range(16)
 
 OPT_GRAPH, OPT_CHUNK, OPT_TIMING = 0, 1, 2
+OPT_FORCE_BLOCK, OPT_CELL_PCT = 3, 4
 K_FORCE, K_DRIFT, K_REBUILD = 0, 1, 2
 
 
@@ -189,6 +190,8 @@ _GPU_ONLY = {
                            _P, _P, C.POINTER(_D)]),
     "batch_autotune": (_I, [_P, C.POINTER(JamParams), _L, _P, _I, _P, _I, C.POINTER(_D),
                            C.POINTER(_L), C.POINTER(_D)]),
+    "batch_autotune_shape": (_I, [_P, C.POINTER(JamParams), _L, C.POINTER(_L), C.POINTER(_L),
+                                  C.POINTER(_D)]),
 }
 
 
@@ -558,6 +561,13 @@ class Batch:
         self._c("batch_autotune", C.byref(params), int(trial_steps), _ptr(c), len(c), _ptr(k),
                 len(k), C.byref(bc), C.byref(bk), C.byref(ms))
         return bc.value, bk.value, ms.value
+
+    def autotune_shape(self, params: JamParams, trial_steps=200):
+        """Force-pass workgroup size and cell width (dpm_batch_autotune_shape)."""
+        blk, pct, ms = C.c_int64(), C.c_int64(), C.c_double()
+        self._c("batch_autotune_shape", C.byref(params), int(trial_steps), C.byref(blk),
+                C.byref(pct), C.byref(ms))
+        return blk.value, pct.value, ms.value
 
     def kernel_time(self):
         """(milliseconds, launches) of the timed force passes (DPM_OPT_TIMING)."""

@@ -134,6 +134,7 @@ struct BDev {
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double n_c, ncm1, two_nc, inv2nc;
+    double cell_scale;  // cell-grid cells: at least cell_scale * rc wide (>= 1)
     int fastdiv, pad;
     // rigid bumpy arena (nvslot = 0: none)
     long long nvslot;
@@ -385,7 +386,7 @@ struct BCGrid {
 
 __device__ __forceinline__ bool bc_grid(const BDev& b, const BSys& S, BCGrid& g) {
     if (S.n < BC_MIN_N || S.n > BC_MAX_N) return false;
-    const double w = S.rc * (1.0 + 1e-9);
+    const double w = S.rc * (1.0 + 1e-9) * b.cell_scale;
     g.nx = (int)fmin((double)BC_MAX_D, floor(S.Lx / w));
     g.ny = (int)fmin((double)BC_MAX_D, floor(S.Ly / w));
     if (g.nx < 3 || g.ny < 3) return false;
@@ -605,7 +606,9 @@ template <bool FASTDIV>
 __global__ __launch_bounds__(BLOCK) void k_b_force(BDev b) {
     // XCD-aware block order: a system's slots (and the neighbours they gather) stay in
     // one XCD's L2
-    const long long i = (long long)xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    // (any workgroup size that is a multiple of 64 up to BLOCK: the pass is per slot)
+    const long long i =
+        (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
@@ -1346,6 +1349,8 @@ struct dpm_batch {
     double2* V_init = nullptr;
     double rc = 1.7, thr_sq = 3.0 * 0.15 * 0.15, e_c = 1.0, n_c = 2.0;
     int64_t steps_per_graph = 32;
+    int force_block = 256;  // k_b_force workgroup size (64, 128, 256)
+    int cell_pct = 100;     // cell-grid build: cell width in % of the cutoff (>= 100)
     bool use_graph = true;
     bool ran = false;  // a dpm_batch_jam / dpm_batch_minimize has started
     int* hflags = nullptr;  // pinned: [0] overflow, [1] largest row, [2] systems done
@@ -1456,6 +1461,7 @@ static void batch_common(dpm_batch* b, bool fastdiv) {
     double m = frexp(d.two_nc, &ex);
     d.inv2nc = (m == 0.5) ? 1.0 / d.two_nc : 0.0;
     d.fastdiv = fastdiv ? 1 : 0;
+    d.cell_scale = b->cell_pct / 100.0;
 }
 
 // The short correctly rounded sqrt/division sequences apply (fastdiv_ok of dpm_system):
@@ -1565,10 +1571,12 @@ static void enqueue_batch_step(dpm_batch* b) {
         eb = batch_event(b);
         (void)hipEventRecord(ea, st);
     }
+    const int fb = b->force_block;
+    dim3 fslots((unsigned)((b->nslot + fb - 1) / fb));
     if (d.fastdiv)
-        hipLaunchKernelGGL(k_b_force<true>, slots, dim3(BLOCK), 0, st, d);
+        hipLaunchKernelGGL(k_b_force<true>, fslots, dim3(fb), 0, st, d);
     else
-        hipLaunchKernelGGL(k_b_force<false>, slots, dim3(BLOCK), 0, st, d);
+        hipLaunchKernelGGL(k_b_force<false>, fslots, dim3(fb), 0, st, d);
     if (b->timing) {
         (void)hipEventRecord(eb, st);
         b->ev_pending.push_back({ea, eb});
@@ -1583,6 +1591,14 @@ static void enqueue_batch_step(dpm_batch* b) {
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_jam, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
+}
+
+static void drop_graph(dpm_batch* b) {
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+    }
 }
 
 static int ensure_batch_graph(dpm_batch* b, int64_t k) {
@@ -1914,6 +1930,17 @@ int dpm_batch_set_option(dpm_batch* b, int option, int64_t value) {
         b->steps_per_graph = value;
         return 0;
     }
+    if (option == DPM_OPT_FORCE_BLOCK || option == DPM_OPT_CELL_PCT) {
+        const bool blk = option == DPM_OPT_FORCE_BLOCK;
+        if (blk ? !(value == 64 || value == 128 || value == 256) : !(value >= 100 && value <= 400)) {
+            set_error(blk ? "dpm_batch_set_option: force block must be 64, 128 or 256"
+                          : "dpm_batch_set_option: cell width must be 100-400 % of the cutoff");
+            return -1;
+        }
+        (blk ? b->force_block : b->cell_pct) = (int)value;
+        drop_graph(b);
+        return 0;
+    }
     if (option == DPM_OPT_TIMING) {
         batch_collect_timing(b);
         b->timing = value != 0;
@@ -2085,6 +2112,63 @@ int dpm_batch_minimize(dpm_batch* b, const dpm_fire_params* F, int64_t max_batch
     return batch_run(b, &P, true, max_batched_steps, out, steps_done);
 }
 
+}  // extern "C"
+
+namespace dpm {
+
+// The state as set (positions, and the rigid systems' angles and vertices) saved to, or
+// restored from, the autotuner's copies.
+static hipError_t tune_state(dpm_batch* b, bool save) {
+    auto cp = [&](void* dst, void* src, size_t n) {
+        return save ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream)
+                    : hipMemcpyAsync(src, dst, n, hipMemcpyDeviceToDevice, b->stream);
+    };
+    hipError_t e = cp(b->P_init, b->d.P, sizeof(double4) * b->nslot);
+    if (e == hipSuccess && b->nvslot > 0) e = cp(b->A_init, b->d.ang, sizeof(double) * b->nslot);
+    if (e == hipSuccess && b->nvslot > 0)
+        e = cp(b->V_init, b->d.vpos, sizeof(double2) * b->nvslot);
+    return e;
+}
+
+static int tune_begin(dpm_batch* b) {
+    if (!b->P_init && balloc_n(b, &b->P_init, b->nslot)) return -1;
+    if (b->nvslot > 0 && !b->A_init &&
+        (balloc_n(b, &b->A_init, b->nslot) || balloc_n(b, &b->V_init, b->nvslot)))
+        return -1;
+    DPM_HIP(tune_state(b, true));
+    return 0;
+}
+
+// One trial with the batch's current settings: an untimed launch unit from the state as
+// set (graph capture and instantiation), then trial_steps timed batched steps from it
+// again; ms = wall milliseconds per batched step.
+static int tune_trial(dpm_batch* b, const dpm_jam_params* P, int64_t trial_steps, double* ms) {
+    std::vector<dpm_jam_result> tmp(b->nsys);
+    drop_graph(b);
+    DPM_HIP(tune_state(b, false));
+    int64_t done = 0;
+    if (batch_run(b, P, false, b->steps_per_graph, tmp.data(), &done)) return -1;
+    DPM_HIP(tune_state(b, false));
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    if (batch_run(b, P, false, trial_steps, tmp.data(), &done)) return -1;
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+              .count() /
+          (double)std::max<int64_t>(done, 1);
+    return 0;
+}
+
+static int tune_end(dpm_batch* b) {
+    drop_graph(b);
+    DPM_HIP(tune_state(b, false));
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+}  // namespace dpm
+
+extern "C" {
+
 // Autotuner: runs `trial_steps` batched steps of the jam from the configured state for
 // every (cutoff, steps per launch) pair, restores the state after each trial, and keeps
 // the fastest pair.  Neither choice changes any result bit (rows only add listed
@@ -2097,23 +2181,17 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
         set_error("dpm_batch_autotune: bad arguments");
         return -1;
     }
-    if (!b->P_init && balloc_n(b, &b->P_init, b->nslot)) return -1;
-    if (b->nvslot > 0 && !b->A_init &&
-        (balloc_n(b, &b->A_init, b->nslot) || balloc_n(b, &b->V_init, b->nvslot)))
-        return -1;
-    // the state as set: positions, and the rigid systems' angles and vertices
-    auto state_copy = [&](bool save) -> hipError_t {
-        auto cp = [&](void* dst, void* src, size_t n) {
-            return save ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream)
-                        : hipMemcpyAsync(src, dst, n, hipMemcpyDeviceToDevice, b->stream);
-        };
-        hipError_t e = cp(b->P_init, b->d.P, sizeof(double4) * b->nslot);
-        if (e == hipSuccess && b->nvslot > 0) e = cp(b->A_init, b->d.ang, sizeof(double) * b->nslot);
-        if (e == hipSuccess && b->nvslot > 0)
-            e = cp(b->V_init, b->d.vpos, sizeof(double2) * b->nvslot);
-        return e;
-    };
-    DPM_HIP(state_copy(true));
+    for (int i = 0; i < n_cutoffs; ++i)
+        if (!(cutoffs[i] > 0)) {
+            set_error("dpm_batch_autotune: bad candidate");
+            return -1;
+        }
+    for (int j = 0; j < n_chunks; ++j)
+        if (chunks[j] < 1 || chunks[j] > 1024) {
+            set_error("dpm_batch_autotune: bad candidate");
+            return -1;
+        }
+    if (tune_begin(b)) return -1;
     // the cutoff candidates apply to the disk systems (rigid systems keep their own)
     double rmax_disk = 0.0;
     for (int64_t s = 0; s < b->nsys; ++s)
@@ -2122,35 +2200,16 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
     const int64_t ch0 = b->steps_per_graph;
     double best = 1e300, bc = rc0;
     int64_t bk = ch0;
-    std::vector<dpm_jam_result> tmp(b->nsys);
     for (int i = 0; i < n_cutoffs; ++i) {
         for (int j = 0; j < n_chunks; ++j) {
-            if (!(cutoffs[i] > 0) || chunks[j] < 1 || chunks[j] > 1024) {
-                set_error("dpm_batch_autotune: bad candidate");
-                return -1;
-            }
             // rebuild when a particle moved half the skin (the threshold keeps its form)
             double skin = cutoffs[i] - 2.0 * rmax_disk;
             if (!(skin > 0)) continue;
             b->rc = cutoffs[i];
             b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
             b->steps_per_graph = chunks[j];
-            if (b->exec) {
-                (void)hipGraphExecDestroy(b->exec);
-                (void)hipGraphDestroy(b->graph);
-                b->exec = nullptr;
-            }
-            DPM_HIP(state_copy(false));
-            // one untimed launch unit first (graph capture and instantiation)
-            int64_t done = 0;
-            if (dpm_batch_jam(b, P, chunks[j], tmp.data(), &done)) return -1;
-            DPM_HIP(state_copy(false));
-            DPM_HIP(hipStreamSynchronize(b->stream));
-            auto t0 = std::chrono::steady_clock::now();
-            if (dpm_batch_jam(b, P, trial_steps, tmp.data(), &done)) return -1;
-            double ms = std::chrono::duration<double, std::milli>(
-                            std::chrono::steady_clock::now() - t0).count() /
-                        (double)std::max<int64_t>(done, 1);
+            double ms;
+            if (tune_trial(b, P, trial_steps, &ms)) return -1;
             if (ms < best) {
                 best = ms;
                 bc = cutoffs[i];
@@ -2169,15 +2228,50 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* P, int64_t trial_step
     b->rc = bc;
     b->thr_sq = 3.0 * (0.5 * skin) * (0.5 * skin);
     b->steps_per_graph = bk;
-    if (b->exec) {
-        (void)hipGraphExecDestroy(b->exec);
-        (void)hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-    }
-    DPM_HIP(state_copy(false));
-    DPM_HIP(hipStreamSynchronize(b->stream));
+    if (tune_end(b)) return -1;
     if (best_cutoff) *best_cutoff = bc;
     if (best_chunk) *best_chunk = bk;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return 0;
+}
+
+// Launch shape: the force pass's workgroup size (64, 128 or 256 slots) and the cell-grid
+// build's cell width (100, 125 or 150 % of the cutoff), tried one after the other with
+// the current cutoff and steps per launch.  Neither changes a result bit: the force pass
+// is per slot, and cells at least the cutoff wide give the same rows.
+int dpm_batch_autotune_shape(dpm_batch* b, const dpm_jam_params* P, int64_t trial_steps,
+                             int64_t* best_block, int64_t* best_cell_pct,
+                             double* best_ms_per_step) {
+    if (!P || trial_steps <= 0) {
+        set_error("dpm_batch_autotune_shape: bad arguments");
+        return -1;
+    }
+    if (tune_begin(b)) return -1;
+    double best = 1e300;
+    int bb = b->force_block, bp = b->cell_pct;
+    for (int blk : {64, 128, 256}) {
+        b->force_block = blk;
+        double ms;
+        if (tune_trial(b, P, trial_steps, &ms)) return -1;
+        if (ms < best) {
+            best = ms;
+            bb = blk;
+        }
+    }
+    b->force_block = bb;
+    for (int pct : {100, 125, 150}) {
+        b->cell_pct = pct;
+        double ms;
+        if (tune_trial(b, P, trial_steps, &ms)) return -1;
+        if (ms < best) {
+            best = ms;
+            bp = pct;
+        }
+    }
+    b->cell_pct = bp;
+    if (tune_end(b)) return -1;
+    if (best_block) *best_block = bb;
+    if (best_cell_pct) *best_cell_pct = bp;
     if (best_ms_per_step) *best_ms_per_step = best;
     return 0;
 }

@@ -294,6 +294,14 @@ int dpm_batch_autotune(dpm_batch* b, const dpm_jam_params* p, int64_t trial_step
                        const double* cutoffs, int n_cutoffs, const int64_t* chunks,
                        int n_chunks, double* best_cutoff, int64_t* best_chunk,
                        double* best_ms_per_step);
+/* Launch-shape autotuner: with the current cutoff and steps per launch, times trial_steps
+ * batched steps for each force-pass workgroup size (64, 128, 256), keeps the fastest, then
+ * for each cell width of the cell-grid rows (100, 125, 150 % of the cutoff), restoring the
+ * configured state after each trial; the choices stay set (DPM_OPT_FORCE_BLOCK,
+ * DPM_OPT_CELL_PCT). */
+int dpm_batch_autotune_shape(dpm_batch* b, const dpm_jam_params* p, int64_t trial_steps,
+                             int64_t* best_block, int64_t* best_cell_pct,
+                             double* best_ms_per_step);
 
 /* ---- serving loop (no reference counterpart) ---------------------------------------------
  * Continuous batching of jamming and minimisation requests.  dpm_server_submit queues a
@@ -361,6 +369,10 @@ int dpm_disk_forces_noalloc(const double* x, const double* y, const double* radi
  *   FIRE loops (chunk time is charged to DPM_K_FORCE, recovery chains to DPM_K_REBUILD).
  *   dpm_kernel_time returns the summed milliseconds and the count per kernel class. */
 enum { DPM_OPT_GRAPH = 0, DPM_OPT_CHUNK = 1, DPM_OPT_TIMING = 2 };
+/* batched engine only: the force pass's workgroup size (64, 128 or 256 slots; default 256)
+ * and the cell-grid build's cell width in percent of the cutoff (100-400; default 100).
+ * Neither changes a result bit. */
+enum { DPM_OPT_FORCE_BLOCK = 3, DPM_OPT_CELL_PCT = 4 };
 enum { DPM_K_FORCE = 0, DPM_K_DRIFT = 1, DPM_K_REBUILD = 2 };
 int dpm_set_option(dpm_system* s, int option, int64_t value);
 int dpm_kernel_time(dpm_system* s, int kernel_class, double* total_ms, int64_t* launches);

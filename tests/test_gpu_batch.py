@@ -201,8 +201,34 @@ def test_autotune_keeps_results():
     cut, chunk, ms = b.autotune(prm, trial_steps=64, cutoffs=(1.6, 1.9), chunks=(8, 32))
     say("autotune", cut, chunk, ms)
     assert cut in (1.6, 1.9) and chunk in (8, 32) and ms > 0
+    blk, pct, ms = b.autotune_shape(prm, trial_steps=64)
+    say("autotune_shape", blk, pct, ms)
+    assert blk in (64, 128, 256) and pct in (100, 125, 150) and ms > 0
     res, _ = b.jam(prm)
     b.close()
+    for k in range(len(sts)):
+        for f in FIELDS:
+            assert getattr(res[k], f) == getattr(ref[k], f), (k, f)
+
+
+@pytest.mark.parametrize("block,pct", [(64, 100), (128, 150), (256, 400)])
+def test_launch_shape_keeps_results(block, pct):
+    """The force pass's workgroup size and the cell width leave every result bit alone."""
+    sts = systems(SPEC[:4])
+    prm = params(max_steps=600)
+    ref, _, _ = batched(sts, prm)
+    b = P.Batch([st.n for st in sts])
+    b.config(1.0, 2.0, 1.7, 3.0 * 0.15 * 0.15)
+    for k, st in enumerate(sts):
+        b.set_system(k, st.x, st.y, st.rad, st.mass, st.side)
+    b.set_option(P.OPT_FORCE_BLOCK, block)
+    b.set_option(P.OPT_CELL_PCT, pct)
+    for opt, bad in ((P.OPT_FORCE_BLOCK, 96), (P.OPT_CELL_PCT, 99), (P.OPT_CELL_PCT, 401)):
+        with pytest.raises(P.DpmError):
+            b.set_option(opt, bad)
+    res, _ = b.jam(prm)
+    b.close()
+    say("shape", block, pct)
     for k in range(len(sts)):
         for f in FIELDS:
             assert getattr(res[k], f) == getattr(ref[k], f), (k, f)
