@@ -528,9 +528,10 @@ __global__ void k_bumpy_init_vertices(Dev d, double rv) {
 //   k_build_cell / k_build_verlet, k_bumpy_vertex_build   (gated on that decision)
 //   k_bfire_forces    an in-chunk rebuild that overflowed the capacity halts the loop;
 //                     else vertex forces, torques, energies (kernelCalcRigidBumpyForces1)
-//                     and their per-particle sums (kernelCalcRigidBumpyParticleForces1) +
-//                     kick(dt/2), one pass
-//   k_bfire_partials  chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2)
+//                     and their per-particle sums (kernelCalcRigidBumpyParticleForces1),
+//                     one pass
+//   k_bfire_partials  kick(dt/2), then the chunk partials of P = F.v + tau.w, PE and
+//                     max(F^2, tau^2)
 //   k_fire_ctl        stop test of the step and Fire::step's decision for the next
 __device__ __forceinline__ bool bfire_skip(const Dev& d, int gate) {
     const Ctl* c = d.ctl;
@@ -675,19 +676,15 @@ __device__ __forceinline__ int first_particle_from(const Dev& d, int64_t v) {
 template <int LDSV>
 constexpr int bf_threads() { return BF_TILE; }
 
-// Eight waves per SIMD (62 VGPRs, no spills): the closing kick loads its per-particle
-// values after the vertex pass instead of holding them through it (issued first they
-// cost ten registers: six waves unconstrained, or seven with two spilled registers,
-// whose scratch stores were most of the pass's HBM writes).
+// Eight waves per SIMD (no spills): the closing kick runs in k_bfire_partials, and the
+// vertex ranges of the per-particle sums are read before the vertex pass (two registers).
 // (the general pair law and the large, LDS-bound form keep the compiler's own budget)
 template <int LDSV, bool HOT>
 constexpr int bf_waves() { return LDSV == BF_LDS_SMALL && HOT ? 8 : 1; }
 
 template <int LDSV, bool HOT>
-__global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k_bfire_forces(Dev d, PairParams p,
-                                                                     double rv, int kick,
-                                                                     int gate, int rotation,
-                                                                     int check_pos) {
+__global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k_bfire_forces(
+    Dev d, PairParams p, double rv, int gate, int check_pos) {
     constexpr int NT = bf_threads<LDSV>();
     if (bfire_skip(d, gate)) return;
     if (check_pos >= 0) {  // k_bfire_check's test, made by every block (same words read)
@@ -713,8 +710,14 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k
     const int np = p1 - p0;
     const int64_t lo = d.vstart[p0];
     const int64_t hi = p1 < d.N ? (int64_t)d.vstart[p1] : d.Nv;
-    // the per-particle phase below: np <= BF_TILE <= NT, one particle per lane
+    // the per-particle phase below: np <= BF_TILE <= NT, one particle per lane; the
+    // first of its work items reads its particle's vertex range now
     const bool own = (int)threadIdx.x < np;
+    int st0 = 0, nv0 = 0;
+    if ((int)threadIdx.x < 4 * np) {
+        st0 = d.vstart[p0 + (threadIdx.x >> 2)];
+        nv0 = d.nverts[p0 + (threadIdx.x >> 2)];
+    }
     for (int64_t v = lo + threadIdx.x; v < hi; v += NT) {
         const double4 cq = d.P0[d.vpart[v]];
         const int nn = min(d.vnum[v], d.vcap);
@@ -772,8 +775,9 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k
     for (int j = threadIdx.x; j < 4 * np; j += NT) {
         int q = j >> 2, comp = j & 3;
         int i = p0 + q;
-        int64_t st = d.vstart[i] - lo;
-        int nv = d.nverts[i];
+        const bool first = j == (int)threadIdx.x;
+        int64_t st = (first ? st0 : d.vstart[i]) - lo;
+        int nv = first ? nv0 : d.nverts[i];
         const double* src = sv[comp] + st;
         double acc = 0.0;
 #pragma unroll 8
@@ -781,30 +785,21 @@ __global__ __launch_bounds__(bf_threads<LDSV>(), (bf_waves<LDSV, HOT>())) void k
         sres[comp][q] = 0.0 + acc;
     }
     __syncthreads();
-    const double hdt = d.fire->dt / 2.0;
     if (own) {
         const int q = threadIdx.x, i = p0 + q;
-        double fx = sres[0][q], fy = sres[1][q], t = sres[2][q];
-        d.frc[i] = make_double2(fx, fy);
-        d.torque[i] = t;
+        d.frc[i] = make_double2(sres[0][q], sres[1][q]);
+        d.torque[i] = sres[2][q];
         d.pe[i] = sres[3][q];
-        if (kick) {
-            double2 v = d.vel[i];
-            double w = d.omega[i];
-            if (rotation) w += t * hdt / d.moi[i];
-            else w = 0.0;
-            const double mq = d.P0[i].w;
-            v.x += fx * hdt / mq;
-            v.y += fy * hdt / mq;
-            d.vel[i] = v;
-            d.omega[i] = w;
-        }
     }
 }
 
-// Chunk partials of P = F.v + tau.w, PE and max(F^2, tau^2) over the particles, in the
-// canonical order (64-particle chunks) that k_fire_ctl finishes.
-__global__ __launch_bounds__(BLOCK) void k_bfire_partials(Dev d, int gate) {
+// The closing kick of k_bfire_forces' particles (kick: 0 = forces only, minimizeFire's
+// initial evaluation; 1 = + kick(dt/2)), then the chunk partials of P = F.v + tau.w, PE
+// and max(F^2, tau^2) over the particles, in the canonical order (64-particle chunks)
+// that k_fire_ctl finishes.  (The kick here rather than at the end of the force pass
+// takes a dependent load off the force pass's tail.)
+__global__ __launch_bounds__(BLOCK) void k_bfire_partials(Dev d, int gate, int kick,
+                                                          int rotation) {
     if (bfire_skip(d, gate)) return;
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double pxv = 0.0, pyv = 0.0, ptw = 0.0, pe = 0.0, f2 = 0.0;
@@ -812,6 +807,16 @@ __global__ __launch_bounds__(BLOCK) void k_bfire_partials(Dev d, int gate) {
         double2 f = d.frc[i], v = d.vel[i];
         double t = d.torque[i], w = d.omega[i];
         pe = d.pe[i];
+        if (kick) {
+            const double hdt = d.fire->dt / 2.0;
+            if (rotation) w += t * hdt / d.moi[i];
+            else w = 0.0;
+            const double mq = d.P0[i].w;
+            v.x += f.x * hdt / mq;
+            v.y += f.y * hdt / mq;
+            d.vel[i] = v;
+            d.omega[i] = w;
+        }
         pxv = f.x * v.x;
         pyv = f.y * v.y;
         ptw = t * w;
@@ -1055,9 +1060,9 @@ void bumpy_enqueue_fire_forces(dpm_system* s, int kick, int gate, int check_pos)
                               : k_bfire_forces<BF_LDS_SMALL, false>)
                        : (hot ? k_bfire_forces<BF_LDS, true> : k_bfire_forces<BF_LDS, false>);
         hipLaunchKernelGGL(k, dim3(nb), dim3(small ? bf_threads<BF_LDS_SMALL>() : bf_threads<BF_LDS>()),
-                           0, s->stream, s->d, p, s->rv, kick, gate, (int)s->rotation, check_pos);
+                           0, s->stream, s->d, p, s->rv, gate, check_pos);
         hipLaunchKernelGGL(k_bfire_partials, dim3((unsigned)s->nblk), dim3(BLOCK), 0, s->stream,
-                           s->d, gate);
+                           s->d, gate, kick, (int)s->rotation);
         return;
     }
     if (check_pos >= 0)
