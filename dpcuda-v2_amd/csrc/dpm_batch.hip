@@ -1080,58 +1080,59 @@ __device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const Pair
                                         long long i, double& sxx, double& syy, double& z) {
     b.vel[i] = make_double2(0.0, 0.0);
     b.omg[i] = 0.0;
-    const double4 qi = b.P[i];
     const int st = b.vst[i], nv = b.nvt[i];
     const double rv = S.rv;
-    const double vm = qi.w / static_cast<double>(nv);  // k_bumpy_init_vertices' vertex mass
-    const double vvx = 0.0, vvy = 0.0;                   // zero velocity and spin
-    double axx = 0.0, ayy = 0.0;
-    for (int a = 0; a < nv; ++a) {
-        const int v = st + a;
-        const double2 xv = b.vpos[v];
-        axx += vm * vvx * vvx;
-        ayy += vm * vvy * vvy;
-        const int nn = min(b.vnum[v], b.vcap);
-        for (int k = 0; k < nn; ++k) {
-            const int u = b.vnbr[(long long)k * b.nvslot + v];
-            if (b.vpart[u] == i) continue;
-            const double2 xu = b.vpos[u];
-            const double xd = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
-            const double yd = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
-            double fx, fy;
-            pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
-            axx += xd * fx / 2;
-            ayy += yd * fy / 2;
-        }
-    }
-    const double area = S.Lx * S.Ly;
-    sxx = axx / area;
-    syy = ayy / area;
-    // a neighbour q of i counts when some vertex of i touches some vertex of q; every
+    // one pass over the vertex rows for both the stress and the contacts.  The kinetic
+    // terms of the vertices are vm * 0 * 0 = +0.0 (zero velocity and spin), which leave
+    // the sums unchanged (a sum that starts at +0.0 never becomes -0.0), so they are not
+    // added; the pair terms go in the reference's order (vertices, then row entries).
+    // A neighbour q of i counts when some vertex of i touches some vertex of q; every
     // vertex-row entry belongs to a particle of i's row (the rows are built from it), so
-    // the touching entries mark row positions in a mask and the count is its population
+    // the touching entries mark row positions in a mask and the count is its population.
+    double axx = 0.0, ayy = 0.0;
     const double vrs = 2 * rv;
     const int nn = min(b.nnum[i], b.cap);
     unsigned long long seen = 0;  // row positions (the row capacity is at most 64)
+    long long lastq = -1;         // touching partners repeat along the ring: looked up once
     for (int a = 0; a < nv; ++a) {
         const int v = st + a;
         const double2 xv = b.vpos[v];
         const int vn = min(b.vnum[v], b.vcap);
         for (int k2 = 0; k2 < vn; ++k2) {
             const int u = b.vnbr[(long long)k2 * b.nvslot + v];
-            const double2 xu = b.vpos[u];
-            const double dx = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
-            const double dy = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
-            const double dist = sqrt(dx * dx + dy * dy);
-            if (!(dist < vrs)) continue;
             const int q = b.vpart[u];
-            for (int k = 0; k < nn; ++k)
-                if (b.nbr[(long long)k * b.nslot + i] == q) {
-                    seen |= 1ull << k;
+            const double2 xu = b.vpos[u];
+            const double xd = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+            const double yd = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
+            if (q != i) {
+                double fx, fy;
+                pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+                axx += xd * fx / 2;
+                ayy += yd * fy / 2;
+            }
+            const double dist = sqrt(xd * xd + yd * yd);
+            if (!(dist < vrs) || q == lastq) continue;
+            lastq = q;
+            // q's position in the row, four entries per probe (loads issued together)
+            for (int k = 0; k < nn; k += 4) {
+                int r[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    r[t] = k + t < nn ? b.nbr[(long long)(k + t) * b.nslot + i] : -1;
+                int hit = -1;
+#pragma unroll
+                for (int t = 3; t >= 0; --t)
+                    if (r[t] == q) hit = k + t;
+                if (hit >= 0) {
+                    seen |= 1ull << hit;
                     break;
                 }
+            }
         }
     }
+    const double area = S.Lx * S.Ly;
+    sxx = axx / area;
+    syy = ayy / area;
     const int nc = __popcll(seen);
     z = (double)nc;
 }
