@@ -1352,6 +1352,7 @@ struct dpm_batch {
     int64_t steps_per_graph = 32;
     int force_block = 256;  // k_b_force workgroup size (64, 128, 256)
     int cell_pct = 100;     // cell-grid build: cell width in % of the cutoff (>= 100)
+    bool disks = true;      // the arena can hold disk systems (the disk force pass runs)
     bool use_graph = true;
     bool ran = false;  // a dpm_batch_jam / dpm_batch_minimize has started
     int* hflags = nullptr;  // pinned: [0] overflow, [1] largest row, [2] systems done
@@ -1566,21 +1567,23 @@ static void enqueue_batch_step(dpm_batch* b) {
     if (rigid)
         hipLaunchKernelGGL(k_rb_vbuild, dim3((unsigned)(b->nslot * RB_HALF / BLOCK)), dim3(BLOCK),
                            0, st, d);
-    hipEvent_t ea = nullptr, eb = nullptr;
-    if (b->timing) {
-        ea = batch_event(b);
-        eb = batch_event(b);
-        (void)hipEventRecord(ea, st);
-    }
-    const int fb = b->force_block;
-    dim3 fslots((unsigned)((b->nslot + fb - 1) / fb));
-    if (d.fastdiv)
-        hipLaunchKernelGGL(k_b_force<true>, fslots, dim3(fb), 0, st, d);
-    else
-        hipLaunchKernelGGL(k_b_force<false>, fslots, dim3(fb), 0, st, d);
-    if (b->timing) {
-        (void)hipEventRecord(eb, st);
-        b->ev_pending.push_back({ea, eb});
+    if (b->disks) {  // a batch of rigid systems only leaves the disk force pass out
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (b->timing) {
+            ea = batch_event(b);
+            eb = batch_event(b);
+            (void)hipEventRecord(ea, st);
+        }
+        const int fb = b->force_block;
+        dim3 fslots((unsigned)((b->nslot + fb - 1) / fb));
+        if (d.fastdiv)
+            hipLaunchKernelGGL(k_b_force<true>, fslots, dim3(fb), 0, st, d);
+        else
+            hipLaunchKernelGGL(k_b_force<false>, fslots, dim3(fb), 0, st, d);
+        if (b->timing) {
+            (void)hipEventRecord(eb, st);
+            b->ev_pending.push_back({ea, eb});
+        }
     }
     if (rigid) {
         dim3 chunks((unsigned)(b->nslot / WAVE));
@@ -1665,6 +1668,8 @@ static dpm_batch* batch_create(int64_t n_systems, const int64_t* n_particles,
         b->voff[s] = vacc;
         vacc += b->nvs[s];
     }
+    b->disks = !n_vertices ||
+               std::any_of(b->nv.begin(), b->nv.end(), [](int64_t v) { return v == 0; });
     if (vacc >= (int64_t)1 << 31) {
         set_error("dpm_batch_create: vertex arena exceeds 2^31 slots");
         delete b;
