@@ -445,6 +445,41 @@ def batch_force_roofline(P, sts, steps):
             "row_mean": k, "particles": n}
 
 
+def batch_rigid_roofline(P, W, sts, steps):
+    """The batched rigid force pass (k_rb_force) timed with HIP events over `steps` batched
+    steps of a batch holding the rigid systems `sts` from their initial state (every
+    system inside a minimisation).  Algorithmic bytes: per vertex 16 (position) + 4
+    (owner) + 4 (row length) + 4 k_v (row; the neighbours' positions are the same array,
+    not counted again); per particle 32 (record: lever arm and mass) + 8 (vertex range) +
+    32 (force, torque, energy written) + 48 (velocity and spin read and written) + 8
+    (moment of inertia)."""
+    import numpy as np
+    b = P.Batch([st.n for st in sts], [st.nv for st in sts])
+    b.config(1.0, 2.0, 1.7, 3.0 * 0.15 * 0.15)
+    for k, st in enumerate(sts):
+        b.set_rigid_system(k, st.x, st.y, st.rad, st.mass, st.angle, st.nverts, st.rv,
+                           st.side, e_c=W.bumpy_contact_scale(st))
+    b.set_option(P.OPT_TIMING, 1)
+    prm = P.jam_params(pressure_target=1e-2, window_rel=0.5, strain_step=1e-2,
+                       fire=P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0,
+                                          fmax_target=1e-12, max_steps=10 * steps))
+    b.jam(prm, steps)
+    ms, launches = b.kernel_time()
+    ventries = b.vertex_row_entries()
+    b.close()
+    n = int(np.sum([st.n for st in sts]))
+    nv = int(np.sum([st.nv for st in sts]))
+    kv = ventries / nv
+    per = ms / max(launches, 1) * 1e-3
+    byts = nv * (24.0 + 4.0 * kv) + n * 128.0
+    gbs = byts / per / 1e9
+    return {"bound": "hbm", "kernel": "k_rb_force (batched rigid force pass)", "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": byts,
+            "kernel_us_per_launch": per * 1e6, "timed_launches": launches,
+            "vertex_row_mean": kv, "particles": n, "vertices": nv}
+
+
 def oracle_jam_sample(P, W, jobs, budget_s):
     """The CPU oracle jamming (system, parameters) jobs of the same workload one after
     another until the time budget is spent (the plain reference algorithm on the host)."""
@@ -545,10 +580,11 @@ def jam_batch(args, P, W):
         s.close()
     plain_jams_s = 1.0 / (sum(t_plain) / len(t_plain))
     value = jammed / elapsed
-    roof = None
-    if not rigid:
-        print("[bench] force-pass roofline (event-timed batched steps)", file=sys.stderr,
-              flush=True)
+    print("[bench] force-pass roofline (event-timed batched steps)", file=sys.stderr,
+          flush=True)
+    if rigid:
+        roof = batch_rigid_roofline(P, W, sts, args.roofline_steps)
+    else:
         roof = batch_force_roofline(P, sts, args.roofline_steps)
     cpu = None
     if not args.no_cpu_baseline:

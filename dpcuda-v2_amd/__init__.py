@@ -180,6 +180,7 @@ _GPU_ONLY = {
     "batch_minimize": (_I, [_P, C.POINTER(FireParams), _L, _P, C.POINTER(_L)]),
     "batch_kernel_time": (_I, [_P, C.POINTER(_D), C.POINTER(_L)]),
     "batch_row_entries": (_I, [_P, C.POINTER(_L)]),
+    "batch_vertex_row_entries": (_I, [_P, C.POINTER(_L)]),
     "server_release": (_I, [_P, _L]),
     "disk_forces_noalloc": (_I, [_P, _P, _P, _L, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     "server_create": (_P, [_L, _L, _L, _L]),
@@ -578,6 +579,11 @@ class Batch:
     def row_entries(self):
         t = C.c_int64()
         self._c("batch_row_entries", C.byref(t))
+        return t.value
+
+    def vertex_row_entries(self):
+        t = C.c_int64()
+        self._c("batch_vertex_row_entries", C.byref(t))
         return t.value
 
     def finish_steps(self):

@@ -1590,11 +1590,22 @@ static void enqueue_batch_step(dpm_batch* b) {
         }
     }
     if (rigid) {
+        // DPM_OPT_TIMING times this pass instead when the batch holds rigid systems only
+        hipEvent_t ra = nullptr, rb = nullptr;
+        if (b->timing && !b->disks) {
+            ra = batch_event(b);
+            rb = batch_event(b);
+            (void)hipEventRecord(ra, st);
+        }
         dim3 chunks((unsigned)(b->nslot / WAVE));
         if (d.fastdiv)
             hipLaunchKernelGGL(k_rb_force<true>, chunks, dim3(RB_FT), 0, st, d);
         else
             hipLaunchKernelGGL(k_rb_force<false>, chunks, dim3(RB_FT), 0, st, d);
+        if (ra) {
+            (void)hipEventRecord(rb, st);
+            b->ev_pending.push_back({ra, rb});
+        }
     }
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
@@ -1966,6 +1977,24 @@ int dpm_batch_kernel_time(dpm_batch* b, double* force_ms, int64_t* launches) {
     batch_collect_timing(b);
     if (force_ms) *force_ms = b->force_ms;
     if (launches) *launches = b->force_launches;
+    return 0;
+}
+
+int dpm_batch_vertex_row_entries(dpm_batch* b, int64_t* total) {
+    if (!total) {
+        set_error("dpm_batch_vertex_row_entries: null output");
+        return -1;
+    }
+    DPM_HIP(hipStreamSynchronize(b->stream));
+    int64_t t = 0;
+    if (b->nvslot > 0) {
+        std::vector<int> nn(b->nvslot), own(b->nvslot);
+        BCOPY(nn.data(), b->d.vnum, sizeof(int) * b->nvslot, hipMemcpyDeviceToHost);
+        BCOPY(own.data(), b->d.vpart, sizeof(int) * b->nvslot, hipMemcpyDeviceToHost);
+        for (int64_t k = 0; k < b->nvslot; ++k)
+            if (own[k] >= 0) t += std::min(nn[k], b->d.vcap);
+    }
+    *total = t;
     return 0;
 }
 

@@ -281,11 +281,16 @@ int dpm_batch_minimize(dpm_batch* b, const dpm_fire_params* p, int64_t max_batch
  * it has not */
 int dpm_batch_finish_steps(dpm_batch* b, int64_t* steps_out);
 /* Measurement: with dpm_batch_set_option(b, DPM_OPT_TIMING, 1) every batched step is
- * launched eagerly with HIP events around its force pass; dpm_batch_kernel_time returns
+ * launched eagerly with HIP events around its force pass (the disk pass k_b_force, or the
+ * rigid pass k_rb_force in a batch of rigid systems only); dpm_batch_kernel_time returns
  * the summed milliseconds and the number of timed passes since the option was set.
  * dpm_batch_row_entries: the stored neighbour-row entries of every system's particles. */
 int dpm_batch_kernel_time(dpm_batch* b, double* force_ms, int64_t* launches);
 int dpm_batch_row_entries(dpm_batch* b, int64_t* total);
+/* The stored vertex-row entries of every rigid system's vertices (0 without vertex slots);
+ * no counterpart in the reference (the batched engine has none), used for the force pass's
+ * algorithmic bytes. */
+int dpm_batch_vertex_row_entries(dpm_batch* b, int64_t* total);
 /* Autotuner: times trial_steps batched steps of dpm_batch_jam for every pair of a Verlet
  * cutoff (rebuild threshold: half its skin) and a number of steps per graph launch,
  * restoring the configured state after each trial, and keeps the fastest pair.  Neither

@@ -179,3 +179,31 @@ def test_bad_rigid_set_up_fails_loudly():
         b.set_rigid_system(0, st.x, st.y, st.rad, st.mass, st.angle, st.nverts + 1, st.rv,
                            st.side)
     b.close()
+
+
+def test_rigid_force_pass_timing_and_vertex_rows():
+    """In a batch of rigid systems only, DPM_OPT_TIMING times the rigid force pass (a
+    timed launch per launched batched step; launches go by whole chunks, so at least one
+    per executed step; the same results), and the stored vertex-row entries equal the
+    listed vertex pairs of the same minimisation alone."""
+    st = W.bumpy_config(150, 0.76, seed=32)
+    fp = P.fire_params(avg_pe_target=0.0, avg_pe_diff_target=0.0, fmax_target=1e-9,
+                       max_steps=3000)
+    b, _ = mixed_batch([st])
+    b.set_option(P.OPT_TIMING, 1)
+    res, steps = b.minimize(fp)
+    ms, launches = b.kernel_time()
+    entries = b.vertex_row_entries()
+    got, _ = state_of(b, 0, True)
+    b.close()
+    say("timed rigid passes", launches, ms, "batched steps", steps, "vertex rows", entries)
+    assert launches >= steps > 0 and ms > 0
+    r, ref = plain_rigid(st, lambda s: s.minimize_fire(fp))
+    check_arrays(got, ref, 0)
+    s = W.make_bumpy_system(P.System, st, e_c=W.bumpy_contact_scale(st), method=P.NBR_VERLET)
+    s.minimize_fire(fp)
+    assert entries == len(s.vertex_pairs()[0])
+    s.close()
+    d, _ = mixed_batch([W.disk_config(64, 0.8, seed=2)])
+    assert d.vertex_row_entries() == 0
+    d.close()
