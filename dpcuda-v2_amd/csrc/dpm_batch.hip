@@ -51,6 +51,8 @@
 //   k_rb_force  per 64-particle chunk: vertex forces, torques and energies from the vertex
 //               rows into LDS, their sums over each particle's vertices in vertex order,
 //               the closing kick and the chunk partials
+//   k_rb_eval   per particle (a half-wave over its vertices), before k_b_eval: the
+//               stress and contacts of a rigid system that finished a minimisation
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -143,6 +145,8 @@ struct BDev {
     double2 *vpos, *vsnap;
     double *ang, *asnap, *omg, *tq, *moi;
     double2 *lpos, *mpos, *rot0, *rot1;
+    double2* evs;  // k_rb_eval's per-particle stress (xx, yy) for k_b_eval
+    int* evz;      // and contact count
 };
 
 __device__ __forceinline__ double* bpart(const BDev& b, int which) {
@@ -1141,6 +1145,113 @@ __device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const Pair
     z = (double)nc;
 }
 
+// The evaluation of rigid particles at the end of a minimisation (rb_eval's results)
+// with a half-wave per particle, lanes over its vertices: each lane walks its vertex row
+// (the gathers and pair laws of all the particle's vertices at once) and keeps the row's
+// nonzero stress terms, in row order, in LDS; then one lane adds them up in the order of
+// the serial walk, vertex after vertex.  Terms that are +-0.0 (pairs not in contact) are
+// left out: the running sums start at +0.0 and never become -0.0, so adding them changes
+// nothing.  The contact mask is the union of the lanes' masks.  A vertex with more than
+// RE_CAP nonzero terms sends its particle down the serial walk (rb_eval).
+constexpr int RE_CAP = 8;
+constexpr int RE_GRID = 1024;  // workgroups of the grid-stride launch (most slots skip)
+__device__ __forceinline__ void rb_eval_half(const BDev& b, long long t,
+                                             double2 (*sterm)[RE_CAP]) {
+    const long long i = t / RB_HALF;
+    const int lane = (int)(t % RB_HALF);
+    const int s = b.sys_of[i];
+    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
+    const BSys& S = b.sys[s];
+    if (S.kind != BK_RIGID || S.mode != BM_EVAL) return;
+    if (i - S.off >= S.n) return;  // uniform over the half-wave
+    const PairParams p = bparams(b, S);
+    const int st = b.vst[i], nv = b.nvt[i];
+    const double rv = S.rv, vrs = 2 * rv;
+    const int nn = min(b.nnum[i], b.cap);
+    const int hs = (int)(threadIdx.x % WAVE) / RB_HALF * RB_HALF;  // the half's first lane
+    double2* mine = sterm[threadIdx.x];
+    double axx = 0.0, ayy = 0.0;        // lane 0's running sums
+    unsigned long long seen = 0;         // this lane's touching partners' row positions
+    bool spill = false;
+    for (int a0 = 0; a0 < nv; a0 += RB_HALF) {
+        const int a = a0 + lane;
+        int cnt = 0;
+        if (a < nv) {
+            const int v = st + a;
+            const double2 xv = b.vpos[v];
+            const int vn = min(b.vnum[v], b.vcap);
+            long long lastq = -1;
+            for (int k2 = 0; k2 < vn; ++k2) {
+                const int u = b.vnbr[(long long)k2 * b.nvslot + v];
+                const int q = b.vpart[u];
+                const double2 xu = b.vpos[u];
+                const double xd = pbc(xv.x, xu.x, p.Lx, p.invLx, p.halfLx, p.fast);
+                const double yd = pbc(xv.y, xu.y, p.Ly, p.invLy, p.halfLy, p.fast);
+                if (q != i) {
+                    double fx, fy;
+                    pair_law(xv.x, xv.y, rv, xu.x, xu.y, rv, p, fx, fy);
+                    const double tx = xd * fx / 2, ty = yd * fy / 2;
+                    if (tx != 0.0 || ty != 0.0) {
+                        if (cnt < RE_CAP) mine[cnt] = make_double2(tx, ty);
+                        cnt++;
+                    }
+                }
+                const double dist = sqrt(xd * xd + yd * yd);
+                if (!(dist < vrs) || q == lastq) continue;
+                lastq = q;
+                for (int k = 0; k < nn; ++k)
+                    if (b.nbr[(long long)k * b.nslot + i] == q) {
+                        seen |= 1ull << k;
+                        break;
+                    }
+            }
+        }
+        spill = spill || cnt > RE_CAP;
+        // lane 0 adds the half's terms in vertex order (sterm is written by the lanes of
+        // this half only; a wave's LDS accesses complete in order)
+        __syncwarp();
+        for (int l = 0; l < RB_HALF && a0 + l < nv; ++l) {
+            const int cl = __shfl(cnt, l, RB_HALF);
+            if (lane == 0) {
+                const double2* src = sterm[threadIdx.x + l];
+                for (int k = 0; k < min(cl, RE_CAP); ++k) {
+                    axx += src[k].x;
+                    ayy += src[k].y;
+                }
+            }
+        }
+        __syncwarp();  // the next vertex group rewrites sterm
+    }
+    const bool any_spill = (__builtin_amdgcn_ballot_w64(spill) >> hs) & 0xffffffffull;
+    // the union of the lanes' contact masks
+#pragma unroll
+    for (int o = RB_HALF / 2; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)seen, o, RB_HALF);
+        const unsigned hi = __shfl_xor((unsigned)(seen >> 32), o, RB_HALF);
+        seen |= ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane != 0) return;
+    double sxx, syy, z;
+    if (any_spill) {
+        rb_eval(b, S, p, i, sxx, syy, z);
+    } else {
+        const double area = S.Lx * S.Ly;
+        sxx = axx / area;
+        syy = ayy / area;
+        z = (double)__popcll(seen);
+    }
+    b.evs[i] = make_double2(sxx, syy);
+    b.evz[i] = (int)z;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_eval(BDev b) {
+    __shared__ double2 sterm[BLOCK][RE_CAP];
+    const long long total = b.nslot * RB_HALF;  // a multiple of BLOCK (64-aligned slots)
+    for (long long t = (long long)blockIdx.x * BLOCK + threadIdx.x; t < total;
+         t += (long long)gridDim.x * BLOCK)
+        rb_eval_half(b, t, sterm);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= b.nslot) return;
@@ -1152,8 +1263,16 @@ __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     const bool live = i - S.off < S.n;
     double sxx = 0.0, syy = 0.0, r2 = 0.0, z = 0.0, d2 = 0.0;
     if (live) {
-        if (S.kind == BK_RIGID) rb_eval(b, S, p, i, sxx, syy, z);
-        else b_eval_disk(b, S, p, i, sxx, syy, z);
+        if (S.kind == BK_RIGID) {  // k_rb_eval's results
+            b.vel[i] = make_double2(0.0, 0.0);
+            b.omg[i] = 0.0;
+            const double2 ev = b.evs[i];
+            sxx = ev.x;
+            syy = ev.y;
+            z = (double)b.evz[i];
+        } else {
+            b_eval_disk(b, S, p, i, sxx, syy, z);
+        }
         const double4 q = b.P[i];
         r2 = q.z * q.z;
         // the closing displacement maxima of dpm_minimize_fire (k_disp_partials), which
@@ -1446,7 +1565,8 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
             balloc_n(b, &d.asnap, nslot) || balloc_n(b, &d.omg, nslot) ||
             balloc_n(b, &d.tq, nslot) || balloc_n(b, &d.moi, nslot) ||
             balloc_n(b, &d.lpos, nslot) || balloc_n(b, &d.mpos, nslot) ||
-            balloc_n(b, &d.rot0, nslot) || balloc_n(b, &d.rot1, nslot))
+            balloc_n(b, &d.rot0, nslot) || balloc_n(b, &d.rot1, nslot) ||
+            balloc_n(b, &d.evs, nslot) || balloc_n(b, &d.evz, nslot))
             return -1;
         DPM_HIP(bsync(b, hipMemsetAsync(d.vsys_of, 0xff, sizeof(int) * nvslot, b->stream)));
         DPM_HIP(bsync(b, hipMemsetAsync(d.vpart, 0xff, sizeof(int) * nvslot, b->stream)));
@@ -1608,6 +1728,11 @@ static void enqueue_batch_step(dpm_batch* b) {
         }
     }
     hipLaunchKernelGGL(k_b_ctl, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
+    if (rigid)
+        hipLaunchKernelGGL(k_rb_eval,
+                           dim3((unsigned)std::min<long long>(b->nslot * RB_HALF / BLOCK,
+                                                              RE_GRID)),
+                           dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_eval, slots, dim3(BLOCK), 0, st, d);
     hipLaunchKernelGGL(k_b_jam, dim3((unsigned)b->nsys), dim3(WAVE), 0, st, d);
 }
@@ -2382,7 +2507,7 @@ namespace dpm {
 
 // device bytes of one arena slot and of one vertex slot (arrays of batch_alloc)
 constexpr int64_t SLOT_BYTES = 32 + 32 + 16 + 16 + 16 + 8 + 4 + 4 + 32 * 4 + 8;
-constexpr int64_t RIGID_SLOT_BYTES = 4 + 4 + 8 * 5 + 16 * 4;
+constexpr int64_t RIGID_SLOT_BYTES = 4 + 4 + 8 * 5 + 16 * 4 + 16 + 4;
 constexpr int64_t VSLOT_BYTES = 4 + 4 + 4 + 32 * 4 + 16 * 2;
 
 static void serve_upload_tiles(dpm_server* sv) {
