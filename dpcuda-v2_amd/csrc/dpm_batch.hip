@@ -132,7 +132,9 @@ struct BDev {
     long long nctiles; // capacity of ctiles (the cell-grid build's grid size)
     int* flags;        // [0] capacity overflow, [1] largest row, [2] systems finished,
                        // [3] batched steps executed, [4] build tiles in use,
-                       // [5] cell-grid workgroups in use, [6] largest vertex row
+                       // [5] cell-grid workgroups in use, [6] largest vertex row,
+                       // [9] rigid systems entering their evaluation (evlist entries)
+    int* evlist;       // those systems, appended by k_b_ctl
     int* done_list;    // ids of finished systems, in order (ring of nsys entries)
     // common parameters
     double n_c, ncm1, two_nc, inv2nc;
@@ -957,6 +959,13 @@ __device__ __forceinline__ void scale_box_of(BSys& S, double f) {
     S.scale_f = f;
 }
 
+// the end of a minimisation: the system is evaluated in this step (k_rb_eval lists the
+// rigid ones)
+__device__ __forceinline__ void enter_eval(const BDev& b, BSys& S, int s) {
+    S.mode = BM_EVAL;
+    if (S.kind == BK_RIGID) b.evlist[atomicAdd(&b.flags[9], 1)] = s;
+}
+
 // one wave per system: minimizeFire's stop test (minimization.cu:96-105, with the
 // optional |F|max criterion) and Fire::step's decision for the next iteration
 // (fire.cu:14-33) -- k_fire_ctl's arithmetic on the system's chunk partials
@@ -999,7 +1008,7 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
         S.mode = BM_FIRE;
         if (S.J.max_steps <= 0) {
             S.done = 1;
-            S.mode = BM_EVAL;
+            enter_eval(b, S, blockIdx.x);
             return;
         }
     } else {
@@ -1010,12 +1019,12 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
         if (diff < S.J.pe_diff_t || avg < S.J.pe_t || fok) {
             S.converged = 1;
             S.done = 1;
-            S.mode = BM_EVAL;
+            enter_eval(b, S, blockIdx.x);
             return;
         }
         if (S.step >= S.J.max_steps) {
             S.done = 1;
-            S.mode = BM_EVAL;
+            enter_eval(b, S, blockIdx.x);
             return;
         }
     }
@@ -1038,7 +1047,7 @@ __global__ __launch_bounds__(WAVE) void k_b_ctl(BDev b) {
             S.stopped = 1;
             S.step += 1;
             S.done = 1;
-            S.mode = BM_EVAL;
+            enter_eval(b, S, blockIdx.x);
             return;
         }
         S.dt = fmax(S.dt * 0.5, S.dt_min);
@@ -1154,16 +1163,10 @@ __device__ __forceinline__ void rb_eval(const BDev& b, const BSys& S, const Pair
 // nothing.  The contact mask is the union of the lanes' masks.  A vertex with more than
 // RE_CAP nonzero terms sends its particle down the serial walk (rb_eval).
 constexpr int RE_CAP = 8;
-constexpr int RE_GRID = 1024;  // workgroups of the grid-stride launch (most slots skip)
-__device__ __forceinline__ void rb_eval_half(const BDev& b, long long t,
+constexpr int RE_GRID = 1024;  // workgroups of the launch (at most)
+__device__ __forceinline__ void rb_eval_half(const BDev& b, const BSys& S, long long i,
                                              double2 (*sterm)[RE_CAP]) {
-    const long long i = t / RB_HALF;
-    const int lane = (int)(t % RB_HALF);
-    const int s = b.sys_of[i];
-    if (s < 0) return;  // free arena slots (64-aligned: whole waves)
-    const BSys& S = b.sys[s];
-    if (S.kind != BK_RIGID || S.mode != BM_EVAL) return;
-    if (i - S.off >= S.n) return;  // uniform over the half-wave
+    const int lane = (int)(threadIdx.x % RB_HALF);
     const PairParams p = bparams(b, S);
     const int st = b.vst[i], nv = b.nvt[i];
     const double rv = S.rv, vrs = 2 * rv;
@@ -1244,16 +1247,23 @@ __device__ __forceinline__ void rb_eval_half(const BDev& b, long long t,
     b.evz[i] = (int)z;
 }
 
+// the systems k_b_ctl listed in this step, a half-wave per particle over the grid (a step
+// in which no rigid system finishes a minimisation returns at once)
 __global__ __launch_bounds__(BLOCK) void k_rb_eval(BDev b) {
     __shared__ double2 sterm[BLOCK][RE_CAP];
-    const long long total = b.nslot * RB_HALF;  // a multiple of BLOCK (64-aligned slots)
-    for (long long t = (long long)blockIdx.x * BLOCK + threadIdx.x; t < total;
-         t += (long long)gridDim.x * BLOCK)
-        rb_eval_half(b, t, sterm);
+    const int count = b.flags[9];
+    const int halves = BLOCK / RB_HALF;
+    const long long h0 = (long long)blockIdx.x * halves + threadIdx.x / RB_HALF;
+    for (int e = 0; e < count; ++e) {
+        const BSys& S = b.sys[b.evlist[e]];
+        for (long long h = h0; h < S.n; h += (long long)gridDim.x * halves)
+            rb_eval_half(b, S, S.off + h, sterm);  // uniform over the half-wave
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_b_eval(BDev b) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i == 0) b.flags[9] = 0;  // k_rb_eval has read this step's list
     if (i >= b.nslot) return;
     const int s = b.sys_of[i];
     if (s < 0) return;  // free arena slots (64-aligned: whole waves)
@@ -1549,7 +1559,7 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
         balloc_n(b, &d.lref, nslot) || balloc_n(b, &d.pe, nslot) || balloc_n(b, &d.nnum, nslot) ||
         balloc_n(b, &d.nbr, nslot * d.cap) || balloc_n(b, &d.part, (nslot / WAVE) * BP_COUNT) ||
         balloc_n(b, &d.tiles, tiles_cap) || balloc_n(b, &d.ctiles, d.nctiles) ||
-        balloc_n(b, &d.flags, 8) ||
+        balloc_n(b, &d.flags, 10) || balloc_n(b, &d.evlist, nsys) ||
         balloc_n(b, &d.done_list, nsys))
         return -1;
     DPM_HIP(bsync(b, hipMemsetAsync(d.sys_of, 0xff, sizeof(int) * nslot, b->stream)));  // -1: free
@@ -2192,7 +2202,7 @@ static int batch_run(dpm_batch* b, const dpm_jam_params* P, bool min_only,
     DPM_HIP(hipMemcpyAsync(d.sys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice,
                            b->stream));
     b->ran = true;
-    DPM_HIP(hipMemsetAsync(d.flags, 0, 8 * sizeof(int), b->stream));
+    DPM_HIP(hipMemsetAsync(d.flags, 0, 10 * sizeof(int), b->stream));
     const int nt[2] = {(int)b->ntiles, b->nct};  // live until the synchronisation below
     DPM_HIP(hipMemcpyAsync(d.flags + 4, nt, 2 * sizeof(int), hipMemcpyHostToDevice, b->stream));
     DPM_HIP(hipStreamSynchronize(b->stream));
