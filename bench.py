@@ -207,7 +207,7 @@ def rigid_fire(args, P, W):
 
 # ---------------------------------------------------------------- NVE with logging
 def nve_logged(args, P, W):
-    """Config-2 NVE run by the run_nve driver in the reference's run-directory form: the
+    """NVE of N=262,144 disks run by the run_nve driver in the reference's run-directory form: the
     FIRE-relaxed state is written here as a reference-format input directory
     (system/{init,restart}/*.dat in %.17e, particle and log configs), loaded by
     load_configs / loadParticle, and run with the reference's log groups (energy.csv every

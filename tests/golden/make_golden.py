@@ -1,6 +1,6 @@
 """Writes tests/golden/disk_n1024.npz: a 1,024-disk state (φ = 0.80) and the oracle's outputs on it.
 
-Config 1: N=1024 bidisperse disks (diameters 1.0/1.4, 50/50, large first), packing
+The state: N=1024 bidisperse disks (diameters 1.0/1.4, 50/50, large first), packing
 fraction 0.80, harmonic e_c=1, m=1, seeded uniform positions relaxed by exactly 200
 FIRE steps (alpha0=0.1, dt=0.01), Gaussian velocities at T=1e-4 (mean removed,
 rescaled), cell list with rc=1.7 and rebuild at max displacement > 0.15.

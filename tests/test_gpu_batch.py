@@ -103,7 +103,7 @@ def test_batched_minimization_equals_minimize_fire_alone():
     b.close()
 
 
-def test_config4_batched_jamming():
+def test_4096_uneven_systems_batched_jamming():
     """The batched-jamming configuration: 4,096 seeded systems, N log-uniform in
     [64, 2048], phi0 = 0.80, jammed in one batch with the relaxed protocol the benchmark
     states (strain 1e-2, FIRE to |F|max < 1e-6 in at most 3,000 steps, target pressure
@@ -117,14 +117,14 @@ def test_config4_batched_jamming():
                                           fmax_target=1e-6, max_steps=3000))
     res, pos, steps = batched(sts, prm)
     status = np.array([r.status for r in res])
-    say("config 4: batched steps", steps, "statuses", np.unique(status, return_counts=True))
+    say("4096 systems: batched steps", steps, "statuses", np.unique(status, return_counts=True))
     assert np.all(status == 0)
     for r in res:
         assert 1e-2 <= r.pressure <= 1.5e-2 and r.contacts > 0
     small = np.argsort([st.n for st in sts], kind="stable")[:16]
     for k in small:
         ro, ox, oy = plain(sts[k], prm, backend=oracle())
-        say("config 4 oracle sample", k, sts[k].n, ro.compression_steps, ro.fire_steps_total)
+        say("4096 systems: oracle sample", k, sts[k].n, ro.compression_steps, ro.fire_steps_total)
         same_result(res[k], ro, int(k))
         bx, by, side = pos[k]
         assert np.array_equal(bx, ox) and np.array_equal(by, oy), k

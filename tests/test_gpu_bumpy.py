@@ -217,7 +217,7 @@ def test_snapshot_restore():
         assert np.array_equal(g.download(getattr(P, f)), o.download(getattr(P, f))), f
 
 
-def test_config3_size_properties():
+def test_n65536_rigid_size_properties():
     n = 65536
     say("rigid benchmark size", n)
     st = W.bumpy_config(n, 0.8, seed=12345)

@@ -44,7 +44,7 @@ def forces(s):
 
 
 # ------------------------------------------------------------------ fixture
-def test_golden_fixture_config1():
+def test_golden_fixture_n1024():
     z = np.load(f"{ROOT}/tests/golden/disk_n1024.npz")
     say("golden N=1024 phi=0.80")
     st = W.DiskState(1024, float(z["side"]), z["x"], z["y"], z["rad"], z["mass"])
@@ -383,7 +383,7 @@ def test_snapshot_restore_and_scaling():
 
 
 # ------------------------------------------------------------------ full size
-def test_config2_size_properties():
+def test_n262144_size_properties():
     n = 262144
     say("benchmark size", n)
     st = W.disk_config(n, 0.85, seed=12345)

@@ -73,7 +73,7 @@ def test_disk_pairs_general_exponent():
             np.testing.assert_allclose(a[k], b[k], rtol=1e-13, atol=1e-300, err_msg=k)
 
 
-def test_disk_pairs_config2_size_properties():
+def test_disk_pairs_n262144_size_properties():
     """262,144 disks: full symmetric pair set, antisymmetric forces, H_ij = -H_ii exactly,
     fill values in unused slots, and per-particle sums of the slot forces equal to the
     force kernel's result (same row order, so bit-identical)."""

@@ -151,7 +151,7 @@ def submit_request(sv, q, prm, mode=P.REQ_JAM):
     return sv.submit(st.x, st.y, st.rad, st.mass, st.side, prm, mode=mode)
 
 
-def test_config5_poisson_mix_alone_and_in_the_mix():
+def test_poisson_70_30_mix_alone_and_in_the_mix():
     """The serving configuration at a reduced count: ten seeded Poisson arrivals, 70 %
     disks with N log-uniform in [128, 16384] and 30 % rigid bumpy particles with N
     log-uniform in [128, 4096], target pressure log-uniform in [1e-7, 1e-3] (relaxed FIRE
