@@ -715,6 +715,20 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
     if (i - S.off < S.n) {  // uniform over the half-wave
         const int st = b.vst[i], nv = b.nvt[i];
         const int nn = min(b.nnum[i], b.cap);
+        // the particle row, one entry per lane (nn <= cap = RB_HALF): the neighbour's
+        // record and vertex range, loaded at once and passed round by cross-lane reads
+        // instead of two dependent loads per entry
+        int rf = 0, rn = 0;
+        double rx = 0.0, ry = 0.0, rr = 0.0;
+        if (lane < nn) {
+            const int q = b.nbr[(long long)lane * b.nslot + i];
+            const double4 c = b.P[q];
+            rx = c.x;
+            ry = c.y;
+            rr = c.z;
+            rf = b.vst[q];
+            rn = b.nvt[q];
+        }
         for (int a0 = 0; a0 < nv; a0 += RB_HALF) {
             // lane: vertex a0 + lane of this particle; the rows of the half's vertices grow
             // together, q by q, each q's vertices tested by all 32 lanes at once for one
@@ -723,8 +737,9 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
             const double2 xv = has ? b.vpos[st + a0 + lane] : make_double2(0.0, 0.0);
             int cnt = 0;
             for (int k = 0; k < nn; ++k) {
-                const int q = b.nbr[(long long)k * b.nslot + i];
-                const double4 cq = b.P[q];
+                const double4 cq = make_double4(__shfl(rx, k, RB_HALF), __shfl(ry, k, RB_HALF),
+                                                __shfl(rr, k, RB_HALF), 0.0);
+                const int f0 = __shfl(rf, k, RB_HALF), nq = __shfl(rn, k, RB_HALF);
                 const double kx = image_index(xv.x - cq.x, pp.Lx, pp.invLx, pp.halfLx, pp.fast);
                 const double ky = image_index(xv.y - cq.y, pp.Ly, pp.invLy, pp.halfLy, pp.fast);
                 const double dx = pbc_known(xv.x, cq.x, pp.Lx, kx, pp.fast);  // = pbc(xv, cq)
@@ -733,7 +748,6 @@ __global__ __launch_bounds__(BLOCK) void k_rb_vbuild(BDev b) {
                 const bool gate = has && d2 < S.vp2 && vertex_ring_near(d2, cq.z, S.rv, vc);
                 unsigned gm = (unsigned)(__builtin_amdgcn_ballot_w64(gate) >> hs);
                 if (gm == 0) continue;
-                const int f0 = b.vst[q], nq = b.nvt[q];
                 for (int u0 = 0; u0 < nq; u0 += RB_HALF) {
                     const bool uin = u0 + lane < nq;
                     const int u = f0 + u0 + lane;
@@ -1552,7 +1566,8 @@ static int batch_alloc(dpm_batch* b, int64_t nsys, int64_t nslot, int64_t tiles_
     // parts = ceil(n / BC_PER) <= n / BC_PER + n / BC_MIN_N for n >= BC_MIN_N, and the
     // systems share the nslot slots
     d.nctiles = nslot / BC_PER + nslot / BC_MIN_N + 1;
-    d.cap = 32;  // at most 64: rb_eval marks row positions in a 64-bit mask
+    d.cap = 32;  // at most 32: k_rb_vbuild holds a row entry per half-wave lane (and
+                 // rb_eval marks row positions in a 64-bit mask)
     DPM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     if (balloc_n(b, &d.sys, nsys) || balloc_n(b, &d.sys_of, nslot) || balloc_n(b, &d.P, nslot) ||
         balloc_n(b, &d.snap, nslot) || balloc_n(b, &d.vel, nslot) || balloc_n(b, &d.frc, nslot) ||
